@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Stage-2 gamma-estimation benchmark on one MI355X.
+
+Metric: PS-candidate x interferogram x gamma-iteration per second of the
+stage-2 gamma loop (first fill .. final stop decision), with the patch's
+inputs already resident in HBM.  A step is one whole gamma loop over the
+patch (to convergence or gamma_max_iterations).
+
+Default workload: 500,000 candidates in 15x15 km, 60 interferograms, bperp
+U[-250, 250] m, 40 % coherent pixels, StaMPS defaults (50 m grid => 299x299
+cells, 2,209 CLAP windows per interferogram, 13 trials), 3 iterations max.
+The FP32 form is timed; the plain FP64 GPU form and the CPU restatement of the
+reference (the oracle in its reference-precision mode, all host cores) are
+timed beside it.
+
+    python bench.py [--steps K] [--warmup W] [--config 2] [--no-cpu] [--no-fp64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def clap_windows(n):
+    return max(0, math.ceil(n / 6) - 3)
+
+
+def kernel_work(prep, name):
+    """Algorithmic work per launch: (flops, bytes)."""
+    P, I, T = prep.n_ps, prep.n_ifg, len(prep.trial_values)
+    C = prep.n_i * prep.n_j
+    W = clap_windows(prep.n_i) * clap_windows(prep.n_j)
+    if name == "clap_strip":
+        # 2 FFT2(32) at 5 N^2 log2(N^2) + 7-tap separable smoothing + pointwise
+        return 143_360.0 * W * I, 16.0 * C * I
+    if name == "scan":
+        return 8.0 * P * I * T, 12.0 * P * I
+    if name == "refine":
+        return 24.0 * P * I, 12.0 * P * I
+    if name == "fill_grid":
+        return 16.0 * P * I, 12.0 * P * I + 8.0 * C * I
+    if name == "clap_combine":
+        return 0.0, 8.0 * C * I + 8.0 * C * I
+    return 0.0, 0.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 4, 5])
+    ap.add_argument("--n-ps", type=int, default=0, help="override the candidate count")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--nr-rows", type=int, default=300_000)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs stage 2 on one GPU; --gpus must be 1 "
+                                   "(multi-GPU patch sharding is a design note, not built)"}))
+        sys.exit(2)
+
+    import numpy as np
+    from pystamps_amd import synth
+    from pystamps_amd.prepare import prepare_stage2_inputs
+    from pystamps_amd.stage2 import Stage2Session
+
+    cfg = synth.CONFIGS[args.config]
+    if args.n_ps:
+        cfg = synth.scaled(cfg, args.n_ps)
+    t = time.perf_counter()
+    raw = synth.generate(cfg)
+    log(f"[bench] generated {cfg.name}: {cfg.n_ps} x {cfg.n_ifg} in {time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    prep = prepare_stage2_inputs(raw, nr_rows=args.nr_rows)
+    prep_s = time.perf_counter() - t
+    del raw
+    P, I = prep.n_ps, prep.n_ifg
+    log(f"[bench] prep (incl. GPU Nr, {args.nr_rows} rows) {prep_s:.2f}s grid {prep.n_i}x{prep.n_j} "
+        f"trials {len(prep.trial_values)}")
+
+    def time_form(form, steps, warmup):
+        t0 = time.perf_counter()
+        s = Stage2Session(prep, form=form)
+        setup = time.perf_counter() - t0
+        for _ in range(warmup):
+            s.run()
+        s.set_timing(True)
+        iters = 0
+        dev_s = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            i_loop, secs = s.run()
+            iters += i_loop
+            dev_s += secs
+        wall = time.perf_counter() - t0
+        kt = s.kernel_times()
+        out = s.download()
+        s.close()
+        log(f"[bench] {form}: {steps} loops, {iters} iterations, wall {wall * 1e3:.2f} ms, "
+            f"device {dev_s * 1e3:.2f} ms, setup {setup:.2f}s")
+        for name, (ms, n) in kt.items():
+            if n:
+                log(f"[bench]   {name:13s} {ms / n * 1e3:9.1f} us/launch x {n}")
+        return {"wall": wall, "iters": iters, "dev": dev_s, "kt": kt, "out": out, "setup": setup}
+
+    r32 = time_form("fp32", args.steps, args.warmup)
+    value = P * I * r32["iters"] / r32["wall"]
+    ms_per_step = r32["wall"] / args.steps * 1e3
+
+    # roofline of the dominant kernel (largest summed device time)
+    kt = r32["kt"]
+    dom = max((k for k in kt if kt[k][1] > 0), key=lambda k: kt[k][0])
+    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
+    flops, nbytes = kernel_work(prep, dom)
+    if flops / max(nbytes, 1.0) > PEAK_FP32_TFLOPS * 1e3 / PEAK_HBM_GBS:
+        roof = {"bound": "mfma", "engine": "fp32 VALU (no matrix op on this path)",
+                "achieved": flops / avg_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": nbytes / avg_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = dom
+    roof["avg_launch_us"] = avg_s * 1e6
+
+    result = {
+        "metric": "stage-2 PS·ifg·iter/s, FP32+batched vs FP64 GPU at matched K/gamma error",
+        "value": value,
+        "unit": "PS·ifg·iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, BASELINE config %d)" % args.config,
+        "config": {"workload": cfg.name, "n_ps": P, "n_ifg": I, "grid": [prep.n_i, prep.n_j],
+                   "clap_windows_per_ifg": clap_windows(prep.n_i) * clap_windows(prep.n_j),
+                   "n_trial": len(prep.trial_values), "iterations_per_step": r32["iters"] // args.steps,
+                   "form": "fp32 bulk + fp64 near-tie re-score"},
+        "roofline": roof,
+        "rescored_rows_per_loop": r32["out"]["n_rescored"],
+    }
+
+    if not args.no_fp64:
+        r64 = time_form("fp64", args.fp64_steps, 1)
+        v64 = P * I * r64["iters"] / r64["wall"]
+        o32, o64 = r32["out"], r64["out"]
+        agree = float(np.mean(o32["trial_ix"] == o64["trial_ix"]))
+        dcoh = np.abs(o32["coh_ps"] - o64["coh_ps"])
+        result["fp64_plain"] = {"value": v64, "ms_per_step": r64["wall"] / args.fp64_steps * 1e3,
+                                "fp32_speedup": value / v64,
+                                "trial_ix_agreement": agree,
+                                "coh_abs_err_max": float(dcoh.max()),
+                                "coh_abs_err_p999": float(np.quantile(dcoh, 0.999)),
+                                "i_loop_equal": bool(o32["i_loop"] == o64["i_loop"])}
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        ro = O.run(prep, mode=O.REF32, threads=threads, diagnostics=False)
+        cpu_s = time.perf_counter() - t0
+        cpu_v = P * I * ro["i_loop"] / cpu_s
+        result["cpu_baseline"] = {
+            "value": cpu_v, "unit": "PS·ifg·iter/s", "cores": threads, "kind": "port",
+            "sample": f"one whole gamma loop ({ro['i_loop']} iterations) of the same {P} x {I} "
+                      f"workload, oracle in the reference's precision map, OpenMP over rows/ifgs",
+            "seconds": cpu_s,
+            "cpu": _cpu_model(),
+        }
+        result["gpu_vs_cpu"] = value / cpu_v
+        log(f"[bench] cpu ref32: {cpu_s:.2f}s for {ro['i_loop']} iterations on {threads} threads")
+    print(json.dumps(result), flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

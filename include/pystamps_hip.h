@@ -1,0 +1,179 @@
+/*
+ * pystamps_hip — MI355X (gfx950) implementation of pySTAMPS stage 2, PS gamma
+ * estimation (the StaMPS ps_est_gamma_quick loop), behind a plain C ABI.
+ *
+ * The loop this library runs is run_stage2_native_inner's loop body
+ * (crates/pystamps-core/src/native_stage2.rs:242-406): fill_phase_weight,
+ * accumulate_grid, clap_filter_grid_stack, extract_patch_phase,
+ * fit_stage2_rows, the gamma-change RMS stop test and the P-square / SNR
+ * weighting update.  The caller keeps the reference's prep
+ * (prepare_stage2_inputs, native_stage2.rs:415-561) and its pm1.mat writer
+ * (write_pm1, native_stage2.rs:1899-1965); this library takes the prepared
+ * inputs and fills the write_pm1 arguments.
+ *
+ * Conventions
+ *  - All pointers passed in or out are HOST memory owned by the caller.  The
+ *    library owns its device buffers and stream.  Nothing allocated by the
+ *    library is returned to the caller except opaque session handles.
+ *  - Complex arrays are interleaved (re, im) pairs, row-major [rows][cols],
+ *    exactly the reference's Vec<Complex32> layouts.
+ *  - Every entry point returns 0 on success or a negative PS2_E* code; the
+ *    message of the last failure on the calling thread is in
+ *    ps2_hip_last_error().  The reference's error convention is
+ *    CoreError::NativeStage{stage: 2, message} (native_stage2.rs:2447-2453);
+ *    a Rust caller maps a negative return to that variant with this message.
+ *  - A session serialises its own calls; distinct sessions may run on
+ *    distinct host threads.
+ */
+#ifndef PYSTAMPS_HIP_H
+#define PYSTAMPS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PS2_HIP_ABI_VERSION 1
+
+/* return codes */
+#define PS2_OK 0
+#define PS2_E_ARG (-1)      /* invalid argument / shape (message says which) */
+#define PS2_E_DEVICE (-2)   /* HIP runtime failure */
+#define PS2_E_NOMEM (-3)    /* device allocation failed */
+#define PS2_E_STATE (-4)    /* session used out of order */
+
+/* computation forms */
+#define PS2_FORM_FP64 0     /* plain FP64 form: every step in f64 */
+#define PS2_FORM_FP32 1     /* FP32 bulk + FP64 near-tie trial re-score */
+
+/* weighting (Stage2Parms::filter_weighting, native_stage2.rs:364-384) */
+#define PS2_WEIGHT_PSQUARE 0
+#define PS2_WEIGHT_SNR 1
+
+#define PS2_COH_BINS 100
+
+/* Prepared per-patch inputs: mirror of Stage2Prepared + Stage2Options +
+ * Stage2Parms (native_stage2.rs:115-182). */
+typedef struct {
+    int64_t n_ps;             /* Stage2Prepared::n_ps */
+    int64_t n_ifg;            /* Stage2Prepared::n_ifg (master column dropped) */
+    int64_t n_i, n_j;         /* grid extent */
+    const float *ph_nm;       /* [n_ps][n_ifg] complex64: ph / |ph| */
+    const float *amp;         /* [n_ps][n_ifg]: |ph|, 1 where |ph| == 0 */
+    int32_t row_invariant;    /* row_invariant_bperp */
+    const double *bperp_row;  /* [n_ifg] row_bperp_nm */
+    const double *bperp_mat;  /* [n_ps][n_ifg]; may be NULL when row_invariant */
+    const int64_t *grid_lin;  /* [n_ps] cell = (i-1)*n_j + (j-1), each < n_i*n_j */
+    const double *d_a;        /* [n_ps] amplitude dispersion */
+    const double *low_pass;   /* [n_win_ex][n_win_ex] */
+    int64_t n_win_ex;         /* low_pass dimension = FFT size (clap_win) */
+    int64_t clap_window;      /* round(0.75 * clap_win) */
+    double clap_alpha, clap_beta;
+    const double *trial_values; /* [n_trial] */
+    int64_t n_trial;
+    const double *nr_base;    /* [100] random-coherence histogram */
+    double nr_max_nz_ix;
+    const double *coh_bins;   /* [100] */
+    int64_t low_coh_thresh;   /* 31 (15 small-baseline) */
+    int32_t weighting;        /* PS2_WEIGHT_* */
+    double gamma_change_convergence;
+    int64_t gamma_max_iterations;
+} ps2_patch_in;
+
+/* Outputs: mirror of the write_pm1 argument list (native_stage2.rs:1899-1916).
+ * Any pointer may be NULL to skip that output. */
+typedef struct {
+    double *k_ps, *c_ps, *coh_ps, *n_opt, *coh_ps_save; /* [n_ps] */
+    float *ph_res;            /* [n_ps][n_ifg] */
+    float *ph_patch;          /* [n_ps][n_ifg] complex64 */
+    float *ph_grid;           /* [n_i*n_j][n_ifg] complex64 */
+    float *ph_weight;         /* [n_ps][n_ifg] complex64 */
+    double *nr;               /* [100] last scaled Nr */
+    double gamma_change_save;
+    int64_t i_loop;
+    /* diagnostics */
+    int32_t *trial_ix;        /* [n_ps] selected trial index, -1 for invalid rows */
+    int64_t n_rescored;       /* FP32 form: rows re-scored in FP64, summed over iterations */
+    double loop_seconds;      /* device time of the gamma loop (first fill .. stop decision) */
+} ps2_patch_out;
+
+typedef struct {
+    int32_t form;             /* PS2_FORM_* */
+    int32_t device;           /* HIP device ordinal */
+    double rescore_bound;     /* FP32 form: coherence margin below which the top trials
+                                 are re-scored in FP64; <= 0 selects the derived bound */
+    int32_t verbose;          /* 1: per-iteration phase timers on stderr */
+    int32_t reserved;
+} ps2_run_opts;
+
+/* ABI version (PS2_HIP_ABI_VERSION). */
+int ps2_hip_abi_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *ps2_hip_last_error(void);
+
+/* Number of visible HIP devices (0 when none), or a negative code. */
+int ps2_hip_device_count(void);
+
+/* Whole stage-2 gamma loop for one patch: replaces the loop of
+ * run_stage2_native_inner (native_stage2.rs:242-406) and fills the write_pm1
+ * arguments (native_stage2.rs:319-336). */
+int ps2_hip_run(const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts);
+
+/* Several patches, each with its own convergence; the reference runs them one
+ * after another (pystamps-native.rs:158-176). */
+int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out,
+                      const ps2_run_opts *opts);
+
+/* Random-coherence histogram Nr of the stage-2 prep
+ * (random_coherence_histogram, native_stage2.rs:1310-1356, single-master
+ * branch): n_rows MATLAB-v5 uniform phase rows (seed 2005 in the reference,
+ * 300000 rows) fitted in FP64 on the device.  out_nr[100] receives the counts,
+ * *out_nr_max_nz_ix the last non-zero bin + 1 (1 if none). */
+int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
+                                       const double *trial_values, int64_t n_trial,
+                                       const double *coh_bins, int64_t n_rows, uint32_t seed,
+                                       int32_t device, double *out_nr,
+                                       double *out_nr_max_nz_ix);
+
+/* ---- Sessions: device-resident inputs, for repeated runs and per-step use ----
+ * A session uploads a patch's constant inputs once.  ps2_hip_session_run then
+ * runs the whole loop from the initial state (K = 0, weighting = 1/D_A) with
+ * every input already in HBM, and returns after the stop decision; outputs
+ * stay on the device until ps2_hip_session_download. */
+typedef struct ps2_session ps2_session;
+
+int ps2_hip_session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session **out);
+int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_seconds);
+int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out);
+void ps2_hip_session_destroy(ps2_session *s);
+
+/* Per-kernel device time of the session's runs so far (the reference prints
+ * the same phase breakdown per iteration, native_stage2.rs:263-305).
+ * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
+ * rescore, refine, rms_hist, weights.  ms[k] is the summed HIP-event time of
+ * slot k over launches[k] launches.  Timing events are recorded only when
+ * ps2_hip_session_set_timing(s, 1) was called before the runs. */
+#define PS2_NUM_KERNEL_SLOTS 8
+int ps2_hip_session_set_timing(ps2_session *s, int32_t enable);
+int ps2_hip_session_kernel_times(ps2_session *s, double *ms, int64_t *launches);
+
+/* Single steps on a session, for state injection.  Real-typed arrays are float
+ * (complex64) for PS2_FORM_FP32 and double (complex128) for PS2_FORM_FP64.
+ *  fill_grid: fill_phase_weight + accumulate_grid (native_stage2.rs:629-657,
+ *             722-740) from host K [n_ps] and weighting [n_ps]; ph_grid out.
+ *  clap:      clap_filter_grid_stack (:742-922) of host ph_grid; ph_filt out.
+ *  fit:       extract_patch_phase + fit_stage2_rows (:1186-1205, :659-720) of
+ *             host ph_filt; the out struct's k/c/coh/n_opt/ph_res/ph_patch/
+ *             trial_ix are filled (ph_res / ph_patch as float / complex64). */
+int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *weighting,
+                           void *ph_grid_out);
+int ps2_hip_step_clap(ps2_session *s, const void *ph_grid, void *ph_filt_out);
+int ps2_hip_step_fit(ps2_session *s, const void *ph_filt, ps2_patch_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYSTAMPS_HIP_H */

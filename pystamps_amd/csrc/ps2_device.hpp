@@ -1,0 +1,169 @@
+// Device helpers shared by the stage-2 kernels (gfx950).
+//
+// Arithmetic conventions: the library is compiled with -ffp-contract=off so a
+// product followed by a sum rounds twice, as in the reference (rustc never
+// fuses).  Where a kernel wants a fused multiply-add it calls fma()/fmaf()
+// explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace ps2 {
+
+template <typename R>
+struct cplx {
+    R re, im;
+};
+
+// Real-type traits: the FP32 form runs on float, the plain FP64 form on double.
+template <typename R>
+struct RT;
+
+template <>
+struct RT<float> {
+    using key_t = uint32_t;
+    static constexpr int key_top_bit = 30;  // keys are of non-negative floats
+    __device__ static inline float pi4() { return 0.785398185253143310546875f; }  // f32(pi)/4
+    __device__ static inline void sincos_(float x, float *s, float *c) { sincosf(x, s, c); }
+    __device__ static inline float atan2_(float y, float x) { return atan2f(y, x); }
+    __device__ static inline float sqrt_(float x) { return sqrtf(x); }
+    __device__ static inline float pow_(float x, float y) { return powf(x, y); }
+    // glibc hypotf: sqrt of the f64 sum of f64 squares, rounded once to f32
+    // (both squares are exact in f64).
+    __device__ static inline float hypot_(float x, float y) {
+        float ax = fabsf(x), ay = fabsf(y);
+        if (__builtin_isinf(ax) || __builtin_isinf(ay)) return INFINITY;
+        if (__builtin_isnan(ax) || __builtin_isnan(ay)) return ax * ay;
+        if (ax == 0.0f) return ay;
+        if (ay == 0.0f) return ax;
+        double dx = (double)x, dy = (double)y;
+        return (float)__builtin_sqrt(dx * dx + dy * dy);
+    }
+    __device__ static inline key_t key(float v) { return __float_as_uint(v); }
+    __device__ static inline float from_key(key_t k) { return __uint_as_float(k); }
+    __device__ static inline bool isnan_(float v) { return __builtin_isnan(v); }
+};
+
+template <>
+struct RT<double> {
+    using key_t = uint64_t;
+    static constexpr int key_top_bit = 62;
+    __device__ static inline double pi4() { return 0.78539816339744830962; }
+    __device__ static inline void sincos_(double x, double *s, double *c) { sincos(x, s, c); }
+    __device__ static inline double atan2_(double y, double x) { return atan2(y, x); }
+    __device__ static inline double sqrt_(double x) { return sqrt(x); }
+    __device__ static inline double pow_(double x, double y) { return pow(x, y); }
+    __device__ static inline double hypot_(double x, double y) { return hypot(x, y); }
+    __device__ static inline key_t key(double v) { return (uint64_t)__double_as_longlong(v); }
+    __device__ static inline double from_key(key_t k) { return __longlong_as_double((long long)k); }
+    __device__ static inline bool isnan_(double v) { return __builtin_isnan(v); }
+};
+
+// num-complex Complex<T> * Complex<T>: (a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re)
+template <typename R>
+__device__ inline cplx<R> cmul(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// cos(k*pi/16), k = 0..8
+template <typename R>
+__device__ constexpr R c16(int k) {
+    constexpr double t[9] = {1.0,
+                             0.98078528040323044913,
+                             0.92387953251128675613,
+                             0.83146961230254523708,
+                             0.70710678118654752440,
+                             0.55557023301960222474,
+                             0.38268343236508977173,
+                             0.19509032201612826785,
+                             0.0};
+    return (R)t[k];
+}
+// e^{-2 pi i k / 32} = cos(k pi/16) - i sin(k pi/16), 0 <= k < 16
+template <typename R>
+__device__ constexpr R twr(int k) {
+    return k <= 8 ? c16<R>(k) : -c16<R>(16 - k);
+}
+template <typename R>
+__device__ constexpr R twi_fwd(int k) {
+    return -(k <= 8 ? c16<R>(8 - k) : c16<R>(k - 8));
+}
+
+__device__ constexpr int bitrev5(int v) {
+    return ((v & 1) << 4) | ((v & 2) << 2) | (v & 4) | ((v & 8) >> 2) | ((v & 16) >> 4);
+}
+
+// 32-point radix-2 decimation-in-time FFT held in registers, unnormalised:
+// forward X[k] = sum x[n] e^{-2 pi i nk/32}, inverse with e^{+...} (the
+// rustfft conventions of native_stage2.rs:1055-1095).
+template <typename R, bool INV>
+__device__ inline void fft32(R (&xr)[32], R (&xi)[32]) {
+    R ar[32], ai[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        ar[bitrev5(k)] = xr[k];
+        ai[bitrev5(k)] = xi[k];
+    }
+#pragma unroll
+    for (int s = 1; s <= 5; ++s) {
+        const int m = 1 << s, half = m >> 1, step = 32 / m;
+#pragma unroll
+        for (int b = 0; b < 32; b += m) {
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const int tw = j * step;
+                R br = ar[b + j + half], bi = ai[b + j + half];
+                R vr, vi;
+                if (tw == 0) {
+                    vr = br;
+                    vi = bi;
+                } else if (tw == 8) {
+                    // multiply by -i (forward) or +i (inverse)
+                    if (INV) {
+                        vr = -bi;
+                        vi = br;
+                    } else {
+                        vr = bi;
+                        vi = -br;
+                    }
+                } else {
+                    const R wr = twr<R>(tw);
+                    const R wi = INV ? -twi_fwd<R>(tw) : twi_fwd<R>(tw);
+                    vr = br * wr - bi * wi;
+                    vi = br * wi + bi * wr;
+                }
+                R ur = ar[b + j], ui = ai[b + j];
+                ar[b + j] = ur + vr;
+                ai[b + j] = ui + vi;
+                ar[b + j + half] = ur - vr;
+                ai[b + j + half] = ui - vi;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        xr[k] = ar[k];
+        xi[k] = ai[k];
+    }
+}
+
+// Sum over the 32 lanes of a half-wave team (lanes 0-31 or 32-63).
+__device__ inline int team_sum_i32(int v) {
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
+
+template <typename K>
+__device__ inline K team_min_key(K v) {
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+        K w = __shfl_xor(v, o, 32);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+}  // namespace ps2

@@ -1,0 +1,751 @@
+// Stage-2 gamma-loop kernels for gfx950, templated on the real type R
+// (float: FP32 form, double: plain FP64 form).
+//
+// Device layouts (set up once per patch by the host, see ps2_stage2.hip):
+//   pixels are stored in cell-sorted order k (stable sort of grid_lin), so the
+//   pixels of one grid cell are contiguous and each cell is a CSR range
+//   cell_start[c] .. cell_start[c+1];
+//   phnm  [P][I] complex R   ph_nm rows in sorted order
+//   bp    [P][I] R           bperp_mat rows (or [I] when the rows are invariant)
+//   grid, filt [C][I] complex R  (cell-major, the reference's ph_grid layout)
+//   part  [nwi][24][n_j][I] complex R  per-window-row CLAP partial sums
+#pragma once
+
+#include "ps2_device.hpp"
+
+namespace ps2 {
+
+constexpr int CLAP_FFT = 32;   // n_win_ex
+constexpr int CLAP_WIN = 24;   // n_win
+constexpr int CLAP_INC = 6;    // n_inc = n_win / 4
+
+// ---------------------------------------------------------------------------
+// fill_phase_weight value (native_stage2.rs:648-652):
+//   phase = -(bp * K); src * (cos, sin) * w   (all in R)
+template <typename R>
+__device__ inline cplx<R> weight_value(cplx<R> s, R b, double K, double W) {
+    const R kk = (R)K, ww = (R)W;
+    const R phase = -(b * kk);
+    R sn, cs;
+    RT<R>::sincos_(phase, &sn, &cs);
+    const R re = s.re * cs - s.im * sn;
+    const R im = s.re * sn + s.im * cs;
+    return {re * ww, im * ww};
+}
+
+// fill_phase_weight + accumulate_grid (native_stage2.rs:629-657, 722-740):
+// one thread per (cell, ifg); the cell's pixels are summed in ascending row
+// order, which is the reference's sequential order, so no atomics are needed.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ phnm,
+                                                   const R *__restrict__ bp, int bp_rowinv,
+                                                   const double *__restrict__ K,
+                                                   const double *__restrict__ W,
+                                                   const int32_t *__restrict__ cell_start,
+                                                   cplx<R> *__restrict__ grid, int64_t C, int I) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= C * I) return;
+    const int64_t c = gid / I;
+    const int f = (int)(gid - c * I);
+    const int32_t k0 = cell_start[c], k1 = cell_start[c + 1];
+    R sre = (R)0, sim = (R)0;
+    for (int32_t k = k0; k < k1; ++k) {
+        const int64_t e = (int64_t)k * I + f;
+        const R b = bp_rowinv ? bp[f] : bp[e];
+        const cplx<R> v = weight_value<R>(phnm[e], b, K[k], W[k]);
+        sre += v.re;
+        sim += v.im;
+    }
+    grid[gid] = {sre, sim};
+}
+
+// ph_weight output of the last fill, written in the caller's row order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict__ phnm,
+                                                      const R *__restrict__ bp, int bp_rowinv,
+                                                      const double *__restrict__ K,
+                                                      const double *__restrict__ W,
+                                                      const int64_t *__restrict__ order,
+                                                      float2 *__restrict__ out, int64_t P, int I) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= P * I) return;
+    const int64_t k = gid / I;
+    const int f = (int)(gid - k * I);
+    const R b = bp_rowinv ? bp[f] : bp[gid];
+    const cplx<R> v = weight_value<R>(phnm[gid], b, K[k], W[k]);
+    out[order[k] * I + f] = make_float2((float)v.re, (float)v.im);
+}
+
+// ---------------------------------------------------------------------------
+// CLAP (native_stage2.rs:742-1184).
+//
+// One workgroup per (window row ix, group of TEAMS interferograms).  A team of
+// 32 lanes (half a wavefront) owns one interferogram and walks the windows of
+// the row left to right; each window is a 32x32 FFT held one row / one column
+// per lane in registers, transposed through LDS.  The weighted window outputs
+// are summed per team in a 24-row rolling accumulator in LDS; every column
+// block that no later window of the row touches is flushed to `part`.  A
+// second kernel adds the (at most four) window rows covering each cell in
+// ascending row order.  No atomics, so the result is deterministic.
+
+template <typename R>
+struct ClapParams {
+    R k7[7];        // gausswin(7, 2.5) smoothing taps (native_stage2.rs:1044-1053)
+    R alpha, beta;  // clap_alpha, clap_beta
+    int alpha_is_one;
+    int ni, nj, I, nwi, nwj, ngroups;
+};
+
+// median of the 1024 non-negative values held 32 per lane by a team
+// (native_stage2.rs:1168-1184: sort, mean of the two middle values).
+template <typename R>
+__device__ inline R team_median1024(const R (&h)[32]) {
+    using K = typename RT<R>::key_t;
+    constexpr K signmask = ~(K(1) << (sizeof(K) * 8 - 1));
+    K keys[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) keys[r] = RT<R>::key(h[r]) & signmask;
+    // smallest v with count(x <= v) >= 512 is the 0-based 511th value
+    K ans = 0;
+    for (int bit = RT<R>::key_top_bit; bit >= 0; --bit) {
+        const K t = ans | ((K(1) << bit) - 1);
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) cnt += keys[r] <= t ? 1 : 0;
+        cnt = team_sum_i32(cnt);
+        if (cnt < 512) ans |= K(1) << bit;
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) cnt += keys[r] <= ans ? 1 : 0;
+    cnt = team_sum_i32(cnt);
+    K nxt = ans;
+    if (cnt < 513) {
+        K mn = ~K(0);
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            if (keys[r] > ans && keys[r] < mn) mn = keys[r];
+        nxt = team_min_key<K>(mn);
+    }
+    return (RT<R>::from_key(ans) + RT<R>::from_key(nxt)) / (R)2;
+}
+
+template <typename R, int TEAMS>
+__global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__restrict__ grid,
+                                                           cplx<R> *__restrict__ part,
+                                                           const R *__restrict__ lp,
+                                                           const R *__restrict__ wfac,
+                                                           ClapParams<R> prm) {
+    const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
+    const int ix = blockIdx.x / prm.ngroups;
+    const int g = blockIdx.x - ix * prm.ngroups;
+    const int f = g * TEAMS + team;
+    const bool active = f < prm.I;
+    const int ni = prm.ni, nj = prm.nj, I = prm.I;
+    int i1 = ix * CLAP_INC, rs = 0;
+    if (i1 + CLAP_WIN > ni) {
+        rs = i1 + CLAP_WIN - ni;
+        i1 = ni - CLAP_WIN;
+    }
+    __shared__ cplx<R> tb[TEAMS][32][33];
+    __shared__ cplx<R> acc[TEAMS][CLAP_WIN][CLAP_WIN];
+    for (int q = lane; q < CLAP_WIN * CLAP_WIN; q += 32)
+        acc[team][q / CLAP_WIN][q % CLAP_WIN] = {(R)0, (R)0};
+    const int cS = (lane + 16) & 31;  // fftshift of this lane's column
+
+    for (int jx = 0; jx < prm.nwj; ++jx) {
+        int j1 = jx * CLAP_INC, cs = 0;
+        if (j1 + CLAP_WIN > nj) {
+            cs = j1 + CLAP_WIN - nj;
+            j1 = nj - CLAP_WIN;
+        }
+        R xr[32], xi[32];
+        // window row `lane`, zero padded to 32 (NaN -> 0, native_stage2.rs:852-858)
+        if (lane < CLAP_WIN && active) {
+            const cplx<R> *src = grid + ((int64_t)(i1 + lane) * nj + j1) * I + f;
+#pragma unroll
+            for (int c = 0; c < CLAP_WIN; ++c) {
+                cplx<R> v = src[(int64_t)c * I];
+                if (RT<R>::isnan_(v.re) || RT<R>::isnan_(v.im)) v = {(R)0, (R)0};
+                xr[c] = v.re;
+                xi[c] = v.im;
+            }
+#pragma unroll
+            for (int c = CLAP_WIN; c < 32; ++c) xr[c] = xi[c] = (R)0;
+            fft32<R, false>(xr, xi);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 32; ++c) xr[c] = xi[c] = (R)0;
+        }
+        __syncthreads();  // previous window's readers of tb are done
+#pragma unroll
+        for (int c = 0; c < 32; ++c) tb[team][lane][c] = {xr[c], xi[c]};
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const cplx<R> v = tb[team][r][lane];
+            xr[r] = v.re;
+            xi[r] = v.im;
+        }
+        fft32<R, false>(xr, xi);  // lane c now holds column c of FFT2
+        __syncthreads();
+        // |F| at fftshifted position (.re plane)
+#pragma unroll
+        for (int r = 0; r < 32; ++r) tb[team][(r + 16) & 31][cS].re = RT<R>::hypot_(xr[r], xi[r]);
+        __syncthreads();
+        {   // separable 7-tap smoothing, zero fill: rows (lane = shifted row) -> .im plane
+            R h[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c) h[c] = tb[team][lane][c].re;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) {
+                R s = (R)0;
+#pragma unroll
+                for (int kc = 0; kc < 7; ++kc) {
+                    const int sc = c + kc - 3;
+                    if (sc >= 0 && sc < 32) s += h[sc] * prm.k7[kc];
+                }
+                tb[team][lane][c].im = s;
+            }
+        }
+        __syncthreads();
+        {   // columns (lane = shifted column) -> .re plane
+            R t[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) t[r] = tb[team][r][lane].im;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                R s = (R)0;
+#pragma unroll
+                for (int kr = 0; kr < 7; ++kr) {
+                    const int sr = r + kr - 3;
+                    if (sr >= 0 && sr < 32) s += t[sr] * prm.k7[kr];
+                }
+                tb[team][r][lane].re = s;
+            }
+        }
+        __syncthreads();
+        // ifftshift back: lane c takes H[r][c] = Hf[(r+16)%32][(c+16)%32]
+        R h[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) h[r] = tb[team][(r + 16) & 31][cS].re;
+        const R med = team_median1024<R>(h);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            R v = h[r];
+            if (med != (R)0) v /= med;
+            v = prm.alpha_is_one ? v - (R)1 : RT<R>::pow_(v, prm.alpha) - (R)1;
+            if (v < (R)0) v = (R)0;
+            const R gsc = v * prm.beta + lp[r * 32 + lane];
+            xr[r] *= gsc;
+            xi[r] *= gsc;
+        }
+        fft32<R, true>(xr, xi);  // inverse along columns
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 32; ++r) tb[team][r][lane] = {xr[r], xi[r]};
+        __syncthreads();
+        if (lane < CLAP_WIN) {
+#pragma unroll
+            for (int c = 0; c < 32; ++c) {
+                const cplx<R> v = tb[team][lane][c];
+                xr[c] = v.re;
+                xi[c] = v.im;
+            }
+            fft32<R, true>(xr, xi);  // inverse along rows: spatial row `lane`
+            if (active && lane >= rs) {
+                const int lr = lane - rs;
+#pragma unroll
+                for (int lj = 0; lj < CLAP_WIN; ++lj) {
+                    if (lj < cs) continue;
+                    const int e = lj - cs;
+                    const int slot = (jx * CLAP_INC + e) % CLAP_WIN;
+                    const R fct = wfac[lr * CLAP_WIN + e];
+                    acc[team][lr][slot].re += xr[lj] * fct;
+                    acc[team][lr][slot].im += xi[lj] * fct;
+                }
+                // flush the columns no later window of this row touches
+                const int cbeg = jx * CLAP_INC;
+                const int cend = (jx == prm.nwj - 1) ? nj : cbeg + CLAP_INC;
+                cplx<R> *dst = part + (((int64_t)ix * CLAP_WIN + lr) * nj) * I + f;
+                for (int col = cbeg; col < cend; ++col) {
+                    const int slot = col % CLAP_WIN;
+                    dst[(int64_t)col * I] = acc[team][lr][slot];
+                    acc[team][lr][slot] = {(R)0, (R)0};
+                }
+            }
+        }
+    }
+}
+
+// Sum of the window rows covering each cell (ascending row order).
+template <typename R>
+__global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
+                                                      cplx<R> *__restrict__ filt, int ni, int nj,
+                                                      int I, int nwi) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t C = (int64_t)ni * nj;
+    if (gid >= C * I) return;
+    const int64_t cell = gid / I;
+    const int f = (int)(gid - cell * I);
+    const int r = (int)(cell / nj), c = (int)(cell - (int64_t)r * nj);
+    int ix_hi = r / CLAP_INC;
+    if (ix_hi > nwi - 1) ix_hi = nwi - 1;
+    const int ix_lo = r >= CLAP_WIN - 1 ? (r - (CLAP_WIN - 1) + CLAP_INC - 1) / CLAP_INC : 0;
+    R sre = (R)0, sim = (R)0;
+    for (int ix = ix_lo; ix <= ix_hi; ++ix) {
+        const int lr = r - ix * CLAP_INC;
+        const cplx<R> v = part[(((int64_t)ix * CLAP_WIN + lr) * nj + c) * I + f];
+        sre += v.re;
+        sim += v.im;
+    }
+    filt[gid] = {sre, sim};
+}
+
+// ---------------------------------------------------------------------------
+// extract_patch_phase (native_stage2.rs:1186-1205) fused with the psdph
+// product of fit_stage2_rows (:686-694): conj(unit(filt)) * ph_nm.
+template <typename R>
+__device__ inline cplx<R> unit_phase(cplx<R> v) {
+    const R mag = RT<R>::hypot_(v.re, v.im);
+    if (mag != (R)0) {
+        v.re /= mag;
+        v.im /= mag;
+    }
+    return v;
+}
+
+template <typename R>
+__device__ inline cplx<R> psdph_value(cplx<R> filt, cplx<R> nm) {
+    const cplx<R> p = unit_phase<R>(filt);
+    const R ar = p.re, ai = -p.im;
+    return {ar * nm.re - ai * nm.im, ar * nm.im + ai * nm.re};
+}
+
+// v * e^{-i*phase} with the reference's literal complex product
+template <typename R>
+__device__ inline cplx<R> rotate_neg(cplx<R> v, R phase) {
+    R sn, cs;
+    RT<R>::sincos_(phase, &sn, &cs);
+    const R nsn = -sn;
+    return {v.re * cs - v.im * nsn, v.re * nsn + v.im * cs};
+}
+
+struct FitParams {
+    int64_t P;
+    int I, tn, T, rowinv;
+    float rescore_bound;  // FP32 form only
+};
+
+// Trial scan of topofit_row (native_stage2.rs:1207-1256) for one pixel per
+// thread.  Trials are processed in chunks of TC positive trials t and their
+// negatives -t.
+//   FP32 form (R=float): per ifg the rotation e^{-i phi t} comes from the
+//     complex recurrence z_{t+1} = z_t * e^{-i phi}, re-anchored with an exact
+//     sincos at the first trial of each chunk; e^{-i phi (-t)} = conj(z_t).
+//     Rows whose best-vs-second margin is below rescore_bound are queued for
+//     the FP64 re-score.
+//   FP64 form (R=double): every rotation is an exact sincos, as the reference.
+// Output: tix (selected trial index, -1 for invalid rows), rng (bperp range).
+template <typename R, int TC>
+__global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
+                                              const R *__restrict__ bp,
+                                              const cplx<R> *__restrict__ filt,
+                                              const int32_t *__restrict__ cell_of,
+                                              int32_t *__restrict__ tix, R *__restrict__ rng,
+                                              int32_t *__restrict__ rescore_list,
+                                              uint32_t *__restrict__ rescore_count,
+                                              FitParams prm) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= prm.P) return;
+    const int I = prm.I;
+    const cplx<R> *nm = phnm + k * I;
+    const cplx<R> *fl = filt + (int64_t)cell_of[k] * I;
+    const R *b = prm.rowinv ? bp : bp + k * I;
+
+    bool valid = true;
+    R denom = (R)0;
+    R mn = (R)INFINITY, mx = -(R)INFINITY;
+    for (int i = 0; i < I; ++i) {
+        const cplx<R> v = psdph_value<R>(fl[i], nm[i]);
+        if (v.re == (R)0 && v.im == (R)0) valid = false;
+        denom += RT<R>::hypot_(v.re, v.im);
+        const R bi = b[i];
+        if (bi < mn) mn = bi;
+        if (bi > mx) mx = bi;
+    }
+    if (!valid) {
+        tix[k] = -1;
+        rng[k] = (R)1;
+        return;
+    }
+    if (denom == (R)0) denom = (R)1;
+    R range = mx - mn;
+    if (range == (R)0) range = (R)1;
+    rng[k] = range;
+    const R pi4 = RT<R>::pi4();
+    const int tn = prm.tn;
+
+    R best = -(R)INFINITY, second = -(R)INFINITY;
+    int best_ix = 0;
+    auto offer = [&](R val, int idx) {
+        if (val > best || (val == best && idx < best_ix)) {
+            if (best_ix != idx) second = best > second ? best : second;
+            best = val;
+            best_ix = idx;
+        } else if (val > second) {
+            second = val;
+        }
+    };
+
+    for (int t0 = 0; t0 <= tn; t0 += TC) {
+        // chunk: positive trials t0 .. t0+TC-1 (t0 = 0 holds trial 0) and their negatives
+        R pre[TC], pim[TC], nre[TC], nim[TC];
+#pragma unroll
+        for (int q = 0; q < TC; ++q) pre[q] = pim[q] = nre[q] = nim[q] = (R)0;
+        for (int i = 0; i < I; ++i) {
+            const cplx<R> v = psdph_value<R>(fl[i], nm[i]);
+            const R phi = (b[i] / range) * pi4;
+            if constexpr (sizeof(R) == 4) {
+                R s1, c1;
+                RT<R>::sincos_(phi, &s1, &c1);
+                const R wr = c1, wi = -s1;  // e^{-i phi}
+                R zr, zi;
+                if (t0 == 0) {
+                    zr = (R)1;
+                    zi = (R)0;
+                } else {
+                    R sa, ca;
+                    RT<R>::sincos_(phi * (R)t0, &sa, &ca);
+                    zr = ca;
+                    zi = -sa;
+                }
+#pragma unroll
+                for (int q = 0; q < TC; ++q) {
+                    if (t0 + q <= tn) {
+                        // v*z and v*conj(z)
+                        pre[q] = fmaf(v.re, zr, fmaf(-v.im, zi, pre[q]));
+                        pim[q] = fmaf(v.re, zi, fmaf(v.im, zr, pim[q]));
+                        nre[q] = fmaf(v.re, zr, fmaf(v.im, zi, nre[q]));
+                        nim[q] = fmaf(v.im, zr, fmaf(-v.re, zi, nim[q]));
+                        const R nzr = fmaf(zr, wr, -zi * wi);
+                        const R nzi = fmaf(zr, wi, zi * wr);
+                        zr = nzr;
+                        zi = nzi;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < TC; ++q) {
+                    const int t = t0 + q;
+                    if (t <= tn) {
+                        const cplx<R> a = rotate_neg<R>(v, phi * (R)t);
+                        pre[q] += a.re;
+                        pim[q] += a.im;
+                        if (t != 0) {
+                            const cplx<R> bb = rotate_neg<R>(v, phi * (R)(-t));
+                            nre[q] += bb.re;
+                            nim[q] += bb.im;
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+            const int t = t0 + q;
+            if (t <= tn) {
+                offer(RT<R>::hypot_(pre[q], pim[q]) / denom, t + tn);
+                if (t != 0) offer(RT<R>::hypot_(nre[q], nim[q]) / denom, tn - t);
+            }
+        }
+    }
+    tix[k] = best_ix;
+    if constexpr (sizeof(R) == 4) {
+        if (prm.T > 1 && best - second < (R)prm.rescore_bound) {
+            const uint32_t slot = atomicAdd(rescore_count, 1u);
+            rescore_list[slot] = (int32_t)k;
+        }
+    }
+}
+
+// FP64 re-score of near-tie rows (FP32 form): a 64-lane wavefront per queued
+// row; the lanes split the trials, each summing over all ifgs in ascending
+// order exactly as ref64's topofit_row; first argmax with strict >.
+// bp64 holds the caller's f64 bperp (the f32 copy used by the bulk is exact
+// only to f32).
+template <typename RF>
+__global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phnm,
+                                                const double *__restrict__ bp64,
+                                                const cplx<RF> *__restrict__ filt,
+                                                const int32_t *__restrict__ cell_of,
+                                                const int32_t *__restrict__ rescore_list,
+                                                const uint32_t *__restrict__ rescore_count,
+                                                int32_t *__restrict__ tix, FitParams prm) {
+    const uint32_t n = *rescore_count;
+    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const int64_t k = rescore_list[q];
+        const int I = prm.I;
+        const cplx<RF> *nm = phnm + k * I;
+        const cplx<RF> *fl = filt + (int64_t)cell_of[k] * I;
+        const double *b = prm.rowinv ? bp64 : bp64 + k * I;
+        double denom = 0.0, mn = INFINITY, mx = -INFINITY;
+        for (int i = 0; i < I; ++i) {
+            const cplx<RF> v = psdph_value<RF>(fl[i], nm[i]);
+            denom += hypot((double)v.re, (double)v.im);
+            if (b[i] < mn) mn = b[i];
+            if (b[i] > mx) mx = b[i];
+        }
+        if (denom == 0.0) denom = 1.0;
+        double range = mx - mn;
+        if (range == 0.0) range = 1.0;
+        double bestv = -INFINITY;
+        int besti = 0x7fffffff;
+        for (int t = threadIdx.x; t < prm.T; t += 64) {
+            const double tv = (double)(t - prm.tn);
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < I; ++i) {
+                const cplx<RF> v = psdph_value<RF>(fl[i], nm[i]);
+                const double ph = ((b[i] / range) * 0.78539816339744830962) * tv;
+                double sn, cs;
+                sincos(ph, &sn, &cs);
+                const double nsn = -sn;
+                const double vr = (double)v.re, vi = (double)v.im;
+                sr += vr * cs - vi * nsn;
+                si += vr * nsn + vi * cs;
+            }
+            const double c = hypot(sr, si) / denom;
+            if (c > bestv) {
+                bestv = c;
+                besti = t;
+            }
+        }
+        // wave-wide first argmax: larger value wins, ties go to the lower index
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double ov = __shfl_xor(bestv, o, 64);
+            const int oi = __shfl_xor(besti, o, 64);
+            if (ov > bestv || (ov == bestv && oi < besti)) {
+                bestv = ov;
+                besti = oi;
+            }
+        }
+        if (threadIdx.x == 0 && besti < prm.T) tix[k] = besti;  // else keep the FP32 choice
+    }
+}
+
+// refine_candidate (native_stage2.rs:1259-1303) from the selected trial:
+// offset sum, f64 least-squares step mopt, final residual sum.
+template <typename R>
+__global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm,
+                                                const R *__restrict__ bp,
+                                                const cplx<R> *__restrict__ filt,
+                                                const int32_t *__restrict__ cell_of,
+                                                const int32_t *__restrict__ tix,
+                                                const R *__restrict__ rng, double *__restrict__ Kout,
+                                                double *__restrict__ Cout,
+                                                double *__restrict__ cohout,
+                                                double *__restrict__ nopt, FitParams prm) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= prm.P) return;
+    const int t = tix[k];
+    if (t < 0) {
+        Kout[k] = __builtin_nan("");
+        Cout[k] = 0.0;
+        cohout[k] = 0.0;
+        nopt[k] = 0.0;
+        return;
+    }
+    const int I = prm.I;
+    const cplx<R> *nm = phnm + k * I;
+    const cplx<R> *fl = filt + (int64_t)cell_of[k] * I;
+    const R *b = prm.rowinv ? bp : bp + k * I;
+    const R range = rng[k];
+    R k0 = (R)(RT<R>::pi4() / range * (R)(t - prm.tn));
+    R ore = (R)0, oim = (R)0;
+    for (int i = 0; i < I; ++i) {
+        const cplx<R> a = rotate_neg<R>(psdph_value<R>(fl[i], nm[i]), k0 * b[i]);
+        ore += a.re;
+        oim += a.im;
+    }
+    const R ocr = ore, oci = -oim;
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < I; ++i) {
+        const cplx<R> v = psdph_value<R>(fl[i], nm[i]);
+        const double wgt = (double)RT<R>::hypot_(v.re, v.im);
+        const double wb = wgt * (double)b[i];
+        den += wb * wb;
+        const cplx<R> r = rotate_neg<R>(v, k0 * b[i]);
+        const R pr = r.re * ocr - r.im * oci, pim = r.re * oci + r.im * ocr;
+        num += wb * (wgt * (double)RT<R>::atan2_(pim, pr));
+    }
+    if (den == 0.0) den = 1.0;
+    k0 = (R)((double)k0 + num / den);
+    R mre = (R)0, mim = (R)0, dn = (R)0;
+    for (int i = 0; i < I; ++i) {
+        const cplx<R> r = rotate_neg<R>(psdph_value<R>(fl[i], nm[i]), k0 * b[i]);
+        mre += r.re;
+        mim += r.im;
+        dn += RT<R>::hypot_(r.re, r.im);
+    }
+    if (dn == (R)0) dn = (R)1;
+    Kout[k] = (double)k0;
+    Cout[k] = (double)RT<R>::atan2_(mim, mre);
+    cohout[k] = (double)(RT<R>::hypot_(mre, mim) / dn);
+    nopt[k] = 1.0;
+}
+
+// ph_patch (unit filt) for every row and ph_res = arg(residual) of valid rows,
+// in the caller's row order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restrict__ phnm,
+                                                         const R *__restrict__ bp,
+                                                         const cplx<R> *__restrict__ filt,
+                                                         const int32_t *__restrict__ cell_of,
+                                                         const double *__restrict__ K,
+                                                         const double *__restrict__ nopt,
+                                                         const int64_t *__restrict__ order,
+                                                         float2 *__restrict__ patch_out,
+                                                         float *__restrict__ res_out, int64_t P,
+                                                         int I, int rowinv) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= P * I) return;
+    const int64_t k = gid / I;
+    const int f = (int)(gid - k * I);
+    const cplx<R> fv = filt[(int64_t)cell_of[k] * I + f];
+    const cplx<R> p = unit_phase<R>(fv);
+    const int64_t o = order[k] * I + f;
+    if (patch_out) patch_out[o] = make_float2((float)p.re, (float)p.im);
+    if (res_out) {
+        float pr = 0.0f;
+        if (nopt[k] == 1.0) {
+            const cplx<R> v = psdph_value<R>(fv, phnm[gid]);
+            const R b = rowinv ? bp[f] : bp[gid];
+            const cplx<R> r = rotate_neg<R>(v, (R)K[k] * b);
+            pr = (float)RT<R>::atan2_(r.im, r.re);
+        }
+        res_out[o] = pr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gamma-change RMS partial sums (native_stage2.rs:2045-2057) and the coh
+// histogram (hist_with_centers, :1538-1622); coh_prev <- coh (:310).
+struct HistParams {
+    int64_t P;
+    int equal;       // equal-spacing rule
+    double cutoff0, d;
+    int nbins;
+};
+
+constexpr int RMS_BLOCK = 256;
+constexpr int RMS_PER_THREAD = 8;
+
+__device__ inline int hist_bin(double v, const double *__restrict__ mids, HistParams hp) {
+    if (!__builtin_isfinite(v)) return -1;
+    if (hp.equal) {
+        if (hp.d == 0.0) return 0;
+        double a = ceil((v - hp.cutoff0) / hp.d);
+        if (a < 0.0) a = 0.0;
+        if (a > (double)(hp.nbins - 1)) a = (double)(hp.nbins - 1);
+        return (int)a;
+    }
+    int b = 0;
+    while (b < hp.nbins - 1 && mids[b] < v) ++b;
+    return b;
+}
+
+__global__ __launch_bounds__(RMS_BLOCK) void k_rms_hist(const double *__restrict__ coh,
+                                                        double *__restrict__ coh_prev,
+                                                        double *__restrict__ part,
+                                                        uint32_t *__restrict__ hist,
+                                                        const double *__restrict__ mids,
+                                                        HistParams hp) {
+    __shared__ double red[RMS_BLOCK];
+    __shared__ uint32_t lh[128];
+    for (int q = threadIdx.x; q < 128; q += RMS_BLOCK) lh[q] = 0;
+    __syncthreads();
+    double s = 0.0;
+    const int64_t base = (int64_t)blockIdx.x * RMS_BLOCK * RMS_PER_THREAD;
+#pragma unroll
+    for (int e = 0; e < RMS_PER_THREAD; ++e) {
+        const int64_t k = base + (int64_t)e * RMS_BLOCK + threadIdx.x;
+        if (k < hp.P) {
+            const double c = coh[k];
+            const double d = c - coh_prev[k];
+            s += d * d;
+            coh_prev[k] = c;
+            const int bin = hist_bin(c, mids, hp);
+            if (bin >= 0) atomicAdd(&lh[bin], 1u);
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = RMS_BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    for (int q = threadIdx.x; q < hp.nbins; q += RMS_BLOCK)
+        if (lh[q]) atomicAdd(&hist[q], lh[q]);
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0u;
+}
+
+// psquare weighting lookup (native_stage2.rs:1674-1682):
+// w = (1 - prand_hi[round_half_away(1000 coh)])^2
+__global__ __launch_bounds__(256) void k_weights_psquare(const double *__restrict__ coh,
+                                                         const double *__restrict__ prand_hi,
+                                                         int nhi, double *__restrict__ W,
+                                                         int64_t P) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    const double v = coh[k] * 1000.0;
+    double r = v >= 0.0 ? floor(v + 0.5) : ceil(v - 0.5);
+    const double hi = (double)(nhi > 0 ? nhi - 1 : 0);
+    if (r < 0.0) r = 0.0;
+    if (r > hi) r = hi;
+    const int ix = __builtin_isnan(r) ? 0 : (int)r;
+    const double p = prand_hi[ix];
+    W[k] = (1.0 - p) * (1.0 - p);
+}
+
+// snr_weighting (native_stage2.rs:1875-1897) from the current residuals.
+template <typename R>
+__global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__ phnm,
+                                                     const float *__restrict__ amp,
+                                                     const R *__restrict__ bp,
+                                                     const cplx<R> *__restrict__ filt,
+                                                     const int32_t *__restrict__ cell_of,
+                                                     const double *__restrict__ K,
+                                                     const double *__restrict__ nopt,
+                                                     double *__restrict__ W, int64_t P, int I,
+                                                     int rowinv) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    double g = 0.0, a2 = 0.0;
+    const bool valid = nopt[k] == 1.0;
+    for (int f = 0; f < I; ++f) {
+        const int64_t e = k * I + f;
+        const double a = (double)amp[e];
+        double pr = 0.0;
+        if (valid) {
+            const cplx<R> v = psdph_value<R>(filt[(int64_t)cell_of[k] * I + f], phnm[e]);
+            const R b = rowinv ? bp[f] : bp[e];
+            const cplx<R> r = rotate_neg<R>(v, (R)K[k] * b);
+            pr = (double)(R)RT<R>::atan2_(r.im, r.re);
+        }
+        g += a * cos(pr);
+        a2 += a * a;
+    }
+    const double dd = (double)(I > 1 ? I : 1);
+    g /= dd;
+    a2 /= dd;
+    const double sigma = sqrt(0.5 * (a2 - g * g));
+    W[k] = sigma != 0.0 ? g / sigma : 0.0;
+}
+
+}  // namespace ps2

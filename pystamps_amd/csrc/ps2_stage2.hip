@@ -1,0 +1,857 @@
+// pystamps_hip: stage-2 gamma loop on one MI355X (gfx950).
+// C ABI declared in include/pystamps_hip.h.
+#include "../../include/pystamps_hip.h"
+#include "ps2_host_util.hpp"
+#include "ps2_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace ps2 {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+#define PS2_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)); \
+            return PS2_E_DEVICE;                                                        \
+        }                                                                               \
+    } while (0)
+
+#define PS2_LAUNCHED()                                                                  \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string("kernel launch failed: ") + hipGetErrorString(e_));    \
+            return PS2_E_DEVICE;                                                        \
+        }                                                                               \
+    } while (0)
+
+static int arg_error(const std::string &msg) {
+    set_error(msg);
+    return PS2_E_ARG;
+}
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Device buffer that frees itself.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+        bytes = n;
+        if (n == 0) return PS2_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("device allocation of " + std::to_string(n) + " bytes failed: " +
+                      hipGetErrorString(e));
+            return PS2_E_NOMEM;
+        }
+        return PS2_OK;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+struct HostPinned {
+    void *p = nullptr;
+    ~HostPinned() {
+        if (p) (void)hipHostFree(p);
+    }
+    int alloc(size_t n) {
+        hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error(std::string("pinned allocation failed: ") + hipGetErrorString(e));
+            return PS2_E_NOMEM;
+        }
+        return PS2_OK;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+}  // namespace ps2
+
+using namespace ps2;
+
+struct ps2_session {
+    std::mutex mu;
+    int form = PS2_FORM_FP32;
+    int device = 0;
+    int verbose = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    // sizes / parameters
+    int64_t P = 0, I = 0, ni = 0, nj = 0, C = 0;
+    int tn = 0, T = 0, rowinv = 0;
+    int nwi = 0, nwj = 0;
+    bool windows = false;
+    double alpha = 1.0, beta = 0.3, nr_max_nz_ix = 1.0, conv = 1e-4;
+    int64_t lct = 31, max_iter = 25;
+    int weighting = PS2_WEIGHT_PSQUARE;
+    float rescore_bound = 0.0f;
+    std::vector<int64_t> order;  // sorted position k -> caller row
+    std::vector<double> nr_base;
+    HistRule hist_rule;
+    // device state
+    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, wfac, rng;
+    DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
+    DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
+    int n_rms_blocks = 0;
+    HostPinned h_rms, h_hist, h_cnt, h_prand;
+    // per-kernel timing (ps2_hip_session_set_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    std::vector<std::pair<int, int>> tpending;  // (slot, index of the start event)
+    int tused = 0;
+    double slot_ms[PS2_NUM_KERNEL_SLOTS] = {};
+    int64_t slot_launches[PS2_NUM_KERNEL_SLOTS] = {};
+    // results of the last run
+    bool ran = false;
+    int64_t i_loop = 0;
+    double gamma_change_save = 0.0, loop_seconds = 0.0;
+    std::vector<double> nr;
+    int64_t n_rescored = 0;
+
+    ~ps2_session() {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+            for (auto &e : ev)
+                if (e) (void)hipEventDestroy(e);
+            for (auto &e : tev)
+                if (e) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+    size_t rsz() const { return form == PS2_FORM_FP64 ? sizeof(double) : sizeof(float); }
+};
+
+namespace {
+
+std::mutex g_init_mu;
+
+int validate(const ps2_patch_in *in) {
+    if (!in) return arg_error("ps2_patch_in is NULL");
+    if (in->n_ps <= 0) return arg_error("ps1.mat missing valid n_ps");
+    if (in->n_ifg <= 0) return arg_error("n_ifg must be positive");
+    if (in->n_i <= 0 || in->n_j <= 0) return arg_error("grid extent must be positive");
+    if (!in->ph_nm || !in->grid_lin || !in->d_a || !in->low_pass || !in->trial_values ||
+        !in->nr_base || !in->coh_bins || !in->bperp_row)
+        return arg_error("a required input array is NULL");
+    if (!in->row_invariant && !in->bperp_mat)
+        return arg_error("bp1.bperp_mat is required for non-invariant stage-2 baselines");
+    if (in->n_win_ex != CLAP_FFT || in->clap_window != CLAP_WIN)
+        return arg_error("the HIP backend implements clap_win = 32 (24x24 windows, 32-point FFT); got n_win_ex=" +
+                         std::to_string(in->n_win_ex) + " clap_window=" + std::to_string(in->clap_window));
+    if (in->n_trial < 1 || (in->n_trial % 2) != 1)
+        return arg_error("trial_values must hold 2*tn+1 values");
+    const int64_t tn = in->n_trial / 2;
+    for (int64_t t = 0; t < in->n_trial; ++t)
+        if (in->trial_values[t] != (double)(t - tn))
+            return arg_error("trial_values must be the consecutive integers -tn..tn (trial_values(), native_stage2.rs:1305-1308)");
+    if (tn > 4096) return arg_error("trial count too large");
+    const int64_t C = in->n_i * in->n_j;
+    for (int64_t p = 0; p < in->n_ps; ++p)
+        if (in->grid_lin[p] < 0 || in->grid_lin[p] >= C)
+            return arg_error("grid_lin[" + std::to_string(p) + "] outside the n_i x n_j grid");
+    if (in->weighting == PS2_WEIGHT_SNR && !in->amp) return arg_error("amp is required for SNR weighting");
+    if (in->n_ps * in->n_ifg > (int64_t)1 << 31 || C * in->n_ifg > (int64_t)1 << 31)
+        return arg_error("patch too large for one session (n_ps*n_ifg and cells*n_ifg must stay below 2^31)");
+    return PS2_OK;
+}
+
+template <typename R>
+int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
+    const int64_t P = s->P, I = s->I;
+    // cell-sorted pixel order (stable counting sort of grid_lin)
+    std::vector<int32_t> cstart(s->C + 1, 0);
+    for (int64_t p = 0; p < P; ++p) cstart[in->grid_lin[p] + 1]++;
+    for (int64_t c = 0; c < s->C; ++c) cstart[c + 1] += cstart[c];
+    std::vector<int32_t> fillpos(cstart.begin(), cstart.end() - 1);
+    s->order.assign(P, 0);
+    std::vector<int32_t> cell_of(P);
+    for (int64_t p = 0; p < P; ++p) {
+        const int64_t c = in->grid_lin[p];
+        const int32_t k = fillpos[c]++;
+        s->order[k] = p;
+        cell_of[k] = (int32_t)c;
+    }
+    int rc;
+    if ((rc = s->cell_start.alloc(sizeof(int32_t) * (s->C + 1)))) return rc;
+    if ((rc = s->cell_of.alloc(sizeof(int32_t) * P))) return rc;
+    if ((rc = s->d_order.alloc(sizeof(int64_t) * P))) return rc;
+    PS2_HIP(hipMemcpy(s->cell_start.p, cstart.data(), sizeof(int32_t) * (s->C + 1), hipMemcpyHostToDevice));
+    PS2_HIP(hipMemcpy(s->cell_of.p, cell_of.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
+    PS2_HIP(hipMemcpy(s->d_order.p, s->order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice));
+
+    {   // ph_nm rows in sorted order
+        std::vector<cplx<R>> h((size_t)(P * I));
+        for (int64_t k = 0; k < P; ++k) {
+            const float *src = in->ph_nm + 2 * s->order[k] * I;
+            cplx<R> *dst = h.data() + k * I;
+            for (int64_t f = 0; f < I; ++f) dst[f] = {(R)src[2 * f], (R)src[2 * f + 1]};
+        }
+        if ((rc = s->phnm.alloc(sizeof(cplx<R>) * P * I))) return rc;
+        PS2_HIP(hipMemcpy(s->phnm.p, h.data(), sizeof(cplx<R>) * P * I, hipMemcpyHostToDevice));
+    }
+    {   // baselines: R copy for the bulk, f64 copy for the FP64 re-score
+        const int64_t n = s->rowinv ? I : P * I;
+        std::vector<R> h((size_t)n);
+        std::vector<double> h64((size_t)n);
+        if (s->rowinv) {
+            for (int64_t f = 0; f < I; ++f) {
+                h64[f] = in->bperp_row[f];
+                h[f] = (R)in->bperp_row[f];
+            }
+        } else {
+            for (int64_t k = 0; k < P; ++k) {
+                const double *src = in->bperp_mat + s->order[k] * I;
+                for (int64_t f = 0; f < I; ++f) {
+                    h64[k * I + f] = src[f];
+                    h[k * I + f] = (R)src[f];
+                }
+            }
+        }
+        if ((rc = s->bp.alloc(sizeof(R) * n))) return rc;
+        PS2_HIP(hipMemcpy(s->bp.p, h.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+        if (sizeof(R) == 4) {
+            if ((rc = s->bp64.alloc(sizeof(double) * n))) return rc;
+            PS2_HIP(hipMemcpy(s->bp64.p, h64.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        }
+    }
+    if (in->amp && s->weighting == PS2_WEIGHT_SNR) {
+        std::vector<float> h((size_t)(P * I));
+        for (int64_t k = 0; k < P; ++k)
+            std::memcpy(h.data() + k * I, in->amp + s->order[k] * I, sizeof(float) * I);
+        if ((rc = s->amp.alloc(sizeof(float) * P * I))) return rc;
+        PS2_HIP(hipMemcpy(s->amp.p, h.data(), sizeof(float) * P * I, hipMemcpyHostToDevice));
+    }
+    {   // initial weighting 1/D_A (0 where D_A == 0), native_stage2.rs:242-246
+        std::vector<double> w0(P);
+        for (int64_t k = 0; k < P; ++k) {
+            const double da = in->d_a[s->order[k]];
+            w0[k] = da != 0.0 ? 1.0 / da : 0.0;
+        }
+        if ((rc = s->W0.alloc(sizeof(double) * P))) return rc;
+        PS2_HIP(hipMemcpy(s->W0.p, w0.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    }
+    {   // CLAP constants: low_pass, window weight x 1/n^2
+        std::vector<R> lp(CLAP_FFT * CLAP_FFT);
+        for (int q = 0; q < CLAP_FFT * CLAP_FFT; ++q) lp[q] = (R)in->low_pass[q];
+        if ((rc = s->lp.alloc(sizeof(R) * lp.size()))) return rc;
+        PS2_HIP(hipMemcpy(s->lp.p, lp.data(), sizeof(R) * lp.size(), hipMemcpyHostToDevice));
+        // clap_window_weight (native_stage2.rs:1014-1037): r + c + 1e-6 on a
+        // quadrant, mirrored to the four quadrants
+        const int half = CLAP_WIN / 2;
+        const double inv_scale = 1.0 / (double)(CLAP_FFT * CLAP_FFT);
+        std::vector<R> wf(CLAP_WIN * CLAP_WIN);
+        for (int r = 0; r < CLAP_WIN; ++r)
+            for (int c = 0; c < CLAP_WIN; ++c) {
+                const int qr = r < half ? r : CLAP_WIN - 1 - r;
+                const int qc = c < half ? c : CLAP_WIN - 1 - c;
+                const double w = (double)qr + (double)qc + 1.0e-6;
+                wf[r * CLAP_WIN + c] = (R)(w * inv_scale);
+            }
+        if ((rc = s->wfac.alloc(sizeof(R) * wf.size()))) return rc;
+        PS2_HIP(hipMemcpy(s->wfac.p, wf.data(), sizeof(R) * wf.size(), hipMemcpyHostToDevice));
+    }
+    return PS2_OK;
+}
+
+template <typename R>
+int alloc_state(ps2_session *s) {
+    const int64_t P = s->P, I = s->I, C = s->C;
+    int rc;
+    if ((rc = s->grid.alloc(sizeof(cplx<R>) * C * I))) return rc;
+    if ((rc = s->filt.alloc(sizeof(cplx<R>) * C * I))) return rc;
+    if (s->windows)
+        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * CLAP_WIN * s->nj * I))) return rc;
+    if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
+    if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
+    if ((rc = s->rescore_list.alloc(sizeof(int32_t) * P))) return rc;
+    if ((rc = s->rescore_count.alloc(sizeof(uint32_t) * 4))) return rc;
+    if ((rc = s->hist.alloc(sizeof(uint32_t) * 128))) return rc;
+    for (DBuf *b : {&s->K, &s->Kprev, &s->W, &s->Wprev, &s->Cc, &s->coh, &s->cohprev, &s->nopt})
+        if ((rc = b->alloc(sizeof(double) * P))) return rc;
+    if ((rc = s->prand_hi.alloc(sizeof(double) * 2048))) return rc;
+    s->n_rms_blocks = (int)blocks_for(P, RMS_BLOCK * RMS_PER_THREAD);
+    if ((rc = s->rms_part.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
+    if ((rc = s->mids.alloc(sizeof(double) * 128))) return rc;
+    std::vector<double> mids(128, 0.0);
+    for (size_t k = 0; k < s->hist_rule.mids.size() && k < 128; ++k) mids[k] = s->hist_rule.mids[k];
+    PS2_HIP(hipMemcpy(s->mids.p, mids.data(), sizeof(double) * 128, hipMemcpyHostToDevice));
+    if ((rc = s->h_rms.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
+    if ((rc = s->h_hist.alloc(sizeof(uint32_t) * 128))) return rc;
+    if ((rc = s->h_cnt.alloc(sizeof(uint32_t) * 4))) return rc;
+    if ((rc = s->h_prand.alloc(sizeof(double) * 2048))) return rc;
+    return PS2_OK;
+}
+
+// --- per-kernel timing ---------------------------------------------------------
+
+enum Slot { SLOT_FILL = 0, SLOT_CLAP_STRIP, SLOT_CLAP_COMBINE, SLOT_SCAN, SLOT_RESCORE, SLOT_REFINE,
+            SLOT_RMS, SLOT_WEIGHTS };
+
+struct SlotTimer {
+    ps2_session *s;
+    bool on;
+    SlotTimer(ps2_session *s_, int slot) : s(s_), on(s_->timing && s_->tused + 2 <= (int)s_->tev.size()) {
+        if (!on) return;
+        s->tpending.push_back({slot, s->tused});
+        (void)hipEventRecord(s->tev[s->tused++], s->stream);
+    }
+    ~SlotTimer() {
+        if (on) (void)hipEventRecord(s->tev[s->tused++], s->stream);
+    }
+};
+
+// Accumulates the recorded pairs; the stream must be synchronised.
+void collect_timers(ps2_session *s) {
+    for (auto &pr : s->tpending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s->tev[pr.second], s->tev[pr.second + 1]) == hipSuccess) {
+            s->slot_ms[pr.first] += ms;
+            s->slot_launches[pr.first] += 1;
+        }
+    }
+    s->tpending.clear();
+    s->tused = 0;
+}
+
+// --- kernel launch helpers ---------------------------------------------------
+
+template <typename R>
+int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
+    const int64_t n = s->C * s->I;
+    SlotTimer tm(s, SLOT_FILL);
+    hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                       s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W, s->cell_start.as<int32_t>(),
+                       s->grid.as<cplx<R>>(), s->C, (int)s->I);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+template <typename R>
+ClapParams<R> clap_params(ps2_session *s, int teams) {
+    ClapParams<R> p;
+    const double sd = 6.0 / 5.0;  // (7-1)/(2*2.5)
+    for (int k = 0; k < 7; ++k) {
+        const double x = (double)k - 3.0;
+        p.k7[k] = (R)std::exp(-0.5 * (x / sd) * (x / sd));
+    }
+    p.alpha = (R)s->alpha;
+    p.beta = (R)s->beta;
+    p.alpha_is_one = std::fabs(s->alpha - 1.0) <= 2.220446049250313e-16;
+    p.ni = (int)s->ni;
+    p.nj = (int)s->nj;
+    p.I = (int)s->I;
+    p.nwi = s->nwi;
+    p.nwj = s->nwj;
+    p.ngroups = (int)((s->I + teams - 1) / teams);
+    return p;
+}
+
+template <typename R>
+int launch_clap(ps2_session *s) {
+    const int64_t n = s->C * s->I;
+    if (!s->windows) {
+        PS2_HIP(hipMemsetAsync(s->filt.p, 0, sizeof(cplx<R>) * n, s->stream));
+        return PS2_OK;
+    }
+    constexpr int TEAMS = sizeof(R) == 4 ? 4 : 2;
+    const ClapParams<R> prm = clap_params<R>(s, TEAMS);
+    {
+        SlotTimer tm(s, SLOT_CLAP_STRIP);
+        hipLaunchKernelGGL((k_clap_strip<R, TEAMS>), dim3(s->nwi * prm.ngroups), dim3(32 * TEAMS), 0,
+                           s->stream, s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
+                           s->wfac.as<R>(), prm);
+        PS2_LAUNCHED();
+    }
+    SlotTimer tm(s, SLOT_CLAP_COMBINE);
+    hipLaunchKernelGGL(k_clap_combine<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                       s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
+                       (int)s->I, s->nwi);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+FitParams fit_params(ps2_session *s) {
+    FitParams f;
+    f.P = s->P;
+    f.I = (int)s->I;
+    f.tn = s->tn;
+    f.T = s->T;
+    f.rowinv = s->rowinv;
+    f.rescore_bound = s->rescore_bound;
+    return f;
+}
+
+template <typename R>
+int launch_fit(ps2_session *s) {
+    const FitParams fp = fit_params(s);
+    const unsigned nb = blocks_for(s->P, 256);
+    if (sizeof(R) == 4) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4);
+        PS2_LAUNCHED();
+    }
+    {
+    SlotTimer tm(s, SLOT_SCAN);
+    if (s->tn < 8) {
+        hipLaunchKernelGGL((k_scan<R, 8>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
+                           s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->tix.as<int32_t>(), s->rng.as<R>(), s->rescore_list.as<int32_t>(),
+                           s->rescore_count.as<uint32_t>(), fp);
+    } else {
+        hipLaunchKernelGGL((k_scan<R, 16>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
+                           s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->tix.as<int32_t>(), s->rng.as<R>(), s->rescore_list.as<int32_t>(),
+                           s->rescore_count.as<uint32_t>(), fp);
+    }
+    PS2_LAUNCHED();
+    }
+    if (sizeof(R) == 4) {
+        SlotTimer tm(s, SLOT_RESCORE);
+        hipLaunchKernelGGL(k_rescore<R>, dim3(1024), dim3(64), 0, s->stream, s->phnm.as<cplx<R>>(),
+                           s->bp64.as<double>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(),
+                           s->tix.as<int32_t>(), fp);
+        PS2_LAUNCHED();
+    }
+    SlotTimer tm(s, SLOT_REFINE);
+    hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
+                       s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                       s->tix.as<int32_t>(), s->rng.as<R>(), s->K.as<double>(), s->Cc.as<double>(),
+                       s->coh.as<double>(), s->nopt.as<double>(), fp);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+int launch_rms_hist(ps2_session *s) {
+    HistParams hp;
+    hp.P = s->P;
+    hp.equal = s->hist_rule.equal ? 1 : 0;
+    hp.cutoff0 = s->hist_rule.cutoff0;
+    hp.d = s->hist_rule.d;
+    hp.nbins = PS2_COH_BINS;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s->stream, s->hist.as<uint32_t>(), 128);
+    PS2_LAUNCHED();
+    SlotTimer tm(s, SLOT_RMS);
+    hipLaunchKernelGGL(k_rms_hist, dim3(s->n_rms_blocks), dim3(RMS_BLOCK), 0, s->stream,
+                       s->coh.as<double>(), s->cohprev.as<double>(), s->rms_part.as<double>(),
+                       s->hist.as<uint32_t>(), s->mids.as<double>(), hp);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+template <typename R>
+int run_loop(ps2_session *s) {
+    const int64_t P = s->P;
+    int rc;
+    PS2_HIP(hipEventRecord(s->ev[0], s->stream));
+    // initial state (native_stage2.rs:242-258)
+    PS2_HIP(hipMemsetAsync(s->K.p, 0, sizeof(double) * P, s->stream));
+    PS2_HIP(hipMemsetAsync(s->cohprev.p, 0, sizeof(double) * P, s->stream));
+    PS2_HIP(hipMemcpyAsync(s->W.p, s->W0.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
+    std::vector<double> nr = s->nr_base;
+    double gcs = 0.0;
+    int64_t i_loop = 1;
+    const int64_t max_iter = std::max<int64_t>(1, s->max_iter);
+    int64_t rescored = 0;
+    for (;;) {
+        const auto t_it = std::chrono::steady_clock::now();
+        // K and weighting used by this fill (for the ph_weight output)
+        PS2_HIP(hipMemcpyAsync(s->Kprev.p, s->K.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
+        PS2_HIP(hipMemcpyAsync(s->Wprev.p, s->W.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
+        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
+        if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>()))) return rc;
+        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
+        if ((rc = launch_clap<R>(s))) return rc;
+        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
+        if ((rc = launch_fit<R>(s))) return rc;
+        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
+        if ((rc = launch_rms_hist(s))) return rc;
+        PS2_HIP(hipMemcpyAsync(s->h_rms.p, s->rms_part.p, sizeof(double) * s->n_rms_blocks,
+                               hipMemcpyDeviceToHost, s->stream));
+        PS2_HIP(hipMemcpyAsync(s->h_hist.p, s->hist.p, sizeof(uint32_t) * 128, hipMemcpyDeviceToHost, s->stream));
+        if (sizeof(R) == 4)
+            PS2_HIP(hipMemcpyAsync(s->h_cnt.p, s->rescore_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   s->stream));
+        PS2_HIP(hipStreamSynchronize(s->stream));
+        collect_timers(s);
+        if (sizeof(R) == 4) rescored += s->h_cnt.as<uint32_t>()[0];
+        // gamma-change RMS and stop test (native_stage2.rs:307-314)
+        double sum = 0.0;
+        for (int b = 0; b < s->n_rms_blocks; ++b) sum += s->h_rms.as<double>()[b];
+        const double rms = std::sqrt(sum / (double)std::max<int64_t>(P, 1));
+        const double change = rms - gcs;
+        gcs = rms;
+        const bool stop = std::fabs(change) < s->conv || i_loop >= max_iter;
+        if (s->verbose) {
+            float t_fill = 0, t_clap = 0, t_fit = 0;
+            (void)hipEventElapsedTime(&t_fill, s->ev[2], s->ev[3]);
+            (void)hipEventElapsedTime(&t_clap, s->ev[3], s->ev[4]);
+            (void)hipEventElapsedTime(&t_fit, s->ev[4], s->ev[5]);
+            const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
+            std::fprintf(stderr,
+                         "stage2 hip iteration %lld: fill+grid=%.3fms clap=%.3fms topofit=%.3fms "
+                         "wall=%.3fms gamma_change=%.6e gamma_delta=%.6e stop=%s\n",
+                         (long long)i_loop, t_fill, t_clap, t_fit, wall * 1e3, gcs, change,
+                         stop ? "true" : "false");
+        }
+        if (stop) break;
+        if (s->weighting == PS2_WEIGHT_PSQUARE) {
+            // native_stage2.rs:364-381
+            std::vector<double> na(PS2_COH_BINS);
+            for (int b = 0; b < PS2_COH_BINS; ++b) na[b] = (double)s->h_hist.as<uint32_t>()[b];
+            const int64_t lct = std::min<int64_t>(s->lct, PS2_COH_BINS);
+            double denom = 0.0, num = 0.0;
+            for (int64_t b = 0; b < lct; ++b) denom += nr[b];
+            for (int64_t b = 0; b < lct; ++b) num += na[b];
+            const double scale = denom > 0.0 ? num / denom : 1.0;
+            for (auto &v : nr) v *= scale;
+            const std::vector<double> hi = psquare_prand_hi(nr, na, s->lct, s->nr_max_nz_ix);
+            std::memcpy(s->h_prand.p, hi.data(), sizeof(double) * hi.size());
+            PS2_HIP(hipMemcpyAsync(s->prand_hi.p, s->h_prand.p, sizeof(double) * hi.size(),
+                                   hipMemcpyHostToDevice, s->stream));
+            SlotTimer tm(s, SLOT_WEIGHTS);
+            hipLaunchKernelGGL(k_weights_psquare, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
+                               s->coh.as<double>(), s->prand_hi.as<double>(), (int)hi.size(),
+                               s->W.as<double>(), P);
+            PS2_LAUNCHED();
+        } else {
+            SlotTimer tm(s, SLOT_WEIGHTS);
+            hipLaunchKernelGGL(k_weights_snr<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
+                               s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(),
+                               s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->K.as<double>(),
+                               s->nopt.as<double>(), s->W.as<double>(), P, (int)s->I, s->rowinv);
+            PS2_LAUNCHED();
+        }
+        ++i_loop;
+    }
+    PS2_HIP(hipEventRecord(s->ev[1], s->stream));
+    PS2_HIP(hipEventSynchronize(s->ev[1]));
+    collect_timers(s);
+    float ms = 0.0f;
+    PS2_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+    s->loop_seconds = (double)ms * 1e-3;
+    s->i_loop = i_loop;
+    s->gamma_change_save = gcs;
+    s->nr = nr;
+    s->n_rescored = rescored;
+    s->ran = true;
+    return PS2_OK;
+}
+
+// download helpers ------------------------------------------------------------
+
+int download_sorted_vec(ps2_session *s, const DBuf &src, double *dst) {
+    if (!dst) return PS2_OK;
+    std::vector<double> h(s->P);
+    PS2_HIP(hipMemcpy(h.data(), src.p, sizeof(double) * s->P, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < s->P; ++k) dst[s->order[k]] = h[k];
+    return PS2_OK;
+}
+
+template <typename R>
+int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
+    const int64_t P = s->P, I = s->I, C = s->C;
+    int rc;
+    if ((rc = download_sorted_vec(s, s->K, out->k_ps))) return rc;
+    if ((rc = download_sorted_vec(s, s->Cc, out->c_ps))) return rc;
+    if ((rc = download_sorted_vec(s, s->coh, out->coh_ps))) return rc;
+    if ((rc = download_sorted_vec(s, s->nopt, out->n_opt))) return rc;
+    if (out->trial_ix) {
+        std::vector<int32_t> h(P);
+        PS2_HIP(hipMemcpy(h.data(), s->tix.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < P; ++k) out->trial_ix[s->order[k]] = h[k];
+    }
+    if (out->ph_patch || out->ph_res) {
+        DBuf pbuf, rbuf;
+        if (out->ph_patch && (rc = pbuf.alloc(sizeof(float2) * P * I))) return rc;
+        if (out->ph_res && (rc = rbuf.alloc(sizeof(float) * P * I))) return rc;
+        hipLaunchKernelGGL(k_final_patch_res<R>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+                           s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
+                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
+                           s->d_order.as<int64_t>(), pbuf.as<float2>(), rbuf.as<float>(), P, (int)I,
+                           s->rowinv);
+        PS2_LAUNCHED();
+        PS2_HIP(hipStreamSynchronize(s->stream));
+        if (out->ph_patch)
+            PS2_HIP(hipMemcpy(out->ph_patch, pbuf.p, sizeof(float2) * P * I, hipMemcpyDeviceToHost));
+        if (out->ph_res) PS2_HIP(hipMemcpy(out->ph_res, rbuf.p, sizeof(float) * P * I, hipMemcpyDeviceToHost));
+    }
+    if (!loop_outputs) return PS2_OK;
+    if ((rc = download_sorted_vec(s, s->cohprev, out->coh_ps_save))) return rc;
+    if (out->ph_weight) {
+        DBuf wbuf;
+        if ((rc = wbuf.alloc(sizeof(float2) * P * I))) return rc;
+        hipLaunchKernelGGL(k_final_weight<R>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+                           s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->Kprev.as<double>(),
+                           s->Wprev.as<double>(), s->d_order.as<int64_t>(), wbuf.as<float2>(), P, (int)I);
+        PS2_LAUNCHED();
+        PS2_HIP(hipStreamSynchronize(s->stream));
+        PS2_HIP(hipMemcpy(out->ph_weight, wbuf.p, sizeof(float2) * P * I, hipMemcpyDeviceToHost));
+    }
+    if (out->ph_grid) {
+        if (sizeof(R) == 4) {
+            PS2_HIP(hipMemcpy(out->ph_grid, s->grid.p, sizeof(float2) * C * I, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<cplx<R>> h((size_t)(C * I));
+            PS2_HIP(hipMemcpy(h.data(), s->grid.p, sizeof(cplx<R>) * C * I, hipMemcpyDeviceToHost));
+            for (int64_t q = 0; q < C * I; ++q) {
+                out->ph_grid[2 * q] = (float)h[q].re;
+                out->ph_grid[2 * q + 1] = (float)h[q].im;
+            }
+        }
+    }
+    if (out->nr)
+        for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
+    out->gamma_change_save = s->gamma_change_save;
+    out->i_loop = s->i_loop;
+    out->n_rescored = s->n_rescored;
+    out->loop_seconds = s->loop_seconds;
+    return PS2_OK;
+}
+
+int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session **out) {
+    if (!out) return arg_error("session out pointer is NULL");
+    *out = nullptr;
+    int rc = validate(in);
+    if (rc) return rc;
+    std::unique_ptr<ps2_session> s(new ps2_session());
+    s->form = opts ? opts->form : PS2_FORM_FP32;
+    if (s->form != PS2_FORM_FP32 && s->form != PS2_FORM_FP64) return arg_error("unknown form");
+    s->device = opts ? opts->device : 0;
+    s->verbose = opts ? opts->verbose : 0;
+    {
+        std::lock_guard<std::mutex> lk(g_init_mu);
+        int n = 0;
+        PS2_HIP(hipGetDeviceCount(&n));
+        if (s->device < 0 || s->device >= n)
+            return arg_error("device " + std::to_string(s->device) + " not present (" + std::to_string(n) + " visible)");
+        PS2_HIP(hipSetDevice(s->device));
+        PS2_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto &e : s->ev) PS2_HIP(hipEventCreate(&e));
+    }
+    s->P = in->n_ps;
+    s->I = in->n_ifg;
+    s->ni = in->n_i;
+    s->nj = in->n_j;
+    s->C = in->n_i * in->n_j;
+    s->T = (int)in->n_trial;
+    s->tn = (int)(in->n_trial / 2);
+    s->rowinv = in->row_invariant ? 1 : 0;
+    s->alpha = in->clap_alpha;
+    s->beta = in->clap_beta;
+    s->nr_max_nz_ix = in->nr_max_nz_ix;
+    s->conv = in->gamma_change_convergence;
+    s->lct = in->low_coh_thresh;
+    s->max_iter = in->gamma_max_iterations;
+    s->weighting = in->weighting;
+    s->nr_base.assign(in->nr_base, in->nr_base + PS2_COH_BINS);
+    s->hist_rule = make_hist_rule(in->coh_bins, PS2_COH_BINS);
+    // window rows/columns (native_stage2.rs:754-759)
+    s->nwi = (int)std::ceil((double)s->ni / (double)CLAP_INC) - 3;
+    s->nwj = (int)std::ceil((double)s->nj / (double)CLAP_INC) - 3;
+    s->windows = s->nwi > 0 && s->nwj > 0;
+    if (s->windows && (s->ni < CLAP_WIN || s->nj < CLAP_WIN))
+        return arg_error("grid " + std::to_string(s->ni) + "x" + std::to_string(s->nj) +
+                         " has CLAP windows but is narrower than the 24-cell window (the reference's "
+                         "window placement underflows here)");
+    // derived FP32 re-score bound: the scan's worst-case rounding error on one
+    // coherence, doubled (recurrence drift over a 16-trial chunk + an I-term
+    // f32 sum + the sincos anchors), in units of 2^-24
+    if (opts && opts->rescore_bound > 0.0)
+        s->rescore_bound = (float)opts->rescore_bound;
+    else
+        s->rescore_bound = (float)(2.0 * (16.0 + 8.0 + 2.0 * (double)s->I) * 5.9604644775390625e-08);
+    if (s->form == PS2_FORM_FP64) {
+        if ((rc = upload_inputs<double>(s.get(), in))) return rc;
+        if ((rc = alloc_state<double>(s.get()))) return rc;
+    } else {
+        if ((rc = upload_inputs<float>(s.get(), in))) return rc;
+        if ((rc = alloc_state<float>(s.get()))) return rc;
+    }
+    *out = s.release();
+    return PS2_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+
+extern "C" {
+
+int ps2_hip_abi_version(void) { return PS2_HIP_ABI_VERSION; }
+
+const char *ps2_hip_last_error(void) { return g_last_error.c_str(); }
+
+int ps2_hip_device_count(void) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice) return 0;
+    if (e != hipSuccess) {
+        set_error(std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+        return PS2_E_DEVICE;
+    }
+    return n;
+}
+
+int ps2_hip_session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session **out) {
+    return session_create(in, opts, out);
+}
+
+int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_seconds) {
+    if (!s) return arg_error("session is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    PS2_HIP(hipSetDevice(s->device));
+    int rc = s->form == PS2_FORM_FP64 ? run_loop<double>(s) : run_loop<float>(s);
+    if (rc) return rc;
+    if (out_i_loop) *out_i_loop = s->i_loop;
+    if (out_loop_seconds) *out_loop_seconds = s->loop_seconds;
+    return PS2_OK;
+}
+
+int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out) {
+    if (!s || !out) return arg_error("session or output is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->ran) {
+        set_error("session_download before session_run");
+        return PS2_E_STATE;
+    }
+    PS2_HIP(hipSetDevice(s->device));
+    return s->form == PS2_FORM_FP64 ? download_outputs<double>(s, out, true)
+                                    : download_outputs<float>(s, out, true);
+}
+
+void ps2_hip_session_destroy(ps2_session *s) { delete s; }
+
+int ps2_hip_session_set_timing(ps2_session *s, int32_t enable) {
+    if (!s) return arg_error("session is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    PS2_HIP(hipSetDevice(s->device));
+    if (enable && s->tev.empty()) {
+        s->tev.resize(64, nullptr);
+        for (auto &e : s->tev) PS2_HIP(hipEventCreate(&e));
+    }
+    s->timing = enable != 0;
+    for (int k = 0; k < PS2_NUM_KERNEL_SLOTS; ++k) {
+        s->slot_ms[k] = 0.0;
+        s->slot_launches[k] = 0;
+    }
+    return PS2_OK;
+}
+
+int ps2_hip_session_kernel_times(ps2_session *s, double *ms, int64_t *launches) {
+    if (!s || !ms || !launches) return arg_error("NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int k = 0; k < PS2_NUM_KERNEL_SLOTS; ++k) {
+        ms[k] = s->slot_ms[k];
+        launches[k] = s->slot_launches[k];
+    }
+    return PS2_OK;
+}
+
+int ps2_hip_run(const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts) {
+    if (!out) return arg_error("ps2_patch_out is NULL");
+    ps2_session *s = nullptr;
+    int rc = session_create(in, opts, &s);
+    if (rc) return rc;
+    rc = ps2_hip_session_run(s, nullptr, nullptr);
+    if (!rc) rc = ps2_hip_session_download(s, out);
+    ps2_hip_session_destroy(s);
+    return rc;
+}
+
+int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts) {
+    if (n && (!in || !out)) return arg_error("batch arrays are NULL");
+    for (size_t q = 0; q < n; ++q) {
+        int rc = ps2_hip_run(&in[q], &out[q], opts);
+        if (rc) {
+            set_error("patch " + std::to_string(q) + ": " + g_last_error);
+            return rc;
+        }
+    }
+    return PS2_OK;
+}
+
+// --- single steps -------------------------------------------------------------
+
+int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *weighting, void *ph_grid_out) {
+    if (!s || !k_ps || !weighting || !ph_grid_out) return arg_error("NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    PS2_HIP(hipSetDevice(s->device));
+    std::vector<double> hk(s->P), hw(s->P);
+    for (int64_t k = 0; k < s->P; ++k) {
+        hk[k] = k_ps[s->order[k]];
+        hw[k] = weighting[s->order[k]];
+    }
+    PS2_HIP(hipMemcpy(s->K.p, hk.data(), sizeof(double) * s->P, hipMemcpyHostToDevice));
+    PS2_HIP(hipMemcpy(s->W.p, hw.data(), sizeof(double) * s->P, hipMemcpyHostToDevice));
+    int rc = s->form == PS2_FORM_FP64 ? launch_fill_grid<double>(s, s->K.as<double>(), s->W.as<double>())
+                                      : launch_fill_grid<float>(s, s->K.as<double>(), s->W.as<double>());
+    if (rc) return rc;
+    PS2_HIP(hipStreamSynchronize(s->stream));
+    PS2_HIP(hipMemcpy(ph_grid_out, s->grid.p, 2 * s->rsz() * s->C * s->I, hipMemcpyDeviceToHost));
+    return PS2_OK;
+}
+
+int ps2_hip_step_clap(ps2_session *s, const void *ph_grid, void *ph_filt_out) {
+    if (!s || !ph_grid || !ph_filt_out) return arg_error("NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    PS2_HIP(hipSetDevice(s->device));
+    PS2_HIP(hipMemcpy(s->grid.p, ph_grid, 2 * s->rsz() * s->C * s->I, hipMemcpyHostToDevice));
+    int rc = s->form == PS2_FORM_FP64 ? launch_clap<double>(s) : launch_clap<float>(s);
+    if (rc) return rc;
+    PS2_HIP(hipStreamSynchronize(s->stream));
+    PS2_HIP(hipMemcpy(ph_filt_out, s->filt.p, 2 * s->rsz() * s->C * s->I, hipMemcpyDeviceToHost));
+    return PS2_OK;
+}
+
+int ps2_hip_step_fit(ps2_session *s, const void *ph_filt, ps2_patch_out *out) {
+    if (!s || !ph_filt || !out) return arg_error("NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    PS2_HIP(hipSetDevice(s->device));
+    PS2_HIP(hipMemcpy(s->filt.p, ph_filt, 2 * s->rsz() * s->C * s->I, hipMemcpyHostToDevice));
+    int rc = s->form == PS2_FORM_FP64 ? launch_fit<double>(s) : launch_fit<float>(s);
+    if (rc) return rc;
+    uint32_t cnt = 0;
+    if (s->form == PS2_FORM_FP32)
+        PS2_HIP(hipMemcpyAsync(&cnt, s->rescore_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    PS2_HIP(hipStreamSynchronize(s->stream));
+    out->n_rescored = cnt;
+    return s->form == PS2_FORM_FP64 ? download_outputs<double>(s, out, false)
+                                    : download_outputs<float>(s, out, false);
+}
+
+}  // extern "C"

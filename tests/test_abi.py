@@ -1,0 +1,68 @@
+"""The C-ABI library loads, exports every symbol include/pystamps_hip.h
+declares, and rejects bad input on the host before touching a device.  CPU."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+from pystamps_amd import _native
+
+
+def test_library_exports_every_declared_function():
+    L = _native.lib()
+    names = _native.declared_functions()
+    assert len(names) == 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_native.LIB_PATH)],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for n in names:
+        assert n in exported
+    # nothing else with the ABI prefix is exported undeclared
+    extra = sorted(e for e in exported if e.startswith("ps2_hip_") and e not in names)
+    assert not extra, extra
+
+
+def test_abi_version_and_error_string():
+    L = _native.lib()
+    assert L.ps2_hip_abi_version() == _native.ABI_VERSION
+    assert isinstance(_native.last_error(), str)
+
+
+def test_device_count_is_nonnegative():
+    assert _native.device_count() >= 0
+
+
+def test_invalid_patch_rejected_on_host():
+    L = _native.lib()
+    inp = _native.PatchIn()
+    inp.n_ps = 0
+    out = _native.PatchOut()
+    opts = _native.RunOpts()
+    rc = L.ps2_hip_run(C.byref(inp), C.byref(out), C.byref(opts))
+    assert rc == -1
+    assert "n_ps" in _native.last_error()
+
+
+def test_null_arguments_rejected():
+    L = _native.lib()
+    assert L.ps2_hip_run(None, None, None) == -1
+    assert L.ps2_hip_session_run(None, None, None) == -1
+    nr = np.zeros(100)
+    mx = C.c_double()
+    assert L.ps2_hip_random_coherence_histogram(None, 0, None, 0, None, 0, 0, 0,
+                                                nr.ctypes.data_as(C.POINTER(C.c_double)),
+                                                C.byref(mx)) == -1
+
+
+def test_header_structs_match_ctypes_layout():
+    # the ctypes mirror names every header member, in the header's order
+    names = [f[0] for f in _native.PatchIn._fields_]
+    text = _native.HEADER.read_text()
+    start = text.index("typedef struct {", text.index("Prepared per-patch inputs"))
+    body = text[start:text.index("} ps2_patch_in;")]
+    pos = [body.find(n) for n in names]
+    assert all(p >= 0 for p in pos), [n for n, p in zip(names, pos) if p < 0]
+    assert pos == sorted(pos)
