@@ -5,10 +5,13 @@
 //   pixels are stored in cell-sorted order k (stable sort of grid_lin), so the
 //   pixels of one grid cell are contiguous and each cell is a CSR range
 //   cell_start[c] .. cell_start[c+1];
-//   phnm  [P][I] complex R   ph_nm rows in sorted order
-//   bp    [P][I] R           bperp_mat rows (or [I] when the rows are invariant)
-//   grid, filt [C][I] complex R  (cell-major, the reference's ph_grid layout)
-//   part  [nwi][24][n_j][I] complex R  per-window-row CLAP partial sums
+//   every per-(pixel, ifg) and per-(cell, ifg) array is ifg-major, one plane
+//   per interferogram, so a wavefront of consecutive pixels (or cells) reads
+//   consecutive addresses:
+//   phnm  [I][P] complex R   ph_nm in sorted pixel order
+//   bp    [I][P] R           bperp_mat (or [I] when the rows are invariant)
+//   grid, filt [I][C] complex R
+//   part  [I][nwi][24][n_j] complex R  per-window-row CLAP partial sums
 #pragma once
 
 #include "ps2_device.hpp"
@@ -34,7 +37,7 @@ __device__ inline cplx<R> weight_value(cplx<R> s, R b, double K, double W) {
 }
 
 // fill_phase_weight + accumulate_grid (native_stage2.rs:629-657, 722-740):
-// one thread per (cell, ifg); the cell's pixels are summed in ascending row
+// one thread per (ifg, cell); the cell's pixels are summed in ascending row
 // order, which is the reference's sequential order, so no atomics are needed.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ phnm,
@@ -42,15 +45,16 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
                                                    const double *__restrict__ K,
                                                    const double *__restrict__ W,
                                                    const int32_t *__restrict__ cell_start,
-                                                   cplx<R> *__restrict__ grid, int64_t C, int I) {
+                                                   cplx<R> *__restrict__ grid, int64_t C, int I,
+                                                   int64_t P) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= C * I) return;
-    const int64_t c = gid / I;
-    const int f = (int)(gid - c * I);
+    const int f = (int)(gid / C);
+    const int64_t c = gid - (int64_t)f * C;
     const int32_t k0 = cell_start[c], k1 = cell_start[c + 1];
     R sre = (R)0, sim = (R)0;
     for (int32_t k = k0; k < k1; ++k) {
-        const int64_t e = (int64_t)k * I + f;
+        const int64_t e = (int64_t)f * P + k;
         const R b = bp_rowinv ? bp[f] : bp[e];
         const cplx<R> v = weight_value<R>(phnm[e], b, K[k], W[k]);
         sre += v.re;
@@ -69,8 +73,8 @@ __global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict_
                                                       float2 *__restrict__ out, int64_t P, int I) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= P * I) return;
-    const int64_t k = gid / I;
-    const int f = (int)(gid - k * I);
+    const int f = (int)(gid / P);
+    const int64_t k = gid - (int64_t)f * P;
     const R b = bp_rowinv ? bp[f] : bp[gid];
     const cplx<R> v = weight_value<R>(phnm[gid], b, K[k], W[k]);
     out[order[k] * I + f] = make_float2((float)v.re, (float)v.im);
@@ -162,10 +166,10 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
         R xr[32], xi[32];
         // window row `lane`, zero padded to 32 (NaN -> 0, native_stage2.rs:852-858)
         if (lane < CLAP_WIN && active) {
-            const cplx<R> *src = grid + ((int64_t)(i1 + lane) * nj + j1) * I + f;
+            const cplx<R> *src = grid + (int64_t)f * ni * nj + (int64_t)(i1 + lane) * nj + j1;
 #pragma unroll
             for (int c = 0; c < CLAP_WIN; ++c) {
-                cplx<R> v = src[(int64_t)c * I];
+                cplx<R> v = src[c];
                 if (RT<R>::isnan_(v.re) || RT<R>::isnan_(v.im)) v = {(R)0, (R)0};
                 xr[c] = v.re;
                 xi[c] = v.im;
@@ -267,10 +271,10 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
                 // flush the columns no later window of this row touches
                 const int cbeg = jx * CLAP_INC;
                 const int cend = (jx == prm.nwj - 1) ? nj : cbeg + CLAP_INC;
-                cplx<R> *dst = part + (((int64_t)ix * CLAP_WIN + lr) * nj) * I + f;
+                cplx<R> *dst = part + (((int64_t)f * prm.nwi + ix) * CLAP_WIN + lr) * nj;
                 for (int col = cbeg; col < cend; ++col) {
                     const int slot = col % CLAP_WIN;
-                    dst[(int64_t)col * I] = acc[team][lr][slot];
+                    dst[col] = acc[team][lr][slot];
                     acc[team][lr][slot] = {(R)0, (R)0};
                 }
             }
@@ -286,8 +290,8 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t C = (int64_t)ni * nj;
     if (gid >= C * I) return;
-    const int64_t cell = gid / I;
-    const int f = (int)(gid - cell * I);
+    const int f = (int)(gid / C);
+    const int64_t cell = gid - (int64_t)f * C;
     const int r = (int)(cell / nj), c = (int)(cell - (int64_t)r * nj);
     int ix_hi = r / CLAP_INC;
     if (ix_hi > nwi - 1) ix_hi = nwi - 1;
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
     R sre = (R)0, sim = (R)0;
     for (int ix = ix_lo; ix <= ix_hi; ++ix) {
         const int lr = r - ix * CLAP_INC;
-        const cplx<R> v = part[(((int64_t)ix * CLAP_WIN + lr) * nj + c) * I + f];
+        const cplx<R> v = part[(((int64_t)f * nwi + ix) * CLAP_WIN + lr) * nj + c];
         sre += v.re;
         sim += v.im;
     }
@@ -332,7 +336,7 @@ __device__ inline cplx<R> rotate_neg(cplx<R> v, R phase) {
 }
 
 struct FitParams {
-    int64_t P;
+    int64_t P, C;
     int I, tn, T, rowinv;
     float rescore_bound;  // FP32 form only
 };
@@ -359,18 +363,21 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= prm.P) return;
     const int I = prm.I;
-    const cplx<R> *nm = phnm + k * I;
-    const cplx<R> *fl = filt + (int64_t)cell_of[k] * I;
-    const R *b = prm.rowinv ? bp : bp + k * I;
+    const int64_t P = prm.P;
+    const cplx<R> *nm = phnm + k;                      // nm[i * P]
+    const cplx<R> *fl = filt + cell_of[k];             // fl[i * C]
+    const R *b = prm.rowinv ? bp : bp + k;
+    const int64_t bs = prm.rowinv ? 0 : P;            // b[i * bs] (rowinv: b[i])
+    const int64_t C = prm.C;
 
     bool valid = true;
     R denom = (R)0;
     R mn = (R)INFINITY, mx = -(R)INFINITY;
     for (int i = 0; i < I; ++i) {
-        const cplx<R> v = psdph_value<R>(fl[i], nm[i]);
+        const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
         if (v.re == (R)0 && v.im == (R)0) valid = false;
         denom += RT<R>::hypot_(v.re, v.im);
-        const R bi = b[i];
+        const R bi = prm.rowinv ? b[i] : b[i * bs];
         if (bi < mn) mn = bi;
         if (bi > mx) mx = bi;
     }
@@ -404,8 +411,9 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
 #pragma unroll
         for (int q = 0; q < TC; ++q) pre[q] = pim[q] = nre[q] = nim[q] = (R)0;
         for (int i = 0; i < I; ++i) {
-            const cplx<R> v = psdph_value<R>(fl[i], nm[i]);
-            const R phi = (b[i] / range) * pi4;
+            const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
+            const R bi = prm.rowinv ? b[i] : b[i * bs];
+            const R phi = (bi / range) * pi4;
             if constexpr (sizeof(R) == 4) {
                 R s1, c1;
                 RT<R>::sincos_(phi, &s1, &c1);
@@ -486,15 +494,17 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
     for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
         const int64_t k = rescore_list[q];
         const int I = prm.I;
-        const cplx<RF> *nm = phnm + k * I;
-        const cplx<RF> *fl = filt + (int64_t)cell_of[k] * I;
-        const double *b = prm.rowinv ? bp64 : bp64 + k * I;
+        const int64_t P = prm.P, C = prm.C;
+        const cplx<RF> *nm = phnm + k;
+        const cplx<RF> *fl = filt + cell_of[k];
+        auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
         double denom = 0.0, mn = INFINITY, mx = -INFINITY;
         for (int i = 0; i < I; ++i) {
-            const cplx<RF> v = psdph_value<RF>(fl[i], nm[i]);
+            const cplx<RF> v = psdph_value<RF>(fl[i * C], nm[i * P]);
             denom += hypot((double)v.re, (double)v.im);
-            if (b[i] < mn) mn = b[i];
-            if (b[i] > mx) mx = b[i];
+            const double bi = bat(i);
+            if (bi < mn) mn = bi;
+            if (bi > mx) mx = bi;
         }
         if (denom == 0.0) denom = 1.0;
         double range = mx - mn;
@@ -505,8 +515,8 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
             const double tv = (double)(t - prm.tn);
             double sr = 0.0, si = 0.0;
             for (int i = 0; i < I; ++i) {
-                const cplx<RF> v = psdph_value<RF>(fl[i], nm[i]);
-                const double ph = ((b[i] / range) * 0.78539816339744830962) * tv;
+                const cplx<RF> v = psdph_value<RF>(fl[i * C], nm[i * P]);
+                const double ph = ((bat(i) / range) * 0.78539816339744830962) * tv;
                 double sn, cs;
                 sincos(ph, &sn, &cs);
                 const double nsn = -sn;
@@ -557,25 +567,27 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
         return;
     }
     const int I = prm.I;
-    const cplx<R> *nm = phnm + k * I;
-    const cplx<R> *fl = filt + (int64_t)cell_of[k] * I;
-    const R *b = prm.rowinv ? bp : bp + k * I;
+    const int64_t P = prm.P, C = prm.C;
+    const cplx<R> *nm = phnm + k;
+    const cplx<R> *fl = filt + cell_of[k];
+    auto bat = [&](int i) { return prm.rowinv ? bp[i] : bp[(int64_t)i * P + k]; };
     const R range = rng[k];
     R k0 = (R)(RT<R>::pi4() / range * (R)(t - prm.tn));
     R ore = (R)0, oim = (R)0;
     for (int i = 0; i < I; ++i) {
-        const cplx<R> a = rotate_neg<R>(psdph_value<R>(fl[i], nm[i]), k0 * b[i]);
+        const cplx<R> a = rotate_neg<R>(psdph_value<R>(fl[i * C], nm[i * P]), k0 * bat(i));
         ore += a.re;
         oim += a.im;
     }
     const R ocr = ore, oci = -oim;
     double num = 0.0, den = 0.0;
     for (int i = 0; i < I; ++i) {
-        const cplx<R> v = psdph_value<R>(fl[i], nm[i]);
+        const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
+        const R bi = bat(i);
         const double wgt = (double)RT<R>::hypot_(v.re, v.im);
-        const double wb = wgt * (double)b[i];
+        const double wb = wgt * (double)bi;
         den += wb * wb;
-        const cplx<R> r = rotate_neg<R>(v, k0 * b[i]);
+        const cplx<R> r = rotate_neg<R>(v, k0 * bi);
         const R pr = r.re * ocr - r.im * oci, pim = r.re * oci + r.im * ocr;
         num += wb * (wgt * (double)RT<R>::atan2_(pim, pr));
     }
@@ -583,7 +595,7 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     k0 = (R)((double)k0 + num / den);
     R mre = (R)0, mim = (R)0, dn = (R)0;
     for (int i = 0; i < I; ++i) {
-        const cplx<R> r = rotate_neg<R>(psdph_value<R>(fl[i], nm[i]), k0 * b[i]);
+        const cplx<R> r = rotate_neg<R>(psdph_value<R>(fl[i * C], nm[i * P]), k0 * bat(i));
         mre += r.re;
         mim += r.im;
         dn += RT<R>::hypot_(r.re, r.im);
@@ -607,12 +619,12 @@ __global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restri
                                                          const int64_t *__restrict__ order,
                                                          float2 *__restrict__ patch_out,
                                                          float *__restrict__ res_out, int64_t P,
-                                                         int I, int rowinv) {
+                                                         int I, int rowinv, int64_t C) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= P * I) return;
-    const int64_t k = gid / I;
-    const int f = (int)(gid - k * I);
-    const cplx<R> fv = filt[(int64_t)cell_of[k] * I + f];
+    const int f = (int)(gid / P);
+    const int64_t k = gid - (int64_t)f * P;
+    const cplx<R> fv = filt[(int64_t)f * C + cell_of[k]];
     const cplx<R> p = unit_phase<R>(fv);
     const int64_t o = order[k] * I + f;
     if (patch_out) patch_out[o] = make_float2((float)p.re, (float)p.im);
@@ -723,17 +735,17 @@ __global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__
                                                      const double *__restrict__ K,
                                                      const double *__restrict__ nopt,
                                                      double *__restrict__ W, int64_t P, int I,
-                                                     int rowinv) {
+                                                     int rowinv, int64_t C) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P) return;
     double g = 0.0, a2 = 0.0;
     const bool valid = nopt[k] == 1.0;
     for (int f = 0; f < I; ++f) {
-        const int64_t e = k * I + f;
+        const int64_t e = (int64_t)f * P + k;
         const double a = (double)amp[e];
         double pr = 0.0;
         if (valid) {
-            const cplx<R> v = psdph_value<R>(filt[(int64_t)cell_of[k] * I + f], phnm[e]);
+            const cplx<R> v = psdph_value<R>(filt[(int64_t)f * C + cell_of[k]], phnm[e]);
             const R b = rowinv ? bp[f] : bp[e];
             const cplx<R> r = rotate_neg<R>(v, (R)K[k] * b);
             pr = (double)(R)RT<R>::atan2_(r.im, r.re);

@@ -213,12 +213,11 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     PS2_HIP(hipMemcpy(s->cell_of.p, cell_of.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
     PS2_HIP(hipMemcpy(s->d_order.p, s->order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice));
 
-    {   // ph_nm rows in sorted order
+    {   // ph_nm planes [I][P] in sorted pixel order
         std::vector<cplx<R>> h((size_t)(P * I));
         for (int64_t k = 0; k < P; ++k) {
             const float *src = in->ph_nm + 2 * s->order[k] * I;
-            cplx<R> *dst = h.data() + k * I;
-            for (int64_t f = 0; f < I; ++f) dst[f] = {(R)src[2 * f], (R)src[2 * f + 1]};
+            for (int64_t f = 0; f < I; ++f) h[f * P + k] = {(R)src[2 * f], (R)src[2 * f + 1]};
         }
         if ((rc = s->phnm.alloc(sizeof(cplx<R>) * P * I))) return rc;
         PS2_HIP(hipMemcpy(s->phnm.p, h.data(), sizeof(cplx<R>) * P * I, hipMemcpyHostToDevice));
@@ -236,8 +235,8 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
             for (int64_t k = 0; k < P; ++k) {
                 const double *src = in->bperp_mat + s->order[k] * I;
                 for (int64_t f = 0; f < I; ++f) {
-                    h64[k * I + f] = src[f];
-                    h[k * I + f] = (R)src[f];
+                    h64[f * P + k] = src[f];
+                    h[f * P + k] = (R)src[f];
                 }
             }
         }
@@ -251,7 +250,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     if (in->amp && s->weighting == PS2_WEIGHT_SNR) {
         std::vector<float> h((size_t)(P * I));
         for (int64_t k = 0; k < P; ++k)
-            std::memcpy(h.data() + k * I, in->amp + s->order[k] * I, sizeof(float) * I);
+            for (int64_t f = 0; f < I; ++f) h[f * P + k] = in->amp[s->order[k] * I + f];
         if ((rc = s->amp.alloc(sizeof(float) * P * I))) return rc;
         PS2_HIP(hipMemcpy(s->amp.p, h.data(), sizeof(float) * P * I, hipMemcpyHostToDevice));
     }
@@ -355,7 +354,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
     SlotTimer tm(s, SLOT_FILL);
     hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                        s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W, s->cell_start.as<int32_t>(),
-                       s->grid.as<cplx<R>>(), s->C, (int)s->I);
+                       s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -407,6 +406,7 @@ int launch_clap(ps2_session *s) {
 FitParams fit_params(ps2_session *s) {
     FitParams f;
     f.P = s->P;
+    f.C = s->C;
     f.I = (int)s->I;
     f.tn = s->tn;
     f.T = s->T;
@@ -552,7 +552,7 @@ int run_loop(ps2_session *s) {
             hipLaunchKernelGGL(k_weights_snr<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
                                s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(),
                                s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->K.as<double>(),
-                               s->nopt.as<double>(), s->W.as<double>(), P, (int)s->I, s->rowinv);
+                               s->nopt.as<double>(), s->W.as<double>(), P, (int)s->I, s->rowinv, s->C);
             PS2_LAUNCHED();
         }
         ++i_loop;
@@ -602,7 +602,7 @@ int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
                            s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
                            s->d_order.as<int64_t>(), pbuf.as<float2>(), rbuf.as<float>(), P, (int)I,
-                           s->rowinv);
+                           s->rowinv, s->C);
         PS2_LAUNCHED();
         PS2_HIP(hipStreamSynchronize(s->stream));
         if (out->ph_patch)
@@ -621,17 +621,14 @@ int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
         PS2_HIP(hipStreamSynchronize(s->stream));
         PS2_HIP(hipMemcpy(out->ph_weight, wbuf.p, sizeof(float2) * P * I, hipMemcpyDeviceToHost));
     }
-    if (out->ph_grid) {
-        if (sizeof(R) == 4) {
-            PS2_HIP(hipMemcpy(out->ph_grid, s->grid.p, sizeof(float2) * C * I, hipMemcpyDeviceToHost));
-        } else {
-            std::vector<cplx<R>> h((size_t)(C * I));
-            PS2_HIP(hipMemcpy(h.data(), s->grid.p, sizeof(cplx<R>) * C * I, hipMemcpyDeviceToHost));
-            for (int64_t q = 0; q < C * I; ++q) {
-                out->ph_grid[2 * q] = (float)h[q].re;
-                out->ph_grid[2 * q + 1] = (float)h[q].im;
+    if (out->ph_grid) {   // device [I][C] -> caller [C][I]
+        std::vector<cplx<R>> h((size_t)(C * I));
+        PS2_HIP(hipMemcpy(h.data(), s->grid.p, sizeof(cplx<R>) * C * I, hipMemcpyDeviceToHost));
+        for (int64_t f = 0; f < I; ++f)
+            for (int64_t c = 0; c < C; ++c) {
+                out->ph_grid[2 * (c * I + f)] = (float)h[f * C + c].re;
+                out->ph_grid[2 * (c * I + f) + 1] = (float)h[f * C + c].im;
             }
-        }
     }
     if (out->nr)
         for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
@@ -807,6 +804,29 @@ int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out, cons
 
 // --- single steps -------------------------------------------------------------
 
+// host copy between the caller's [C][I] complex layout and the device's [I][C]
+static int plane_upload(ps2_session *s, void *dev, const void *host) {
+    const int64_t C = s->C, I = s->I;
+    const size_t rs = s->rsz();
+    std::vector<char> t((size_t)(2 * rs * C * I));
+    for (int64_t c = 0; c < C; ++c)
+        for (int64_t f = 0; f < I; ++f)
+            std::memcpy(&t[2 * rs * (f * C + c)], (const char *)host + 2 * rs * (c * I + f), 2 * rs);
+    PS2_HIP(hipMemcpy(dev, t.data(), t.size(), hipMemcpyHostToDevice));
+    return PS2_OK;
+}
+
+static int plane_download(ps2_session *s, void *host, const void *dev) {
+    const int64_t C = s->C, I = s->I;
+    const size_t rs = s->rsz();
+    std::vector<char> t((size_t)(2 * rs * C * I));
+    PS2_HIP(hipMemcpy(t.data(), dev, t.size(), hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < C; ++c)
+        for (int64_t f = 0; f < I; ++f)
+            std::memcpy((char *)host + 2 * rs * (c * I + f), &t[2 * rs * (f * C + c)], 2 * rs);
+    return PS2_OK;
+}
+
 int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *weighting, void *ph_grid_out) {
     if (!s || !k_ps || !weighting || !ph_grid_out) return arg_error("NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
@@ -822,28 +842,28 @@ int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *wei
                                       : launch_fill_grid<float>(s, s->K.as<double>(), s->W.as<double>());
     if (rc) return rc;
     PS2_HIP(hipStreamSynchronize(s->stream));
-    PS2_HIP(hipMemcpy(ph_grid_out, s->grid.p, 2 * s->rsz() * s->C * s->I, hipMemcpyDeviceToHost));
-    return PS2_OK;
+    return plane_download(s, ph_grid_out, s->grid.p);
 }
 
 int ps2_hip_step_clap(ps2_session *s, const void *ph_grid, void *ph_filt_out) {
     if (!s || !ph_grid || !ph_filt_out) return arg_error("NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     PS2_HIP(hipSetDevice(s->device));
-    PS2_HIP(hipMemcpy(s->grid.p, ph_grid, 2 * s->rsz() * s->C * s->I, hipMemcpyHostToDevice));
-    int rc = s->form == PS2_FORM_FP64 ? launch_clap<double>(s) : launch_clap<float>(s);
+    int rc = plane_upload(s, s->grid.p, ph_grid);
+    if (rc) return rc;
+    rc = s->form == PS2_FORM_FP64 ? launch_clap<double>(s) : launch_clap<float>(s);
     if (rc) return rc;
     PS2_HIP(hipStreamSynchronize(s->stream));
-    PS2_HIP(hipMemcpy(ph_filt_out, s->filt.p, 2 * s->rsz() * s->C * s->I, hipMemcpyDeviceToHost));
-    return PS2_OK;
+    return plane_download(s, ph_filt_out, s->filt.p);
 }
 
 int ps2_hip_step_fit(ps2_session *s, const void *ph_filt, ps2_patch_out *out) {
     if (!s || !ph_filt || !out) return arg_error("NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
     PS2_HIP(hipSetDevice(s->device));
-    PS2_HIP(hipMemcpy(s->filt.p, ph_filt, 2 * s->rsz() * s->C * s->I, hipMemcpyHostToDevice));
-    int rc = s->form == PS2_FORM_FP64 ? launch_fit<double>(s) : launch_fit<float>(s);
+    int rc = plane_upload(s, s->filt.p, ph_filt);
+    if (rc) return rc;
+    rc = s->form == PS2_FORM_FP64 ? launch_fit<double>(s) : launch_fit<float>(s);
     if (rc) return rc;
     uint32_t cnt = 0;
     if (s->form == PS2_FORM_FP32)
