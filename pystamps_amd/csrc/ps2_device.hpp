@@ -41,6 +41,13 @@ struct RT<float> {
         double dx = (double)x, dy = (double)y;
         return (float)__builtin_sqrt(dx * dx + dy * dy);
     }
+    // |z| where glibc fidelity is not needed (CLAP spectrum magnitude)
+    __device__ static inline float abs_(float x, float y) {
+        return __builtin_amdgcn_sqrtf(fmaf(x, x, y * y));
+    }
+    // CLAP gain normalisation h / median: one reciprocal per window
+    __device__ static inline float gain_scale(float med) { return med != 0.0f ? 1.0f / med : 1.0f; }
+    __device__ static inline float gain_div(float h, float, float scl) { return h * scl; }
     __device__ static inline key_t key(float v) { return __float_as_uint(v); }
     __device__ static inline float from_key(key_t k) { return __uint_as_float(k); }
     __device__ static inline bool isnan_(float v) { return __builtin_isnan(v); }
@@ -56,6 +63,11 @@ struct RT<double> {
     __device__ static inline double sqrt_(double x) { return sqrt(x); }
     __device__ static inline double pow_(double x, double y) { return pow(x, y); }
     __device__ static inline double hypot_(double x, double y) { return hypot(x, y); }
+    __device__ static inline double abs_(double x, double y) { return hypot(x, y); }
+    __device__ static inline double gain_scale(double) { return 1.0; }
+    __device__ static inline double gain_div(double h, double med, double) {
+        return med != 0.0 ? h / med : h;
+    }
     __device__ static inline key_t key(double v) { return (uint64_t)__double_as_longlong(v); }
     __device__ static inline double from_key(key_t k) { return __longlong_as_double((long long)k); }
     __device__ static inline bool isnan_(double v) { return __builtin_isnan(v); }

@@ -63,6 +63,61 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
     grid[gid] = {sre, sim};
 }
 
+// The same step with the phase weights computed cooperatively: a workgroup
+// covers FILL_CB consecutive cells and FILL_FG interferogram planes; the
+// weights of the cells' pixels (a contiguous sorted-pixel range) are computed
+// with coalesced loads into LDS, then each thread adds its cell's weights in
+// ascending row order, so the sums are the same as k_fill_grid's.
+constexpr int FILL_CB = 64;
+constexpr int FILL_FG = 4;
+constexpr int FILL_CHUNK = 1024;
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restrict__ phnm,
+                                                          const R *__restrict__ bp, int bp_rowinv,
+                                                          const double *__restrict__ K,
+                                                          const double *__restrict__ W,
+                                                          const int32_t *__restrict__ cell_start,
+                                                          cplx<R> *__restrict__ grid, int64_t C,
+                                                          int I, int64_t P) {
+    __shared__ cplx<R> vals[FILL_FG][FILL_CHUNK];
+    const int t = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * FILL_CB;
+    const int f0 = blockIdx.y * FILL_FG;
+    const int64_t cend = c0 + FILL_CB < C ? c0 + FILL_CB : C;
+    const int32_t ks = cell_start[c0], ke = cell_start[cend];
+    const int64_t c = c0 + (t % FILL_CB);
+    const int fs = t / FILL_CB;
+    const int f = f0 + fs;
+    const bool mine = c < C && f < I;
+    const int32_t my0 = mine ? cell_start[c] : 0, my1 = mine ? cell_start[c + 1] : 0;
+    R sre = (R)0, sim = (R)0;
+    for (int32_t base = ks; base < ke; base += FILL_CHUNK) {
+        const int n = ke - base < FILL_CHUNK ? ke - base : FILL_CHUNK;
+#pragma unroll
+        for (int q = 0; q < FILL_FG; ++q) {
+            const int ff = f0 + q;
+            if (ff >= I) break;
+            for (int j = t; j < n; j += 256) {
+                const int64_t k = base + j;
+                const int64_t e = (int64_t)ff * P + k;
+                const R b = bp_rowinv ? bp[ff] : bp[e];
+                vals[q][j] = weight_value<R>(phnm[e], b, K[k], W[k]);
+            }
+        }
+        __syncthreads();
+        const int32_t lo = my0 > base ? my0 : base;
+        const int32_t hi = my1 < base + n ? my1 : base + n;
+        for (int32_t k = lo; k < hi; ++k) {
+            const cplx<R> v = vals[fs][k - base];
+            sre += v.re;
+            sim += v.im;
+        }
+        __syncthreads();
+    }
+    if (mine) grid[(int64_t)f * C + c] = {sre, sim};
+}
+
 // ph_weight output of the last fill, written in the caller's row order.
 template <typename R>
 __global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict__ phnm,
@@ -100,31 +155,60 @@ struct ClapParams {
     int ni, nj, I, nwi, nwj, ngroups;
 };
 
-// median of the 1024 non-negative values held 32 per lane by a team
-// (native_stage2.rs:1168-1184: sort, mean of the two middle values).
-template <typename R>
-__device__ inline R team_median1024(const R (&h)[32]) {
-    using K = typename RT<R>::key_t;
-    constexpr K signmask = ~(K(1) << (sizeof(K) * 8 - 1));
-    K keys[32];
+// Wave-level LDS synchronisation: a team is half a wavefront and never shares
+// LDS with another wavefront, and LDS operations of one wavefront complete in
+// issue order, so ordering the compiler's memory operations at wavefront scope
+// is enough between a team's LDS writes and its reads.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Count over the calling lane's team (32 lanes) of keys[r] <= t, with t uniform
+// within the team.  One ballot per register; the popcounts run on the scalar
+// unit.
+template <typename K>
+__device__ inline int team_count_le(const K (&keys)[32], K t, bool hi_team) {
+    int lo = 0, hi = 0;
 #pragma unroll
-    for (int r = 0; r < 32; ++r) keys[r] = RT<R>::key(h[r]) & signmask;
-    // smallest v with count(x <= v) >= 512 is the 0-based 511th value
-    K ans = 0;
-    for (int bit = RT<R>::key_top_bit; bit >= 0; --bit) {
-        const K t = ans | ((K(1) << bit) - 1);
-        int cnt = 0;
-#pragma unroll
-        for (int r = 0; r < 32; ++r) cnt += keys[r] <= t ? 1 : 0;
-        cnt = team_sum_i32(cnt);
-        if (cnt < 512) ans |= K(1) << bit;
+    for (int r = 0; r < 32; ++r) {
+        const unsigned long long m = __ballot(keys[r] <= t);
+        lo += __popc((unsigned)m);
+        hi += __popc((unsigned)(m >> 32));
     }
-    int cnt = 0;
+    return hi_team ? hi : lo;
+}
+
+// median of the 1024 non-negative values held 32 per lane by a team
+// (native_stage2.rs:1168-1184: sort, mean of the two middle values).  Bitwise
+// selection of the 0-based 511th key, starting below the team's common
+// min/max key prefix; the 512th is that key again or the next larger one.
+template <typename R>
+__device__ inline R team_median1024(const R (&h)[32], bool hi_team) {
+    using K = typename RT<R>::key_t;
+    constexpr int NB = (int)sizeof(K) * 8;
+    constexpr K signmask = ~(K(1) << (NB - 1));
+    K keys[32];
+    K kmin = ~K(0), kmax = 0;
 #pragma unroll
-    for (int r = 0; r < 32; ++r) cnt += keys[r] <= ans ? 1 : 0;
-    cnt = team_sum_i32(cnt);
+    for (int r = 0; r < 32; ++r) {
+        keys[r] = RT<R>::key(h[r]) & signmask;
+        kmin = keys[r] < kmin ? keys[r] : kmin;
+        kmax = keys[r] > kmax ? keys[r] : kmax;
+    }
+    kmin = team_min_key<K>(kmin);
+    kmax = ~team_min_key<K>(~kmax);
+    const K diff = kmin ^ kmax;
+    int top = -1;
+    if (diff) top = NB - 1 - (int)(sizeof(K) == 4 ? __clz((unsigned)diff) : __clzll((long long)diff));
+    K ans = top >= 0 ? (kmin & ~((K(2) << top) - 1)) : kmin;
+    for (int bit = top; bit >= 0; --bit) {
+        const K t = ans | ((K(1) << bit) - 1);
+        if (team_count_le<K>(keys, t, hi_team) < 512) ans |= K(1) << bit;
+    }
     K nxt = ans;
-    if (cnt < 513) {
+    if (team_count_le<K>(keys, ans, hi_team) < 513) {
         K mn = ~K(0);
 #pragma unroll
         for (int r = 0; r < 32; ++r)
@@ -141,21 +225,30 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
                                                            const R *__restrict__ wfac,
                                                            ClapParams<R> prm) {
     const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
+    const bool hi_team = (threadIdx.x & 63) >= 32;
     const int ix = blockIdx.x / prm.ngroups;
     const int g = blockIdx.x - ix * prm.ngroups;
     const int f = g * TEAMS + team;
     const bool active = f < prm.I;
-    const int ni = prm.ni, nj = prm.nj, I = prm.I;
+    const int ni = prm.ni, nj = prm.nj;
     int i1 = ix * CLAP_INC, rs = 0;
     if (i1 + CLAP_WIN > ni) {
         rs = i1 + CLAP_WIN - ni;
         i1 = ni - CLAP_WIN;
     }
-    __shared__ cplx<R> tb[TEAMS][32][33];
-    __shared__ cplx<R> acc[TEAMS][CLAP_WIN][CLAP_WIN];
-    for (int q = lane; q < CLAP_WIN * CLAP_WIN; q += 32)
-        acc[team][q / CLAP_WIN][q % CLAP_WIN] = {(R)0, (R)0};
+    __shared__ cplx<R> tbs[TEAMS][32][33];
+    cplx<R> (&tb)[32][33] = tbs[team];
     const int cS = (lane + 16) & 31;  // fftshift of this lane's column
+    // rolling accumulator of output row lr = lane - rs: slot e holds column
+    // 6*jx + e of the current window jx (native_stage2.rs:904-915)
+    R ar[CLAP_WIN], ai[CLAP_WIN];
+#pragma unroll
+    for (int e = 0; e < CLAP_WIN; ++e) ar[e] = ai[e] = (R)0;
+    const bool out_row = active && lane < CLAP_WIN && lane >= rs;
+    const int lr = lane - rs;
+    const cplx<R> *gsrc = grid + (int64_t)f * ni * nj + (int64_t)(i1 + lane) * nj;
+    const int lrc = lr < 0 ? 0 : (lr > CLAP_WIN - 1 ? CLAP_WIN - 1 : lr);
+    cplx<R> *dst = part + (((int64_t)f * prm.nwi + ix) * CLAP_WIN + lrc) * nj;
 
     for (int jx = 0; jx < prm.nwj; ++jx) {
         int j1 = jx * CLAP_INC, cs = 0;
@@ -166,10 +259,9 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
         R xr[32], xi[32];
         // window row `lane`, zero padded to 32 (NaN -> 0, native_stage2.rs:852-858)
         if (lane < CLAP_WIN && active) {
-            const cplx<R> *src = grid + (int64_t)f * ni * nj + (int64_t)(i1 + lane) * nj + j1;
 #pragma unroll
             for (int c = 0; c < CLAP_WIN; ++c) {
-                cplx<R> v = src[c];
+                cplx<R> v = gsrc[j1 + c];
                 if (RT<R>::isnan_(v.re) || RT<R>::isnan_(v.im)) v = {(R)0, (R)0};
                 xr[c] = v.re;
                 xi[c] = v.im;
@@ -181,26 +273,26 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
 #pragma unroll
             for (int c = 0; c < 32; ++c) xr[c] = xi[c] = (R)0;
         }
-        __syncthreads();  // previous window's readers of tb are done
+        wave_sync();  // the previous window's readers of tb are done
 #pragma unroll
-        for (int c = 0; c < 32; ++c) tb[team][lane][c] = {xr[c], xi[c]};
-        __syncthreads();
+        for (int c = 0; c < 32; ++c) tb[lane][c] = {xr[c], xi[c]};
+        wave_sync();
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const cplx<R> v = tb[team][r][lane];
+            const cplx<R> v = tb[r][lane];
             xr[r] = v.re;
             xi[r] = v.im;
         }
         fft32<R, false>(xr, xi);  // lane c now holds column c of FFT2
-        __syncthreads();
+        wave_sync();
         // |F| at fftshifted position (.re plane)
 #pragma unroll
-        for (int r = 0; r < 32; ++r) tb[team][(r + 16) & 31][cS].re = RT<R>::hypot_(xr[r], xi[r]);
-        __syncthreads();
+        for (int r = 0; r < 32; ++r) tb[(r + 16) & 31][cS].re = RT<R>::abs_(xr[r], xi[r]);
+        wave_sync();
         {   // separable 7-tap smoothing, zero fill: rows (lane = shifted row) -> .im plane
             R h[32];
 #pragma unroll
-            for (int c = 0; c < 32; ++c) h[c] = tb[team][lane][c].re;
+            for (int c = 0; c < 32; ++c) h[c] = tb[lane][c].re;
 #pragma unroll
             for (int c = 0; c < 32; ++c) {
                 R s = (R)0;
@@ -209,14 +301,14 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
                     const int sc = c + kc - 3;
                     if (sc >= 0 && sc < 32) s += h[sc] * prm.k7[kc];
                 }
-                tb[team][lane][c].im = s;
+                tb[lane][c].im = s;
             }
         }
-        __syncthreads();
+        wave_sync();
         {   // columns (lane = shifted column) -> .re plane
             R t[32];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) t[r] = tb[team][r][lane].im;
+            for (int r = 0; r < 32; ++r) t[r] = tb[r][lane].im;
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
                 R s = (R)0;
@@ -225,19 +317,19 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
                     const int sr = r + kr - 3;
                     if (sr >= 0 && sr < 32) s += t[sr] * prm.k7[kr];
                 }
-                tb[team][r][lane].re = s;
+                tb[r][lane].re = s;
             }
         }
-        __syncthreads();
+        wave_sync();
         // ifftshift back: lane c takes H[r][c] = Hf[(r+16)%32][(c+16)%32]
         R h[32];
 #pragma unroll
-        for (int r = 0; r < 32; ++r) h[r] = tb[team][(r + 16) & 31][cS].re;
-        const R med = team_median1024<R>(h);
+        for (int r = 0; r < 32; ++r) h[r] = tb[(r + 16) & 31][cS].re;
+        const R med = team_median1024<R>(h, hi_team);
+        const R scl = RT<R>::gain_scale(med);
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            R v = h[r];
-            if (med != (R)0) v /= med;
+            R v = RT<R>::gain_div(h[r], med, scl);
             v = prm.alpha_is_one ? v - (R)1 : RT<R>::pow_(v, prm.alpha) - (R)1;
             if (v < (R)0) v = (R)0;
             const R gsc = v * prm.beta + lp[r * 32 + lane];
@@ -245,45 +337,85 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
             xi[r] *= gsc;
         }
         fft32<R, true>(xr, xi);  // inverse along columns
-        __syncthreads();
+        wave_sync();
 #pragma unroll
-        for (int r = 0; r < 32; ++r) tb[team][r][lane] = {xr[r], xi[r]};
-        __syncthreads();
-        if (lane < CLAP_WIN) {
+        for (int r = 0; r < 32; ++r) tb[r][lane] = {xr[r], xi[r]};
+        wave_sync();
 #pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                const cplx<R> v = tb[team][lane][c];
-                xr[c] = v.re;
-                xi[c] = v.im;
-            }
-            fft32<R, true>(xr, xi);  // inverse along rows: spatial row `lane`
-            if (active && lane >= rs) {
-                const int lr = lane - rs;
+        for (int c = 0; c < 32; ++c) {
+            const cplx<R> v = tb[lane][c];
+            xr[c] = v.re;
+            xi[c] = v.im;
+        }
+        fft32<R, true>(xr, xi);  // inverse along rows: spatial row `lane`
+        const bool last = jx == prm.nwj - 1;
+        if (cs == 0) {
+            if (out_row) {
 #pragma unroll
-                for (int lj = 0; lj < CLAP_WIN; ++lj) {
-                    if (lj < cs) continue;
-                    const int e = lj - cs;
-                    const int slot = (jx * CLAP_INC + e) % CLAP_WIN;
+                for (int e = 0; e < CLAP_WIN; ++e) {
                     const R fct = wfac[lr * CLAP_WIN + e];
-                    acc[team][lr][slot].re += xr[lj] * fct;
-                    acc[team][lr][slot].im += xi[lj] * fct;
+                    ar[e] += xr[e] * fct;
+                    ai[e] += xi[e] * fct;
                 }
-                // flush the columns no later window of this row touches
-                const int cbeg = jx * CLAP_INC;
-                const int cend = (jx == prm.nwj - 1) ? nj : cbeg + CLAP_INC;
-                cplx<R> *dst = part + (((int64_t)f * prm.nwi + ix) * CLAP_WIN + lr) * nj;
-                for (int col = cbeg; col < cend; ++col) {
-                    const int slot = col % CLAP_WIN;
-                    dst[col] = acc[team][lr][slot];
-                    acc[team][lr][slot] = {(R)0, (R)0};
+            }
+        } else {
+            // clamped last window column: effective column e = lj - cs; the
+            // shift is dynamic, so it goes through the team's LDS tile
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < CLAP_WIN; ++c) tb[lane][c] = {xr[c], xi[c]};
+            wave_sync();
+            if (out_row) {
+#pragma unroll
+                for (int e = 0; e < CLAP_WIN; ++e) {
+                    if (e < CLAP_WIN - cs) {
+                        const cplx<R> v = tb[lane][e + cs];
+                        const R fct = wfac[lr * CLAP_WIN + e];
+                        ar[e] += v.re * fct;
+                        ai[e] += v.im * fct;
+                    }
                 }
             }
         }
+        // flush the columns no later window of this row touches, then shift
+        const int cbeg = jx * CLAP_INC;
+        if (out_row) {
+            if (!last) {
+#pragma unroll
+                for (int e = 0; e < CLAP_INC; ++e) dst[cbeg + e] = {ar[e], ai[e]};
+            } else {
+#pragma unroll
+                for (int e = 0; e < CLAP_WIN; ++e)
+                    if (cbeg + e < nj) dst[cbeg + e] = {ar[e], ai[e]};
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < CLAP_WIN - CLAP_INC; ++e) {
+            ar[e] = ar[e + CLAP_INC];
+            ai[e] = ai[e + CLAP_INC];
+        }
+#pragma unroll
+        for (int e = CLAP_WIN - CLAP_INC; e < CLAP_WIN; ++e) ar[e] = ai[e] = (R)0;
     }
 }
 
-// Sum of the window rows covering each cell (ascending row order).
+// unit phase of a complex value with the reference's f32/f64 arithmetic
+// (normalize_complex_unit_magnitude, native_stage2.rs:1198-1205)
 template <typename R>
+__device__ inline cplx<R> unit_phase(cplx<R> v) {
+    const R mag = RT<R>::hypot_(v.re, v.im);
+    if (mag != (R)0) {
+        v.re /= mag;
+        v.im /= mag;
+    }
+    return v;
+}
+
+// Sum of the window rows covering each cell (ascending row order).  With
+// UNIT the cell's unit phase is stored instead: that is ph_patch of every
+// pixel in the cell (extract_patch_phase gathers then normalises each copy of
+// the same value, native_stage2.rs:1186-1196).
+template <typename R, bool UNIT>
 __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
                                                       cplx<R> *__restrict__ filt, int ni, int nj,
                                                       int I, int nwi) {
@@ -303,25 +435,21 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
         sre += v.re;
         sim += v.im;
     }
-    filt[gid] = {sre, sim};
+    filt[gid] = UNIT ? unit_phase<R>({sre, sim}) : cplx<R>{sre, sim};
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_unit_phase(cplx<R> *__restrict__ v, int64_t n) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < n) v[gid] = unit_phase<R>(v[gid]);
 }
 
 // ---------------------------------------------------------------------------
-// extract_patch_phase (native_stage2.rs:1186-1205) fused with the psdph
-// product of fit_stage2_rows (:686-694): conj(unit(filt)) * ph_nm.
+// psdph of fit_stage2_rows (native_stage2.rs:686-694): conj(ph_patch) * ph_nm,
+// where ph_patch is the cell's unit phase (the loop's filt planes hold unit
+// phases, see k_clap_combine).
 template <typename R>
-__device__ inline cplx<R> unit_phase(cplx<R> v) {
-    const R mag = RT<R>::hypot_(v.re, v.im);
-    if (mag != (R)0) {
-        v.re /= mag;
-        v.im /= mag;
-    }
-    return v;
-}
-
-template <typename R>
-__device__ inline cplx<R> psdph_value(cplx<R> filt, cplx<R> nm) {
-    const cplx<R> p = unit_phase<R>(filt);
+__device__ inline cplx<R> psdph_value(cplx<R> p, cplx<R> nm) {
     const R ar = p.re, ai = -p.im;
     return {ar * nm.re - ai * nm.im, ar * nm.im + ai * nm.re};
 }
@@ -376,7 +504,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     for (int i = 0; i < I; ++i) {
         const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
         if (v.re == (R)0 && v.im == (R)0) valid = false;
-        denom += RT<R>::hypot_(v.re, v.im);
+        denom += RT<R>::abs_(v.re, v.im);
         const R bi = prm.rowinv ? b[i] : b[i * bs];
         if (bi < mn) mn = bi;
         if (bi > mx) mx = bi;
@@ -463,8 +591,8 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
         for (int q = 0; q < TC; ++q) {
             const int t = t0 + q;
             if (t <= tn) {
-                offer(RT<R>::hypot_(pre[q], pim[q]) / denom, t + tn);
-                if (t != 0) offer(RT<R>::hypot_(nre[q], nim[q]) / denom, tn - t);
+                offer(RT<R>::abs_(pre[q], pim[q]) / denom, t + tn);
+                if (t != 0) offer(RT<R>::abs_(nre[q], nim[q]) / denom, tn - t);
             }
         }
     }
@@ -477,11 +605,17 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     }
 }
 
-// FP64 re-score of near-tie rows (FP32 form): a 64-lane wavefront per queued
-// row; the lanes split the trials, each summing over all ifgs in ascending
-// order exactly as ref64's topofit_row; first argmax with strict >.
-// bp64 holds the caller's f64 bperp (the f32 copy used by the bulk is exact
-// only to f32).
+// FP64 re-score of near-tie rows (FP32 form): one 64-lane wavefront per
+// queued row, lanes over interferograms, wave sums per trial; first argmax
+// with strict > over trials in ascending order, as ref64's topofit_row
+// (native_stage2.rs:1243-1253).  bp64 holds the caller's f64 bperp (the f32
+// copy used by the bulk is exact only to f32).
+__device__ inline double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 template <typename RF>
 __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phnm,
                                                 const double *__restrict__ bp64,
@@ -491,6 +625,7 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
                                                 const uint32_t *__restrict__ rescore_count,
                                                 int32_t *__restrict__ tix, FitParams prm) {
     const uint32_t n = *rescore_count;
+    const int lane = threadIdx.x;
     for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
         const int64_t k = rescore_list[q];
         const int I = prm.I;
@@ -499,22 +634,28 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
         const cplx<RF> *fl = filt + cell_of[k];
         auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
         double denom = 0.0, mn = INFINITY, mx = -INFINITY;
-        for (int i = 0; i < I; ++i) {
+        for (int i = lane; i < I; i += 64) {
             const cplx<RF> v = psdph_value<RF>(fl[i * C], nm[i * P]);
             denom += hypot((double)v.re, (double)v.im);
             const double bi = bat(i);
-            if (bi < mn) mn = bi;
-            if (bi > mx) mx = bi;
+            mn = bi < mn ? bi : mn;
+            mx = bi > mx ? bi : mx;
+        }
+        denom = wave_sum_f64(denom);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            mn = fmin(mn, __shfl_xor(mn, o, 64));
+            mx = fmax(mx, __shfl_xor(mx, o, 64));
         }
         if (denom == 0.0) denom = 1.0;
         double range = mx - mn;
         if (range == 0.0) range = 1.0;
         double bestv = -INFINITY;
-        int besti = 0x7fffffff;
-        for (int t = threadIdx.x; t < prm.T; t += 64) {
+        int besti = -1;
+        for (int t = 0; t < prm.T; ++t) {
             const double tv = (double)(t - prm.tn);
             double sr = 0.0, si = 0.0;
-            for (int i = 0; i < I; ++i) {
+            for (int i = lane; i < I; i += 64) {
                 const cplx<RF> v = psdph_value<RF>(fl[i * C], nm[i * P]);
                 const double ph = ((bat(i) / range) * 0.78539816339744830962) * tv;
                 double sn, cs;
@@ -524,23 +665,15 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
                 sr += vr * cs - vi * nsn;
                 si += vr * nsn + vi * cs;
             }
+            sr = wave_sum_f64(sr);
+            si = wave_sum_f64(si);
             const double c = hypot(sr, si) / denom;
-            if (c > bestv) {
+            if (besti < 0 || c > bestv) {
                 bestv = c;
                 besti = t;
             }
         }
-        // wave-wide first argmax: larger value wins, ties go to the lower index
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const double ov = __shfl_xor(bestv, o, 64);
-            const int oi = __shfl_xor(besti, o, 64);
-            if (ov > bestv || (ov == bestv && oi < besti)) {
-                bestv = ov;
-                besti = oi;
-            }
-        }
-        if (threadIdx.x == 0 && besti < prm.T) tix[k] = besti;  // else keep the FP32 choice
+        if (lane == 0 && besti >= 0 && besti < prm.T) tix[k] = besti;
     }
 }
 
@@ -584,7 +717,7 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     for (int i = 0; i < I; ++i) {
         const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
         const R bi = bat(i);
-        const double wgt = (double)RT<R>::hypot_(v.re, v.im);
+        const double wgt = (double)RT<R>::abs_(v.re, v.im);
         const double wb = wgt * (double)bi;
         den += wb * wb;
         const cplx<R> r = rotate_neg<R>(v, k0 * bi);
@@ -598,12 +731,12 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
         const cplx<R> r = rotate_neg<R>(psdph_value<R>(fl[i * C], nm[i * P]), k0 * bat(i));
         mre += r.re;
         mim += r.im;
-        dn += RT<R>::hypot_(r.re, r.im);
+        dn += RT<R>::abs_(r.re, r.im);
     }
     if (dn == (R)0) dn = (R)1;
     Kout[k] = (double)k0;
     Cout[k] = (double)RT<R>::atan2_(mim, mre);
-    cohout[k] = (double)(RT<R>::hypot_(mre, mim) / dn);
+    cohout[k] = (double)(RT<R>::abs_(mre, mim) / dn);
     nopt[k] = 1.0;
 }
 
@@ -625,7 +758,7 @@ __global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restri
     const int f = (int)(gid / P);
     const int64_t k = gid - (int64_t)f * P;
     const cplx<R> fv = filt[(int64_t)f * C + cell_of[k]];
-    const cplx<R> p = unit_phase<R>(fv);
+    const cplx<R> p = fv;
     const int64_t o = order[k] * I + f;
     if (patch_out) patch_out[o] = make_float2((float)p.re, (float)p.im);
     if (res_out) {

@@ -352,9 +352,16 @@ template <typename R>
 int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
     const int64_t n = s->C * s->I;
     SlotTimer tm(s, SLOT_FILL);
-    hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
-                       s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W, s->cell_start.as<int32_t>(),
-                       s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P);
+    if (sizeof(R) == 4) {
+        const dim3 grid(blocks_for(s->C, FILL_CB), blocks_for(s->I, FILL_FG));
+        hipLaunchKernelGGL(k_fill_grid_staged<R>, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
+                           s->bp.as<R>(), s->rowinv, K, W, s->cell_start.as<int32_t>(),
+                           s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P);
+    } else {
+        hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                           s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
+                           s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P);
+    }
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -380,13 +387,13 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
 }
 
 template <typename R>
-int launch_clap(ps2_session *s) {
+int launch_clap(ps2_session *s, bool unit) {
     const int64_t n = s->C * s->I;
     if (!s->windows) {
         PS2_HIP(hipMemsetAsync(s->filt.p, 0, sizeof(cplx<R>) * n, s->stream));
         return PS2_OK;
     }
-    constexpr int TEAMS = sizeof(R) == 4 ? 4 : 2;
+    constexpr int TEAMS = 2;  // one wavefront per workgroup
     const ClapParams<R> prm = clap_params<R>(s, TEAMS);
     {
         SlotTimer tm(s, SLOT_CLAP_STRIP);
@@ -396,9 +403,14 @@ int launch_clap(ps2_session *s) {
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
-    hipLaunchKernelGGL(k_clap_combine<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
-                       s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                       (int)s->I, s->nwi);
+    if (unit)
+        hipLaunchKernelGGL((k_clap_combine<R, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                           s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
+                           (int)s->I, s->nwi);
+    else
+        hipLaunchKernelGGL((k_clap_combine<R, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                           s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
+                           (int)s->I, s->nwi);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -494,7 +506,7 @@ int run_loop(ps2_session *s) {
         if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
         if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>()))) return rc;
         if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
-        if ((rc = launch_clap<R>(s))) return rc;
+        if ((rc = launch_clap<R>(s, true))) return rc;
         if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
         if ((rc = launch_fit<R>(s))) return rc;
         if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
@@ -851,7 +863,7 @@ int ps2_hip_step_clap(ps2_session *s, const void *ph_grid, void *ph_filt_out) {
     PS2_HIP(hipSetDevice(s->device));
     int rc = plane_upload(s, s->grid.p, ph_grid);
     if (rc) return rc;
-    rc = s->form == PS2_FORM_FP64 ? launch_clap<double>(s) : launch_clap<float>(s);
+    rc = s->form == PS2_FORM_FP64 ? launch_clap<double>(s, false) : launch_clap<float>(s, false);
     if (rc) return rc;
     PS2_HIP(hipStreamSynchronize(s->stream));
     return plane_download(s, ph_filt_out, s->filt.p);
@@ -863,6 +875,15 @@ int ps2_hip_step_fit(ps2_session *s, const void *ph_filt, ps2_patch_out *out) {
     PS2_HIP(hipSetDevice(s->device));
     int rc = plane_upload(s, s->filt.p, ph_filt);
     if (rc) return rc;
+    // the loop's filt planes hold unit phases (extract_patch_phase)
+    const int64_t n = s->C * s->I;
+    if (s->form == PS2_FORM_FP64)
+        hipLaunchKernelGGL(k_unit_phase<double>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                           s->filt.as<cplx<double>>(), n);
+    else
+        hipLaunchKernelGGL(k_unit_phase<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
+                           s->filt.as<cplx<float>>(), n);
+    PS2_LAUNCHED();
     rc = s->form == PS2_FORM_FP64 ? launch_fit<double>(s) : launch_fit<float>(s);
     if (rc) return rc;
     uint32_t cnt = 0;
