@@ -11,7 +11,8 @@
 //   phnm  [I][P] complex R   ph_nm in sorted pixel order
 //   bp    [I][P] R           bperp_mat (or [I] when the rows are invariant)
 //   grid, filt [I][C] complex R
-//   part  [I][nwi][24][n_j] complex R  per-window-row CLAP partial sums
+//   part  [I][nwi][2][24][n_j] complex R  per-window-row CLAP partial sums,
+//         alternate window segments in alternate planes
 #pragma once
 
 #include "ps2_device.hpp"
@@ -138,14 +139,17 @@ __global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict_
 // ---------------------------------------------------------------------------
 // CLAP (native_stage2.rs:742-1184).
 //
-// One workgroup per (window row ix, group of TEAMS interferograms).  A team of
-// 32 lanes (half a wavefront) owns one interferogram and walks the windows of
-// the row left to right; each window is a 32x32 FFT held one row / one column
-// per lane in registers, transposed through LDS.  The weighted window outputs
-// are summed per team in a 24-row rolling accumulator in LDS; every column
-// block that no later window of the row touches is flushed to `part`.  A
-// second kernel adds the (at most four) window rows covering each cell in
-// ascending row order.  No atomics, so the result is deterministic.
+// One workgroup per (window row ix, group of TEAMS interferograms, segment of
+// wseg consecutive windows of the row).  A team of 32 lanes (half a wavefront)
+// owns one interferogram and walks its segment's windows left to right; each
+// window is a 32x32 FFT held one row / one column per lane in registers,
+// transposed through LDS.  The weighted window outputs of the segment are
+// summed per team in a rolling 24-column register accumulator; every column
+// block that no later window of the segment touches is flushed to `part`
+// (segment s in plane s % 2, so neighbouring segments, which overlap by 18
+// columns, never write the same element).  A second kernel adds, in a fixed
+// order, the (at most four) window rows and (at most two) segments covering
+// each cell.  No atomics, so the result is deterministic.
 
 template <typename R>
 struct ClapParams {
@@ -153,6 +157,7 @@ struct ClapParams {
     R alpha, beta;  // clap_alpha, clap_beta
     int alpha_is_one;
     int ni, nj, I, nwi, nwj, ngroups;
+    int nseg, wseg;   // window segments per row, windows per segment (>= 3)
 };
 
 // Wave-level LDS synchronisation: a team is half a wavefront and never shares
@@ -219,16 +224,20 @@ __device__ inline R team_median1024(const R (&h)[32], bool hi_team) {
 }
 
 template <typename R, int TEAMS>
-__global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__restrict__ grid,
+__global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(const cplx<R> *__restrict__ grid,
                                                            cplx<R> *__restrict__ part,
                                                            const R *__restrict__ lp,
                                                            const R *__restrict__ wfac,
                                                            ClapParams<R> prm) {
     const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const bool hi_team = (threadIdx.x & 63) >= 32;
-    const int ix = blockIdx.x / prm.ngroups;
-    const int g = blockIdx.x - ix * prm.ngroups;
+    const int seg = blockIdx.x % prm.nseg;
+    const int rowgrp = blockIdx.x / prm.nseg;
+    const int ix = rowgrp / prm.ngroups;
+    const int g = rowgrp - ix * prm.ngroups;
     const int f = g * TEAMS + team;
+    const int jx0 = seg * prm.wseg;
+    const int jx1 = jx0 + prm.wseg < prm.nwj ? jx0 + prm.wseg : prm.nwj;
     const bool active = f < prm.I;
     const int ni = prm.ni, nj = prm.nj;
     int i1 = ix * CLAP_INC, rs = 0;
@@ -236,8 +245,12 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
         rs = i1 + CLAP_WIN - ni;
         i1 = ni - CLAP_WIN;
     }
-    __shared__ cplx<R> tbs[TEAMS][32][33];
-    cplx<R> (&tb)[32][33] = tbs[team];
+    // separate re / im planes; a row of 33 keeps row and column walks on
+    // distinct banks, and each team's plane starts half the banks apart
+    __shared__ R tbr_s[TEAMS][32][33];
+    __shared__ R tbi_s[TEAMS][32][33];
+    R (&tbr)[32][33] = tbr_s[team];
+    R (&tbi)[32][33] = tbi_s[team];
     const int cS = (lane + 16) & 31;  // fftshift of this lane's column
     // rolling accumulator of output row lr = lane - rs: slot e holds column
     // 6*jx + e of the current window jx (native_stage2.rs:904-915)
@@ -248,9 +261,9 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
     const int lr = lane - rs;
     const cplx<R> *gsrc = grid + (int64_t)f * ni * nj + (int64_t)(i1 + lane) * nj;
     const int lrc = lr < 0 ? 0 : (lr > CLAP_WIN - 1 ? CLAP_WIN - 1 : lr);
-    cplx<R> *dst = part + (((int64_t)f * prm.nwi + ix) * CLAP_WIN + lrc) * nj;
+    cplx<R> *dst = part + ((((int64_t)f * prm.nwi + ix) * 2 + (seg & 1)) * CLAP_WIN + lrc) * nj;
 
-    for (int jx = 0; jx < prm.nwj; ++jx) {
+    for (int jx = jx0; jx < jx1; ++jx) {
         int j1 = jx * CLAP_INC, cs = 0;
         if (j1 + CLAP_WIN > nj) {
             cs = j1 + CLAP_WIN - nj;
@@ -273,26 +286,28 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
 #pragma unroll
             for (int c = 0; c < 32; ++c) xr[c] = xi[c] = (R)0;
         }
-        wave_sync();  // the previous window's readers of tb are done
+        wave_sync();  // the previous window's readers of the tile are done
 #pragma unroll
-        for (int c = 0; c < 32; ++c) tb[lane][c] = {xr[c], xi[c]};
+        for (int c = 0; c < 32; ++c) {
+            tbr[lane][c] = xr[c];
+            tbi[lane][c] = xi[c];
+        }
         wave_sync();
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const cplx<R> v = tb[r][lane];
-            xr[r] = v.re;
-            xi[r] = v.im;
+            xr[r] = tbr[r][lane];
+            xi[r] = tbi[r][lane];
         }
         fft32<R, false>(xr, xi);  // lane c now holds column c of FFT2
         wave_sync();
-        // |F| at fftshifted position (.re plane)
+        // |F| at fftshifted position (re plane)
 #pragma unroll
-        for (int r = 0; r < 32; ++r) tb[(r + 16) & 31][cS].re = RT<R>::abs_(xr[r], xi[r]);
+        for (int r = 0; r < 32; ++r) tbr[(r + 16) & 31][cS] = RT<R>::abs_(xr[r], xi[r]);
         wave_sync();
-        {   // separable 7-tap smoothing, zero fill: rows (lane = shifted row) -> .im plane
+        {   // separable 7-tap smoothing, zero fill: rows (lane = shifted row) -> im plane
             R h[32];
 #pragma unroll
-            for (int c = 0; c < 32; ++c) h[c] = tb[lane][c].re;
+            for (int c = 0; c < 32; ++c) h[c] = tbr[lane][c];
 #pragma unroll
             for (int c = 0; c < 32; ++c) {
                 R s = (R)0;
@@ -301,14 +316,14 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
                     const int sc = c + kc - 3;
                     if (sc >= 0 && sc < 32) s += h[sc] * prm.k7[kc];
                 }
-                tb[lane][c].im = s;
+                tbi[lane][c] = s;
             }
         }
         wave_sync();
-        {   // columns (lane = shifted column) -> .re plane
+        {   // columns (lane = shifted column) -> re plane
             R t[32];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) t[r] = tb[r][lane].im;
+            for (int r = 0; r < 32; ++r) t[r] = tbi[r][lane];
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
                 R s = (R)0;
@@ -317,14 +332,14 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
                     const int sr = r + kr - 3;
                     if (sr >= 0 && sr < 32) s += t[sr] * prm.k7[kr];
                 }
-                tb[r][lane].re = s;
+                tbr[r][lane] = s;
             }
         }
         wave_sync();
         // ifftshift back: lane c takes H[r][c] = Hf[(r+16)%32][(c+16)%32]
         R h[32];
 #pragma unroll
-        for (int r = 0; r < 32; ++r) h[r] = tb[(r + 16) & 31][cS].re;
+        for (int r = 0; r < 32; ++r) h[r] = tbr[(r + 16) & 31][cS];
         const R med = team_median1024<R>(h, hi_team);
         const R scl = RT<R>::gain_scale(med);
 #pragma unroll
@@ -339,16 +354,18 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
         fft32<R, true>(xr, xi);  // inverse along columns
         wave_sync();
 #pragma unroll
-        for (int r = 0; r < 32; ++r) tb[r][lane] = {xr[r], xi[r]};
+        for (int r = 0; r < 32; ++r) {
+            tbr[r][lane] = xr[r];
+            tbi[r][lane] = xi[r];
+        }
         wave_sync();
 #pragma unroll
         for (int c = 0; c < 32; ++c) {
-            const cplx<R> v = tb[lane][c];
-            xr[c] = v.re;
-            xi[c] = v.im;
+            xr[c] = tbr[lane][c];
+            xi[c] = tbi[lane][c];
         }
         fft32<R, true>(xr, xi);  // inverse along rows: spatial row `lane`
-        const bool last = jx == prm.nwj - 1;
+        const bool last = jx == jx1 - 1;
         if (cs == 0) {
             if (out_row) {
 #pragma unroll
@@ -363,16 +380,18 @@ __global__ __launch_bounds__(32 * TEAMS) void k_clap_strip(const cplx<R> *__rest
             // shift is dynamic, so it goes through the team's LDS tile
             wave_sync();
 #pragma unroll
-            for (int c = 0; c < CLAP_WIN; ++c) tb[lane][c] = {xr[c], xi[c]};
+            for (int c = 0; c < CLAP_WIN; ++c) {
+                tbr[lane][c] = xr[c];
+                tbi[lane][c] = xi[c];
+            }
             wave_sync();
             if (out_row) {
 #pragma unroll
                 for (int e = 0; e < CLAP_WIN; ++e) {
                     if (e < CLAP_WIN - cs) {
-                        const cplx<R> v = tb[lane][e + cs];
                         const R fct = wfac[lr * CLAP_WIN + e];
-                        ar[e] += v.re * fct;
-                        ai[e] += v.im * fct;
+                        ar[e] += tbr[lane][e + cs] * fct;
+                        ai[e] += tbi[lane][e + cs] * fct;
                     }
                 }
             }
@@ -418,7 +437,7 @@ __device__ inline cplx<R> unit_phase(cplx<R> v) {
 template <typename R, bool UNIT>
 __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
                                                       cplx<R> *__restrict__ filt, int ni, int nj,
-                                                      int I, int nwi) {
+                                                      int I, int nwi, int nseg, int wseg) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t C = (int64_t)ni * nj;
     if (gid >= C * I) return;
@@ -428,12 +447,24 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
     int ix_hi = r / CLAP_INC;
     if (ix_hi > nwi - 1) ix_hi = nwi - 1;
     const int ix_lo = r >= CLAP_WIN - 1 ? (r - (CLAP_WIN - 1) + CLAP_INC - 1) / CLAP_INC : 0;
+    // segments whose flushed columns contain c: s1 and, inside its first 18
+    // columns, the previous one (the spans overlap by n_win - n_inc)
+    const int span = CLAP_INC * wseg;
+    int s1 = c / span;
+    if (s1 > nseg - 1) s1 = nseg - 1;
+    int s0 = s1;
+    if (s1 > 0) {
+        const int prev_last = s1 * wseg;   // first window of s1 = one past s1-1's last
+        if (c < CLAP_INC * prev_last + (CLAP_WIN - CLAP_INC)) s0 = s1 - 1;
+    }
     R sre = (R)0, sim = (R)0;
     for (int ix = ix_lo; ix <= ix_hi; ++ix) {
         const int lr = r - ix * CLAP_INC;
-        const cplx<R> v = part[(((int64_t)f * nwi + ix) * CLAP_WIN + lr) * nj + c];
-        sre += v.re;
-        sim += v.im;
+        for (int sg = s0; sg <= s1; ++sg) {
+            const cplx<R> v = part[((((int64_t)f * nwi + ix) * 2 + (sg & 1)) * CLAP_WIN + lr) * nj + c];
+            sre += v.re;
+            sim += v.im;
+        }
     }
     filt[gid] = UNIT ? unit_phase<R>({sre, sim}) : cplx<R>{sre, sim};
 }

@@ -4,7 +4,7 @@
 // each random phase row is then fitted in FP64 by one device thread, exactly
 // as topofit_phase_row_coh + refine_phase_candidate_coh, and binned with
 // integer atomics, so the counts are exact and order-independent.
-#include "../../include/pystamps_hip.h"
+#include "pystamps_hip.h"
 #include "ps2_host_util.hpp"
 
 #include <hip/hip_runtime.h>

@@ -1,6 +1,6 @@
 // pystamps_hip: stage-2 gamma loop on one MI355X (gfx950).
 // C ABI declared in include/pystamps_hip.h.
-#include "../../include/pystamps_hip.h"
+#include "pystamps_hip.h"
 #include "ps2_host_util.hpp"
 #include "ps2_kernels.hpp"
 
@@ -293,7 +293,7 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->grid.alloc(sizeof(cplx<R>) * C * I))) return rc;
     if ((rc = s->filt.alloc(sizeof(cplx<R>) * C * I))) return rc;
     if (s->windows)
-        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * CLAP_WIN * s->nj * I))) return rc;
+        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * 2 * CLAP_WIN * s->nj * I))) return rc;
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
     if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_list.alloc(sizeof(int32_t) * P))) return rc;
@@ -383,6 +383,14 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
     p.nwi = s->nwi;
     p.nwj = s->nwj;
     p.ngroups = (int)((s->I + teams - 1) / teams);
+    // split window rows into segments until there are ~4 wavefronts per SIMD
+    // (1024 SIMDs); a segment keeps at least 3 windows so only neighbouring
+    // segments overlap
+    const int rows = std::max(1, s->nwi * p.ngroups);
+    int nseg = std::max(1, (4096 + rows - 1) / rows);
+    nseg = std::min(nseg, std::max(1, s->nwj / 3));
+    p.wseg = (s->nwj + nseg - 1) / nseg;
+    p.nseg = (s->nwj + p.wseg - 1) / p.wseg;
     return p;
 }
 
@@ -397,7 +405,7 @@ int launch_clap(ps2_session *s, bool unit) {
     const ClapParams<R> prm = clap_params<R>(s, TEAMS);
     {
         SlotTimer tm(s, SLOT_CLAP_STRIP);
-        hipLaunchKernelGGL((k_clap_strip<R, TEAMS>), dim3(s->nwi * prm.ngroups), dim3(32 * TEAMS), 0,
+        hipLaunchKernelGGL((k_clap_strip<R, TEAMS>), dim3(s->nwi * prm.ngroups * prm.nseg), dim3(32 * TEAMS), 0,
                            s->stream, s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
                            s->wfac.as<R>(), prm);
         PS2_LAUNCHED();
@@ -406,11 +414,11 @@ int launch_clap(ps2_session *s, bool unit) {
     if (unit)
         hipLaunchKernelGGL((k_clap_combine<R, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                           (int)s->I, s->nwi);
+                           (int)s->I, s->nwi, prm.nseg, prm.wseg);
     else
         hipLaunchKernelGGL((k_clap_combine<R, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                           (int)s->I, s->nwi);
+                           (int)s->I, s->nwi, prm.nseg, prm.wseg);
     PS2_LAUNCHED();
     return PS2_OK;
 }
