@@ -6,11 +6,12 @@ when the library has not been built.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libpystamps_hip.so"
+LIB_PATH = Path(os.environ.get("PS2_HIP_LIB", PKG / "libpystamps_hip.so"))
 HEADER = PKG.parent / "include" / "pystamps_hip.h"
 
 ABI_VERSION = 1

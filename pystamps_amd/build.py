@@ -31,29 +31,30 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build the stage-2 library)")
 
 
-def command(out: Path = LIB) -> list[str]:
+def command(out: Path = LIB, defines: tuple[str, ...] = ()) -> list[str]:
     return [
         hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall", "-Wno-unused-result",
         "-I", str(PKG.parent / "include"),
+        *[f"-D{d}" for d in defines],
         "-o", str(out),
         *[str(CSRC / s) for s in SOURCES],
     ]
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = command(tmp)
+def build(verbose: bool = False, out: Path = LIB, defines: tuple[str, ...] = ()) -> Path:
+    tmp = out.with_suffix(".so.tmp")
+    cmd = command(tmp, defines)
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libpystamps_hip.so")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -143,8 +143,8 @@ __device__ inline void fft32(R (&xr)[32], R (&xi)[32]) {
                 } else {
                     const R wr = twr<R>(tw);
                     const R wi = INV ? -twi_fwd<R>(tw) : twi_fwd<R>(tw);
-                    vr = br * wr - bi * wi;
-                    vi = br * wi + bi * wr;
+                    vr = fma(br, wr, -(bi * wi));
+                    vi = fma(br, wi, bi * wr);
                 }
                 R ur = ar[b + j], ui = ai[b + j];
                 ar[b + j] = ur + vr;

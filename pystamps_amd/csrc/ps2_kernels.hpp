@@ -19,6 +19,10 @@
 
 namespace ps2 {
 
+#ifndef PS2_CLAP_MEDIAN_HIST
+#define PS2_CLAP_MEDIAN_HIST 1   // 1: histogram median (FP32 form), 0: bitwise search
+#endif
+
 constexpr int CLAP_FFT = 32;   // n_win_ex
 constexpr int CLAP_WIN = 24;   // n_win
 constexpr int CLAP_INC = 6;    // n_inc = n_win / 4
@@ -185,20 +189,26 @@ __device__ inline int team_count_le(const K (&keys)[32], K t, bool hi_team) {
     return hi_team ? hi : lo;
 }
 
-// median of the 1024 non-negative values held 32 per lane by a team
+// Keys of non-negative reals: the IEEE bit pattern with the sign cleared
+// orders exactly like the value, and converts back without loss.
+template <typename R>
+__device__ inline typename RT<R>::key_t median_key(R v) {
+    using K = typename RT<R>::key_t;
+    constexpr K signmask = ~(K(1) << (sizeof(K) * 8 - 1));
+    return RT<R>::key(v) & signmask;
+}
+
+// median of the 1024 non-negative values held (as keys) 32 per lane by a team
 // (native_stage2.rs:1168-1184: sort, mean of the two middle values).  Bitwise
 // selection of the 0-based 511th key, starting below the team's common
 // min/max key prefix; the 512th is that key again or the next larger one.
 template <typename R>
-__device__ inline R team_median1024(const R (&h)[32], bool hi_team) {
+__device__ inline R team_median_bits(const typename RT<R>::key_t (&keys)[32], bool hi_team) {
     using K = typename RT<R>::key_t;
     constexpr int NB = (int)sizeof(K) * 8;
-    constexpr K signmask = ~(K(1) << (NB - 1));
-    K keys[32];
     K kmin = ~K(0), kmax = 0;
 #pragma unroll
     for (int r = 0; r < 32; ++r) {
-        keys[r] = RT<R>::key(h[r]) & signmask;
         kmin = keys[r] < kmin ? keys[r] : kmin;
         kmax = keys[r] > kmax ? keys[r] : kmax;
     }
@@ -221,6 +231,125 @@ __device__ inline R team_median1024(const R (&h)[32], bool hi_team) {
         nxt = team_min_key<K>(mn);
     }
     return (RT<R>::from_key(ans) + RT<R>::from_key(nxt)) / (R)2;
+}
+
+// 7-tap Gaussian at position c of a zero-extended 32-vector; the taps are
+// symmetric (gausswin), so pairs are added before the multiply.
+template <typename R>
+__device__ inline R smooth7(const R (&h)[32], int c, const R (&k)[7]) {
+    auto at = [&](int q) { return (q >= 0 && q < 32) ? h[q] : (R)0; };
+    R s = h[c] * k[3];
+    s = fma(at(c - 1) + at(c + 1), k[2], s);
+    s = fma(at(c - 2) + at(c + 2), k[1], s);
+    s = fma(at(c - 3) + at(c + 3), k[0], s);
+    return s;
+}
+
+// Exact median for the FP32 form: a 1024-bucket histogram of the keys
+// (bucket = (key - kmin) >> shift, monotone in the value; float keys are
+// close to logarithmic, so the buckets stay narrow near the median) built with
+// LDS atomics in the team's tile, then exact selection among the few keys of
+// the bucket holding rank 511 (rank 512 is in the same bucket or is the
+// smallest key above it).  A crowded bucket falls back to the bitwise search.
+// hist (>= 1024 words) and cand (>= 64 words) are the team's LDS scratch.
+template <typename R>
+__device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi_team, R *hist_mem,
+                                R *cand_mem) {
+    if constexpr (sizeof(R) != 4) {
+        return team_median_bits<R>(keys, hi_team);
+    } else {
+        const int lane = threadIdx.x & 31;
+        uint32_t *hist = reinterpret_cast<uint32_t *>(hist_mem);
+        uint32_t *cand = reinterpret_cast<uint32_t *>(cand_mem);
+        uint32_t kmin = 0xffffffffu, kmax = 0u;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            kmin = keys[r] < kmin ? keys[r] : kmin;
+            kmax = keys[r] > kmax ? keys[r] : kmax;
+        }
+        kmin = team_min_key<uint32_t>(kmin);
+        kmax = ~team_min_key<uint32_t>(~kmax);
+        if (kmin == kmax) return __uint_as_float(kmin);
+        const uint32_t span = kmax - kmin;
+        const int bits = 32 - __clz((int)span);
+        const int shift = bits > 10 ? bits - 10 : 0;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) hist[q * 32 + lane] = 0u;
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < 32; ++r) atomicAdd(&hist[(keys[r] - kmin) >> shift], 1u);
+        wave_sync();
+        // lane L owns buckets [32L, 32L+32): local counts, team inclusive scan
+        uint32_t tot = 0;
+#pragma unroll
+        for (int q = 0; q < 32; ++q) tot += hist[lane * 32 + q];
+        uint32_t incl = tot;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 32);
+            if (lane >= o) incl += v;
+        }
+        const uint32_t excl = incl - tot;
+        uint32_t b511 = 0, below511 = 0;
+        const bool owns = excl <= 511u && 511u < incl;
+        if (owns) {
+            uint32_t run = excl;
+            for (int q = 0; q < 32; ++q) {
+                const uint32_t c = hist[lane * 32 + q];
+                if (run + c > 511u) {
+                    b511 = (uint32_t)(lane * 32 + q);
+                    below511 = run;
+                    break;
+                }
+                run += c;
+            }
+        }
+        const unsigned long long fm = __ballot(owns);
+        const unsigned tm = hi_team ? (unsigned)(fm >> 32) : (unsigned)fm;
+        const int owner = __ffs((int)tm) - 1;
+        b511 = __shfl(b511, owner, 32);
+        below511 = __shfl(below511, owner, 32);
+        const uint32_t n511 = hist[b511];
+        if (n511 > 64u) return team_median_bits<R>(keys, hi_team);
+        // compact the keys of bucket b511 into cand, in team order
+        uint32_t mine = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) mine += ((keys[r] - kmin) >> shift) == b511 ? 1u : 0u;
+        uint32_t off = mine;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t v = __shfl_up(off, o, 32);
+            if (lane >= o) off += v;
+        }
+        off -= mine;
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            if (((keys[r] - kmin) >> shift) == b511) cand[off++] = keys[r];
+        wave_sync();
+        // rank of each candidate within the bucket (ties by position)
+        const uint32_t j0 = 511u - below511;
+        uint32_t v0 = 0xffffffffu, v1 = 0xffffffffu;
+        for (uint32_t q = (uint32_t)lane; q < n511; q += 32u) {
+            const uint32_t x = cand[q];
+            uint32_t rank = 0;
+            for (uint32_t m = 0; m < n511; ++m) {
+                const uint32_t y = cand[m];
+                rank += (y < x || (y == x && m < q)) ? 1u : 0u;
+            }
+            if (rank == j0) v0 = x;
+            if (rank == j0 + 1u) v1 = x;
+        }
+        const uint32_t k511 = team_min_key<uint32_t>(v0);
+        uint32_t k512 = team_min_key<uint32_t>(v1);
+        if (j0 + 1u >= n511) {
+            uint32_t mn = 0xffffffffu;
+#pragma unroll
+            for (int r = 0; r < 32; ++r)
+                if (((keys[r] - kmin) >> shift) > b511 && keys[r] < mn) mn = keys[r];
+            k512 = team_min_key<uint32_t>(mn);
+        }
+        return (__uint_as_float(k511) + __uint_as_float(k512)) / 2.0f;
+    }
 }
 
 template <typename R, int TEAMS>
@@ -264,6 +393,11 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
     cplx<R> *dst = part + ((((int64_t)f * prm.nwi + ix) * 2 + (seg & 1)) * CLAP_WIN + lrc) * nj;
 
     for (int jx = jx0; jx < jx1; ++jx) {
+        // re-read the low-pass and window-weight tables (L1-resident) every
+        // window instead of holding 56 loop-invariant values in registers
+        const R *lpw = lp;
+        const R *wfw = wfac;
+        asm volatile("" : "+s"(lpw), "+s"(wfw));
         int j1 = jx * CLAP_INC, cs = 0;
         if (j1 + CLAP_WIN > nj) {
             cs = j1 + CLAP_WIN - nj;
@@ -309,15 +443,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
             for (int c = 0; c < 32; ++c) h[c] = tbr[lane][c];
 #pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                R s = (R)0;
-#pragma unroll
-                for (int kc = 0; kc < 7; ++kc) {
-                    const int sc = c + kc - 3;
-                    if (sc >= 0 && sc < 32) s += h[sc] * prm.k7[kc];
-                }
-                tbi[lane][c] = s;
-            }
+            for (int c = 0; c < 32; ++c) tbi[lane][c] = smooth7<R>(h, c, prm.k7);
         }
         wave_sync();
         {   // columns (lane = shifted column) -> re plane
@@ -325,29 +451,27 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
             for (int r = 0; r < 32; ++r) t[r] = tbi[r][lane];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                R s = (R)0;
-#pragma unroll
-                for (int kr = 0; kr < 7; ++kr) {
-                    const int sr = r + kr - 3;
-                    if (sr >= 0 && sr < 32) s += t[sr] * prm.k7[kr];
-                }
-                tbr[r][lane] = s;
-            }
+            for (int r = 0; r < 32; ++r) tbr[r][lane] = smooth7<R>(t, r, prm.k7);
         }
         wave_sync();
-        // ifftshift back: lane c takes H[r][c] = Hf[(r+16)%32][(c+16)%32]
-        R h[32];
+        // ifftshift back: lane c takes H[r][c] = Hf[(r+16)%32][(c+16)%32],
+        // kept as median keys (exact: H >= 0)
+        typename RT<R>::key_t hk[32];
 #pragma unroll
-        for (int r = 0; r < 32; ++r) h[r] = tbr[(r + 16) & 31][cS];
-        const R med = team_median1024<R>(h, hi_team);
+        for (int r = 0; r < 32; ++r) hk[r] = median_key<R>(tbr[(r + 16) & 31][cS]);
+        wave_sync();  // the tile is scratch for the median from here to the gain
+#if PS2_CLAP_MEDIAN_HIST
+        const R med = team_median<R>(hk, hi_team, &tbr[0][0], &tbi[0][0]);
+#else
+        const R med = team_median_bits<R>(hk, hi_team);
+#endif
         const R scl = RT<R>::gain_scale(med);
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            R v = RT<R>::gain_div(h[r], med, scl);
+            R v = RT<R>::gain_div(RT<R>::from_key(hk[r]), med, scl);
             v = prm.alpha_is_one ? v - (R)1 : RT<R>::pow_(v, prm.alpha) - (R)1;
             if (v < (R)0) v = (R)0;
-            const R gsc = v * prm.beta + lp[r * 32 + lane];
+            const R gsc = v * prm.beta + lpw[r * 32 + lane];
             xr[r] *= gsc;
             xi[r] *= gsc;
         }
@@ -370,9 +494,9 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
             if (out_row) {
 #pragma unroll
                 for (int e = 0; e < CLAP_WIN; ++e) {
-                    const R fct = wfac[lr * CLAP_WIN + e];
-                    ar[e] += xr[e] * fct;
-                    ai[e] += xi[e] * fct;
+                    const R fct = wfw[lr * CLAP_WIN + e];
+                    ar[e] = fma(xr[e], fct, ar[e]);
+                    ai[e] = fma(xi[e], fct, ai[e]);
                 }
             }
         } else {
@@ -389,7 +513,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
                 for (int e = 0; e < CLAP_WIN; ++e) {
                     if (e < CLAP_WIN - cs) {
-                        const R fct = wfac[lr * CLAP_WIN + e];
+                        const R fct = wfw[lr * CLAP_WIN + e];
                         ar[e] += tbr[lane][e + cs] * fct;
                         ai[e] += tbi[lane][e + cs] * fct;
                     }
