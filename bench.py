@@ -154,7 +154,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded generator, BASELINE config %d)" % args.config,
+        "data": "synthetic (seeded generator, random-init patch of the named workload)",
         "config": {"workload": cfg.name, "n_ps": P, "n_ifg": I, "grid": [prep.n_i, prep.n_j],
                    "clap_windows_per_ifg": clap_windows(prep.n_i) * clap_windows(prep.n_j),
                    "n_trial": len(prep.trial_values), "iterations_per_step": r32["iters"] // args.steps,
