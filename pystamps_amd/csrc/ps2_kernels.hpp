@@ -352,7 +352,7 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
     }
 }
 
-template <typename R, int TEAMS>
+template <typename R, int TEAMS, bool ALPHA_ONE>
 __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(const cplx<R> *__restrict__ grid,
                                                            cplx<R> *__restrict__ part,
                                                            const R *__restrict__ lp,
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             R v = RT<R>::gain_div(RT<R>::from_key(hk[r]), med, scl);
-            v = prm.alpha_is_one ? v - (R)1 : RT<R>::pow_(v, prm.alpha) - (R)1;
+            v = ALPHA_ONE ? v - (R)1 : RT<R>::pow_(v, prm.alpha) - (R)1;
             if (v < (R)0) v = (R)0;
             const R gsc = v * prm.beta + lpw[r * 32 + lane];
             xr[r] *= gsc;
@@ -640,6 +640,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                                               const cplx<R> *__restrict__ filt,
                                               const int32_t *__restrict__ cell_of,
                                               int32_t *__restrict__ tix, R *__restrict__ rng,
+                                              cplx<R> *__restrict__ offset,
                                               int32_t *__restrict__ rescore_list,
                                               uint32_t *__restrict__ rescore_count,
                                               FitParams prm) {
@@ -653,36 +654,31 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     const int64_t bs = prm.rowinv ? 0 : P;            // b[i * bs] (rowinv: b[i])
     const int64_t C = prm.C;
 
-    bool valid = true;
-    R denom = (R)0;
     R mn = (R)INFINITY, mx = -(R)INFINITY;
     for (int i = 0; i < I; ++i) {
-        const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
-        if (v.re == (R)0 && v.im == (R)0) valid = false;
-        denom += RT<R>::abs_(v.re, v.im);
         const R bi = prm.rowinv ? b[i] : b[i * bs];
         if (bi < mn) mn = bi;
         if (bi > mx) mx = bi;
     }
-    if (!valid) {
-        tix[k] = -1;
-        rng[k] = (R)1;
-        return;
-    }
-    if (denom == (R)0) denom = (R)1;
     R range = mx - mn;
     if (range == (R)0) range = (R)1;
-    rng[k] = range;
     const R pi4 = RT<R>::pi4();
     const int tn = prm.tn;
 
+    // validity (any zero psdph element invalidates the row, :686-697) and the
+    // denominator sum |psdph| ride along the first trial chunk
+    bool valid = true;
+    R denom = (R)0;
     R best = -(R)INFINITY, second = -(R)INFINITY;
+    R bre = (R)0, bim = (R)0;   // trial sum at the selected trial: refine's offset
     int best_ix = 0;
-    auto offer = [&](R val, int idx) {
+    auto offer = [&](R val, int idx, R sre, R sim) {
         if (val > best || (val == best && idx < best_ix)) {
             if (best_ix != idx) second = best > second ? best : second;
             best = val;
             best_ix = idx;
+            bre = sre;
+            bim = sim;
         } else if (val > second) {
             second = val;
         }
@@ -695,6 +691,10 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
         for (int q = 0; q < TC; ++q) pre[q] = pim[q] = nre[q] = nim[q] = (R)0;
         for (int i = 0; i < I; ++i) {
             const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
+            if (t0 == 0) {
+                if (v.re == (R)0 && v.im == (R)0) valid = false;
+                denom += RT<R>::abs_(v.re, v.im);
+            }
             const R bi = prm.rowinv ? b[i] : b[i * bs];
             const R phi = (bi / range) * pi4;
             if constexpr (sizeof(R) == 4) {
@@ -742,16 +742,26 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                 }
             }
         }
+        if (t0 == 0) {
+            if (!valid) {
+                tix[k] = -1;
+                rng[k] = (R)1;
+                return;
+            }
+            if (denom == (R)0) denom = (R)1;
+        }
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
             const int t = t0 + q;
             if (t <= tn) {
-                offer(RT<R>::abs_(pre[q], pim[q]) / denom, t + tn);
-                if (t != 0) offer(RT<R>::abs_(nre[q], nim[q]) / denom, tn - t);
+                offer(RT<R>::abs_(pre[q], pim[q]) / denom, t + tn, pre[q], pim[q]);
+                if (t != 0) offer(RT<R>::abs_(nre[q], nim[q]) / denom, tn - t, nre[q], nim[q]);
             }
         }
     }
     tix[k] = best_ix;
+    rng[k] = range;
+    offset[k] = {bre, bim};
     if constexpr (sizeof(R) == 4) {
         if (prm.T > 1 && best - second < (R)prm.rescore_bound) {
             const uint32_t slot = atomicAdd(rescore_count, 1u);
@@ -778,7 +788,8 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
                                                 const int32_t *__restrict__ cell_of,
                                                 const int32_t *__restrict__ rescore_list,
                                                 const uint32_t *__restrict__ rescore_count,
-                                                int32_t *__restrict__ tix, FitParams prm) {
+                                                int32_t *__restrict__ tix,
+                                                cplx<RF> *__restrict__ offset, FitParams prm) {
     const uint32_t n = *rescore_count;
     const int lane = threadIdx.x;
     for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
@@ -805,7 +816,7 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
         if (denom == 0.0) denom = 1.0;
         double range = mx - mn;
         if (range == 0.0) range = 1.0;
-        double bestv = -INFINITY;
+        double bestv = -INFINITY, bsr = 0.0, bsi = 0.0;
         int besti = -1;
         for (int t = 0; t < prm.T; ++t) {
             const double tv = (double)(t - prm.tn);
@@ -826,9 +837,14 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
             if (besti < 0 || c > bestv) {
                 bestv = c;
                 besti = t;
+                bsr = sr;
+                bsi = si;
             }
         }
-        if (lane == 0 && besti >= 0 && besti < prm.T) tix[k] = besti;
+        if (lane == 0 && besti >= 0 && besti < prm.T && besti != tix[k]) {
+            tix[k] = besti;
+            offset[k] = {(RF)bsr, (RF)bsi};
+        }
     }
 }
 
@@ -840,7 +856,9 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
                                                 const cplx<R> *__restrict__ filt,
                                                 const int32_t *__restrict__ cell_of,
                                                 const int32_t *__restrict__ tix,
-                                                const R *__restrict__ rng, double *__restrict__ Kout,
+                                                const R *__restrict__ rng,
+                                                const cplx<R> *__restrict__ offset,
+                                                double *__restrict__ Kout,
                                                 double *__restrict__ Cout,
                                                 double *__restrict__ cohout,
                                                 double *__restrict__ nopt, FitParams prm) {
@@ -861,13 +879,10 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     auto bat = [&](int i) { return prm.rowinv ? bp[i] : bp[(int64_t)i * P + k]; };
     const R range = rng[k];
     R k0 = (R)(RT<R>::pi4() / range * (R)(t - prm.tn));
-    R ore = (R)0, oim = (R)0;
-    for (int i = 0; i < I; ++i) {
-        const cplx<R> a = rotate_neg<R>(psdph_value<R>(fl[i * C], nm[i * P]), k0 * bat(i));
-        ore += a.re;
-        oim += a.im;
-    }
-    const R ocr = ore, oci = -oim;
+    // offset = sum psdph e^{-i k0 bp} (:1261-1267) is the trial scan's sum at
+    // the selected trial (same angles up to rounding), kept by k_scan/k_rescore
+    const cplx<R> off = offset[k];
+    const R ocr = off.re, oci = -off.im;
     double num = 0.0, den = 0.0;
     for (int i = 0; i < I; ++i) {
         const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);

@@ -122,7 +122,7 @@ struct ps2_session {
     std::vector<double> nr_base;
     HistRule hist_rule;
     // device state
-    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, wfac, rng;
+    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, wfac, rng, offset;
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
@@ -295,6 +295,7 @@ int alloc_state(ps2_session *s) {
     if (s->windows)
         if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * 2 * CLAP_WIN * s->nj * I))) return rc;
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
+    if ((rc = s->offset.alloc(sizeof(cplx<R>) * P))) return rc;
     if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_list.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_count.alloc(sizeof(uint32_t) * 4))) return rc;
@@ -405,9 +406,15 @@ int launch_clap(ps2_session *s, bool unit) {
     const ClapParams<R> prm = clap_params<R>(s, TEAMS);
     {
         SlotTimer tm(s, SLOT_CLAP_STRIP);
-        hipLaunchKernelGGL((k_clap_strip<R, TEAMS>), dim3(s->nwi * prm.ngroups * prm.nseg), dim3(32 * TEAMS), 0,
-                           s->stream, s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
-                           s->wfac.as<R>(), prm);
+        const dim3 g(s->nwi * prm.ngroups * prm.nseg);
+        if (prm.alpha_is_one)
+            hipLaunchKernelGGL((k_clap_strip<R, TEAMS, true>), g, dim3(32 * TEAMS), 0, s->stream,
+                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
+                               s->wfac.as<R>(), prm);
+        else
+            hipLaunchKernelGGL((k_clap_strip<R, TEAMS, false>), g, dim3(32 * TEAMS), 0, s->stream,
+                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
+                               s->wfac.as<R>(), prm);
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
@@ -448,12 +455,14 @@ int launch_fit(ps2_session *s) {
     if (s->tn < 8) {
         hipLaunchKernelGGL((k_scan<R, 8>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                            s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->tix.as<int32_t>(), s->rng.as<R>(), s->rescore_list.as<int32_t>(),
+                           s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
+                           s->rescore_list.as<int32_t>(),
                            s->rescore_count.as<uint32_t>(), fp);
     } else {
         hipLaunchKernelGGL((k_scan<R, 16>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                            s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->tix.as<int32_t>(), s->rng.as<R>(), s->rescore_list.as<int32_t>(),
+                           s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
+                           s->rescore_list.as<int32_t>(),
                            s->rescore_count.as<uint32_t>(), fp);
     }
     PS2_LAUNCHED();
@@ -463,13 +472,14 @@ int launch_fit(ps2_session *s) {
         hipLaunchKernelGGL(k_rescore<R>, dim3(1024), dim3(64), 0, s->stream, s->phnm.as<cplx<R>>(),
                            s->bp64.as<double>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(),
-                           s->tix.as<int32_t>(), fp);
+                           s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp);
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_REFINE);
     hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                        s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                       s->tix.as<int32_t>(), s->rng.as<R>(), s->K.as<double>(), s->Cc.as<double>(),
+                       s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
+                       s->Cc.as<double>(),
                        s->coh.as<double>(), s->nopt.as<double>(), fp);
     PS2_LAUNCHED();
     return PS2_OK;
