@@ -29,6 +29,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# rocprofv3 --pmc on a plain 128 MiB device copy reads FETCH_SIZE = 65,548 KiB
+# and WRITE_SIZE = 131,072 KiB on gfx950: HBM bytes = 2.0 x FETCH + 1.0 x WRITE
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
@@ -60,6 +64,30 @@ def kernel_work(prep, name):
     return 0.0, 0.0
 
 
+KERNEL_SYMBOLS = {"clap_strip": "k_clap_strip", "scan": "k_scan", "refine": "k_refine",
+                  "fill_grid": "k_fill_grid", "clap_combine": "k_clap_combine",
+                  "rescore": "k_rescore", "rms_hist": "k_rms_hist", "weights": "k_weights"}
+
+
+def pmc_traffic(pmc_dir, kernel):
+    """HBM bytes per dispatch of `kernel` from rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE counter CSVs found under pmc_dir (calibrated)."""
+    import csv
+    import glob
+    sym = KERNEL_SYMBOLS.get(kernel, kernel)
+    tot = {"FETCH_SIZE": [], "WRITE_SIZE": []}
+    for f in glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if sym in r["Kernel_Name"] and r["Counter_Name"] in tot:
+                    tot[r["Counter_Name"]].append(float(r["Counter_Value"]))
+    if not tot["FETCH_SIZE"] or not tot["WRITE_SIZE"]:
+        return None
+    fetch = sum(tot["FETCH_SIZE"]) / len(tot["FETCH_SIZE"]) * 1024.0
+    write = sum(tot["WRITE_SIZE"]) / len(tot["WRITE_SIZE"]) * 1024.0
+    return FETCH_SCALE * fetch + WRITE_SCALE * write
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +99,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--nr-rows", type=int, default=300_000)
+    ap.add_argument("--pmc-dir", default="", help="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE outputs "
+                    "of this workload, for the roofline traffic field")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -138,7 +168,7 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": nbytes / avg_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = pmc_traffic(args.pmc_dir, dom) if args.pmc_dir else None
     roof["kernel"] = dom
     roof["avg_launch_us"] = avg_s * 1e6
 
@@ -174,6 +204,7 @@ def main():
                                 "trial_ix_agreement": agree,
                                 "coh_abs_err_max": float(dcoh.max()),
                                 "coh_abs_err_p999": float(np.quantile(dcoh, 0.999)),
+                                "coh_frac_above_1e-5": float(np.mean(dcoh > 1e-5)),
                                 "i_loop_equal": bool(o32["i_loop"] == o64["i_loop"])}
 
     if not args.no_cpu:

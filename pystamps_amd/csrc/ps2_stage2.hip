@@ -469,7 +469,7 @@ int launch_fit(ps2_session *s) {
     }
     if (sizeof(R) == 4) {
         SlotTimer tm(s, SLOT_RESCORE);
-        hipLaunchKernelGGL(k_rescore<R>, dim3(1024), dim3(64), 0, s->stream, s->phnm.as<cplx<R>>(),
+        hipLaunchKernelGGL(k_rescore<R>, dim3(std::min<int64_t>(s->P, 16384)), dim3(64), 0, s->stream, s->phnm.as<cplx<R>>(),
                            s->bp64.as<double>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(),
                            s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp);
