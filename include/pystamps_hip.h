@@ -150,6 +150,14 @@ int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_se
 int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out);
 void ps2_hip_session_destroy(ps2_session *s);
 
+/* Runs the loops of n sessions concurrently (each on its own stream and host
+ * thread; sessions may be on the same device) and returns after every loop's
+ * stop decision.  i_loops[k] / seconds[k] receive each session's iteration
+ * count and device loop time (either may be NULL).  This is the batched form
+ * of the reference's sequential per-patch calls (pystamps-native.rs:158-176). */
+int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loops,
+                         double *seconds);
+
 /* Per-kernel device time of the session's runs so far (the reference prints
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,

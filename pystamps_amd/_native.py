@@ -93,6 +93,7 @@ def lib() -> C.CDLL:
     L.ps2_hip_step_fill_grid.argtypes = [C.c_void_p, _f64p, _f64p, C.c_void_p]
     L.ps2_hip_step_clap.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.ps2_hip_step_fit.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(PatchOut)]
+    L.ps2_hip_sessions_run.argtypes = [C.c_size_t, C.POINTER(C.c_void_p), _i64p, _f64p]
     L.ps2_hip_session_set_timing.argtypes = [C.c_void_p, C.c_int32]
     L.ps2_hip_session_kernel_times.argtypes = [C.c_void_p, _f64p, _i64p]
     if L.ps2_hip_abi_version() != ABI_VERSION:

@@ -14,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ps2 {
@@ -820,16 +821,53 @@ int ps2_hip_run(const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *
     return rc;
 }
 
+int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loops, double *seconds) {
+    if (n && !sessions) return arg_error("sessions is NULL");
+    for (size_t q = 0; q < n; ++q)
+        if (!sessions[q]) return arg_error("session " + std::to_string(q) + " is NULL");
+    std::vector<int> rcs(n, PS2_OK);
+    std::vector<std::string> errs(n);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (size_t q = 0; q < n; ++q)
+        th.emplace_back([&, q]() {
+            int64_t il = 0;
+            double sec = 0.0;
+            rcs[q] = ps2_hip_session_run(sessions[q], &il, &sec);
+            if (rcs[q]) errs[q] = g_last_error;
+            if (i_loops) i_loops[q] = il;
+            if (seconds) seconds[q] = sec;
+        });
+    for (auto &t : th) t.join();
+    for (size_t q = 0; q < n; ++q)
+        if (rcs[q]) {
+            set_error("patch " + std::to_string(q) + ": " + errs[q]);
+            return rcs[q];
+        }
+    return PS2_OK;
+}
+
 int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts) {
     if (n && (!in || !out)) return arg_error("batch arrays are NULL");
+    std::vector<ps2_session *> ss(n, nullptr);
+    auto cleanup = [&]() {
+        for (auto *x : ss) ps2_hip_session_destroy(x);
+    };
     for (size_t q = 0; q < n; ++q) {
-        int rc = ps2_hip_run(&in[q], &out[q], opts);
+        int rc = session_create(&in[q], opts, &ss[q]);
         if (rc) {
             set_error("patch " + std::to_string(q) + ": " + g_last_error);
+            cleanup();
             return rc;
         }
     }
-    return PS2_OK;
+    int rc = ps2_hip_sessions_run(n, ss.data(), nullptr, nullptr);
+    for (size_t q = 0; q < n && !rc; ++q) {
+        rc = ps2_hip_session_download(ss[q], &out[q]);
+        if (rc) set_error("patch " + std::to_string(q) + ": " + g_last_error);
+    }
+    cleanup();
+    return rc;
 }
 
 // --- single steps -------------------------------------------------------------

@@ -151,6 +151,34 @@ def run_stage2_patch(raw: RawPatch, form: str = "fp32", device: int = 0,
     return summary, pm1
 
 
+def run_sessions(sessions) -> tuple[list[int], list[float]]:
+    """Runs the loops of several device-resident sessions concurrently."""
+    L = _native.lib()
+    n = len(sessions)
+    hs = (C.c_void_p * n)(*[x.h.value for x in sessions])
+    il = np.zeros(n, np.int64)
+    sec = np.zeros(n)
+    _check(L.ps2_hip_sessions_run(n, hs, _ptr(il, _i64p), _ptr(sec, _f64p)))
+    return [int(v) for v in il], [float(v) for v in sec]
+
+
+def run_stage2_hip_batch(preps, form: str = "fp32", device: int = 0,
+                         rescore_bound: float = 0.0) -> list[dict]:
+    """Several patches in one batched call (ps2_hip_run_batch)."""
+    L = _native.lib()
+    ins = [_InHolder(p) for p in preps]
+    outs = [_OutHolder(p) for p in preps]
+    n = len(preps)
+    arr_in = (_native.PatchIn * n)(*[x.s for x in ins])
+    arr_out = (_native.PatchOut * n)(*[x.s for x in outs])
+    _check(L.ps2_hip_run_batch(n, arr_in, arr_out, C.byref(_opts(form, device, rescore_bound, False))))
+    res = []
+    for k, p in enumerate(preps):
+        outs[k].s = arr_out[k]
+        res.append(outs[k].finish(p))
+    return res
+
+
 class Stage2Session:
     """Device-resident patch: repeated loop runs and single steps."""
 

@@ -289,3 +289,47 @@ def test_row_invariant_and_snr_paths(S, ora):
     assert g["i_loop"] == r["i_loop"]
     assert np.array_equal(g["trial_ix"], r["trial_ix"])
     np.testing.assert_allclose(g["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
+
+
+# --- batched patches ---------------------------------------------------------------
+
+def test_batch_of_uneven_patches(S, ora):
+    """Uneven patches in one batched call (concurrent sessions): each patch's
+    result equals its own single-patch run bit for bit, and the FP64 form
+    matches ref64 per patch, including per-patch iteration counts."""
+    preps = [small_prep(n_ps=n, n_ifg=20, seed=0x5EED1100 + k)
+             for k, n in enumerate((2500, 9000, 4000, 6500))]
+    for form in ("fp32", "fp64"):
+        batch = S.run_stage2_hip_batch(preps, form=form)
+        assert len(batch) == len(preps)
+        for p, b in zip(preps, batch):
+            single = S.run_stage2_hip(p, form=form)
+            assert b["i_loop"] == single["i_loop"]
+            for key in ("K_ps", "C_ps", "coh_ps", "N_opt", "ph_grid", "ph_weight", "ph_res",
+                        "trial_ix"):
+                assert np.array_equal(b[key], single[key], equal_nan=True), (form, key)
+            if form == "fp64":
+                r = ora.run(p, mode=ora.REF64)
+                assert b["i_loop"] == r["i_loop"]
+                assert np.array_equal(b["trial_ix"], r["trial_ix"])
+                np.testing.assert_allclose(b["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
+
+
+def test_sessions_run_concurrently_matches_sequential(S, prep_small):
+    p2 = small_prep(n_ps=3000, n_ifg=24, seed=0x5EED1200)
+    ss = [S.Stage2Session(prep_small, form="fp32"), S.Stage2Session(p2, form="fp32")]
+    try:
+        loops, secs = S.run_sessions(ss)
+        conc = [s.download() for s in ss]
+        seq = []
+        for s in ss:
+            il, _ = s.run()
+            seq.append(s.download())
+            assert il == loops[len(seq) - 1]
+        assert all(x > 0 for x in secs)
+        for a, b in zip(conc, seq):
+            for key in ("K_ps", "coh_ps", "ph_weight", "trial_ix"):
+                assert np.array_equal(a[key], b[key], equal_nan=True), key
+    finally:
+        for s in ss:
+            s.close()
