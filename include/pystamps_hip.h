@@ -158,7 +158,7 @@ void ps2_hip_session_destroy(ps2_session *s);
 int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loops,
                          double *seconds);
 
-/* Per-kernel device time of the session's runs so far (the reference prints
+/* Per-kernel device time of the session's runs and single steps so far (the reference prints
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
  * rescore, refine, rms_hist, weights.  ms[k] is the summed HIP-event time of

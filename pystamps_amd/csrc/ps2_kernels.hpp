@@ -251,7 +251,11 @@ __device__ inline R smooth7(const R (&h)[32], int c, const R (&k)[7]) {
 // LDS atomics in the team's tile, then exact selection among the few keys of
 // the bucket holding rank 511 (rank 512 is in the same bucket or is the
 // smallest key above it).  A crowded bucket falls back to the bitwise search.
-// hist (>= 1024 words) and cand (>= 64 words) are the team's LDS scratch.
+// Bucket b lives at word (b & 31) * 33 + (b >> 5): lane L owns buckets
+// [32L, 32L + 32), and both the per-lane totals (lane L reads q*33 + L) and
+// the owner's bucket row (lane q reads q*33 + owner) hit 32 distinct banks.
+// hist (>= 1056 words) and cand (>= 64 words, distinct from hist) are the
+// team's LDS scratch.
 template <typename R>
 __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi_team, R *hist_mem,
                                 R *cand_mem) {
@@ -274,15 +278,18 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         const int bits = 32 - __clz((int)span);
         const int shift = bits > 10 ? bits - 10 : 0;
 #pragma unroll
-        for (int q = 0; q < 32; ++q) hist[q * 32 + lane] = 0u;
+        for (int q = 0; q < 32; ++q) hist[q * 33 + lane] = 0u;
         wave_sync();
 #pragma unroll
-        for (int r = 0; r < 32; ++r) atomicAdd(&hist[(keys[r] - kmin) >> shift], 1u);
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t bk = (keys[r] - kmin) >> shift;
+            atomicAdd(&hist[(bk & 31u) * 33u + (bk >> 5)], 1u);
+        }
         wave_sync();
-        // lane L owns buckets [32L, 32L+32): local counts, team inclusive scan
+        // per-lane totals, team inclusive scan, owner of rank 511
         uint32_t tot = 0;
 #pragma unroll
-        for (int q = 0; q < 32; ++q) tot += hist[lane * 32 + q];
+        for (int q = 0; q < 32; ++q) tot += hist[q * 33 + lane];
         uint32_t incl = tot;
 #pragma unroll
         for (int o = 1; o < 32; o <<= 1) {
@@ -290,26 +297,25 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
             if (lane >= o) incl += v;
         }
         const uint32_t excl = incl - tot;
-        uint32_t b511 = 0, below511 = 0;
-        const bool owns = excl <= 511u && 511u < incl;
-        if (owns) {
-            uint32_t run = excl;
-            for (int q = 0; q < 32; ++q) {
-                const uint32_t c = hist[lane * 32 + q];
-                if (run + c > 511u) {
-                    b511 = (uint32_t)(lane * 32 + q);
-                    below511 = run;
-                    break;
-                }
-                run += c;
-            }
-        }
-        const unsigned long long fm = __ballot(owns);
+        const unsigned long long fm = __ballot(excl <= 511u && 511u < incl);
         const unsigned tm = hi_team ? (unsigned)(fm >> 32) : (unsigned)fm;
         const int owner = __ffs((int)tm) - 1;
-        b511 = __shfl(b511, owner, 32);
-        below511 = __shfl(below511, owner, 32);
-        const uint32_t n511 = hist[b511];
+        const uint32_t base = __shfl(excl, owner, 32);
+        // the owner's 32 buckets, one per lane, scanned the same way
+        const uint32_t c = hist[lane * 33 + owner];
+        uint32_t run = c;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t v = __shfl_up(run, o, 32);
+            if (lane >= o) run += v;
+        }
+        run += base;
+        const unsigned long long fq = __ballot(run > 511u);
+        const unsigned tq = hi_team ? (unsigned)(fq >> 32) : (unsigned)fq;
+        const int qs = __ffs((int)tq) - 1;
+        const uint32_t n511 = __shfl(c, qs, 32);
+        const uint32_t below511 = __shfl(run, qs, 32) - n511;
+        const uint32_t b511 = (uint32_t)(owner * 32 + qs);
         if (n511 > 64u) return team_median_bits<R>(keys, hi_team);
         // compact the keys of bucket b511 into cand, in team order
         uint32_t mine = 0;
@@ -326,19 +332,30 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         for (int r = 0; r < 32; ++r)
             if (((keys[r] - kmin) >> shift) == b511) cand[off++] = keys[r];
         wave_sync();
-        // rank of each candidate within the bucket (ties by position)
+        // rank of the candidates q = lane and q = lane + 32 within the bucket
+        // (ties by position), eight reads in flight per step
         const uint32_t j0 = 511u - below511;
-        uint32_t v0 = 0xffffffffu, v1 = 0xffffffffu;
-        for (uint32_t q = (uint32_t)lane; q < n511; q += 32u) {
-            const uint32_t x = cand[q];
-            uint32_t rank = 0;
-            for (uint32_t m = 0; m < n511; ++m) {
-                const uint32_t y = cand[m];
-                rank += (y < x || (y == x && m < q)) ? 1u : 0u;
+        const uint32_t qa = (uint32_t)lane, qb = (uint32_t)lane + 32u;
+        const uint32_t xa = qa < n511 ? cand[qa] : 0xffffffffu;
+        const uint32_t xb = qb < n511 ? cand[qb] : 0xffffffffu;
+        uint32_t ra = 0, rb = 0;
+        for (uint32_t m0 = 0; m0 < n511; m0 += 8u) {
+            uint32_t y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) y[u] = cand[m0 + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t m = m0 + (uint32_t)u;
+                const bool in = m < n511;
+                ra += (in && (y[u] < xa || (y[u] == xa && m < qa))) ? 1u : 0u;
+                rb += (in && (y[u] < xb || (y[u] == xb && m < qb))) ? 1u : 0u;
             }
-            if (rank == j0) v0 = x;
-            if (rank == j0 + 1u) v1 = x;
         }
+        uint32_t v0 = 0xffffffffu, v1 = 0xffffffffu;
+        if (qa < n511 && ra == j0) v0 = xa;
+        if (qb < n511 && rb == j0) v0 = xb;
+        if (qa < n511 && ra == j0 + 1u) v1 = xa;
+        if (qb < n511 && rb == j0 + 1u) v1 = xb;
         const uint32_t k511 = team_min_key<uint32_t>(v0);
         uint32_t k512 = team_min_key<uint32_t>(v1);
         if (j0 + 1u >= n511) {

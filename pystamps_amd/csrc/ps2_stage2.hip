@@ -910,6 +910,7 @@ int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *wei
                                       : launch_fill_grid<float>(s, s->K.as<double>(), s->W.as<double>());
     if (rc) return rc;
     PS2_HIP(hipStreamSynchronize(s->stream));
+    collect_timers(s);
     return plane_download(s, ph_grid_out, s->grid.p);
 }
 
@@ -922,6 +923,7 @@ int ps2_hip_step_clap(ps2_session *s, const void *ph_grid, void *ph_filt_out) {
     rc = s->form == PS2_FORM_FP64 ? launch_clap<double>(s, false) : launch_clap<float>(s, false);
     if (rc) return rc;
     PS2_HIP(hipStreamSynchronize(s->stream));
+    collect_timers(s);
     return plane_download(s, ph_filt_out, s->filt.p);
 }
 
@@ -946,6 +948,7 @@ int ps2_hip_step_fit(ps2_session *s, const void *ph_filt, ps2_patch_out *out) {
     if (s->form == PS2_FORM_FP32)
         PS2_HIP(hipMemcpyAsync(&cnt, s->rescore_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
     PS2_HIP(hipStreamSynchronize(s->stream));
+    collect_timers(s);
     out->n_rescored = cnt;
     return s->form == PS2_FORM_FP64 ? download_outputs<double>(s, out, false)
                                     : download_outputs<float>(s, out, false);
