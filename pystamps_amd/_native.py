@@ -11,7 +11,7 @@ import re
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = Path(os.environ.get("PS2_HIP_LIB", PKG / "libpystamps_hip.so"))
+LIB_PATH = Path(os.environ.get("PS2_HIP_LIB") or PKG / "libpystamps_hip.so")
 HEADER = PKG.parent / "include" / "pystamps_hip.h"
 
 ABI_VERSION = 1

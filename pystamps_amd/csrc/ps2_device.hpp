@@ -178,4 +178,68 @@ __device__ inline K team_min_key(K v) {
     return v;
 }
 
+// Team-wide (32-lane) primitives on 32-bit values.  With PS2_TEAM_DPP they use
+// DPP row permutations (rows of 16 lanes) and the gfx950 cross-row swap
+// v_permlane16_swap instead of LDS-routed permutes; a team is two DPP rows
+// (lanes 0-31 or 32-63 of the wavefront) and every lane of the team must be
+// active.
+#ifndef PS2_TEAM_DPP
+#define PS2_TEAM_DPP 1
+#endif
+
+#if PS2_TEAM_DPP
+// DPP move within rows of 16 lanes; sources outside the row read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline uint32_t dpp_u32(uint32_t v) {
+    return __builtin_amdgcn_update_dpp(0u, v, CTRL, ROW_MASK, 0xf, true);
+}
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141,
+              DPP_ROW_MIRROR = 0x140, DPP_ROW_SHR = 0x110, DPP_ROW_BCAST15 = 0x142;
+
+template <>
+__device__ inline uint32_t team_min_key<uint32_t>(uint32_t v) {
+    v = min(v, dpp_u32<DPP_QUAD_XOR1>(v));
+    v = min(v, dpp_u32<DPP_QUAD_XOR2>(v));
+    v = min(v, dpp_u32<DPP_ROW_HALF_MIRROR>(v));
+    v = min(v, dpp_u32<DPP_ROW_MIRROR>(v));
+    // rows 0<->1 and 2<->3: each lane sees both rows' minima
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return min((uint32_t)r[0], (uint32_t)r[1]);
+}
+#endif
+
+// inclusive prefix sum over the team's lanes
+__device__ inline uint32_t team_incl_sum_u32(uint32_t v) {
+#if PS2_TEAM_DPP
+    v += dpp_u32<DPP_ROW_SHR + 1>(v);
+    v += dpp_u32<DPP_ROW_SHR + 2>(v);
+    v += dpp_u32<DPP_ROW_SHR + 4>(v);
+    v += dpp_u32<DPP_ROW_SHR + 8>(v);
+    v += dpp_u32<DPP_ROW_BCAST15, 0xa>(v);  // row 0 total into row 1 (row 2's into row 3)
+    return v;
+#else
+    const int lane = threadIdx.x & 31;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const uint32_t w = __shfl_up(v, o, 32);
+        if (lane >= o) v += w;
+    }
+    return v;
+#endif
+}
+
+// v of team lane ql (team of lanes 0-31) or qh (lanes 32-63); ql and qh are
+// the same on every lane of the wavefront
+__device__ inline uint32_t team_pick_u32(uint32_t v, int ql, int qh, bool hi_team) {
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, ql & 31);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 32 + (qh & 31));
+    return hi_team ? b : a;
+}
+
+// first set bit of each team's half of a wavefront ballot (-1 if none)
+__device__ inline void team_first_lanes(unsigned long long m, int &ql, int &qh) {
+    ql = __ffs((unsigned)m) - 1;
+    qh = __ffs((unsigned)(m >> 32)) - 1;
+}
+
 }  // namespace ps2

@@ -290,44 +290,27 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         uint32_t tot = 0;
 #pragma unroll
         for (int q = 0; q < 32; ++q) tot += hist[q * 33 + lane];
-        uint32_t incl = tot;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t v = __shfl_up(incl, o, 32);
-            if (lane >= o) incl += v;
-        }
+        const uint32_t incl = team_incl_sum_u32(tot);
         const uint32_t excl = incl - tot;
-        const unsigned long long fm = __ballot(excl <= 511u && 511u < incl);
-        const unsigned tm = hi_team ? (unsigned)(fm >> 32) : (unsigned)fm;
-        const int owner = __ffs((int)tm) - 1;
-        const uint32_t base = __shfl(excl, owner, 32);
+        int ol, oh;
+        team_first_lanes(__ballot(excl <= 511u && 511u < incl), ol, oh);
+        const int owner = (hi_team ? oh : ol) & 31;
+        const uint32_t base = team_pick_u32(excl, ol, oh, hi_team);
         // the owner's 32 buckets, one per lane, scanned the same way
         const uint32_t c = hist[lane * 33 + owner];
-        uint32_t run = c;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t v = __shfl_up(run, o, 32);
-            if (lane >= o) run += v;
-        }
-        run += base;
-        const unsigned long long fq = __ballot(run > 511u);
-        const unsigned tq = hi_team ? (unsigned)(fq >> 32) : (unsigned)fq;
-        const int qs = __ffs((int)tq) - 1;
-        const uint32_t n511 = __shfl(c, qs, 32);
-        const uint32_t below511 = __shfl(run, qs, 32) - n511;
+        const uint32_t run = team_incl_sum_u32(c) + base;
+        int sl, sh;
+        team_first_lanes(__ballot(run > 511u), sl, sh);
+        const int qs = (hi_team ? sh : sl) & 31;
+        const uint32_t n511 = team_pick_u32(c, sl, sh, hi_team);
+        const uint32_t below511 = team_pick_u32(run, sl, sh, hi_team) - n511;
         const uint32_t b511 = (uint32_t)(owner * 32 + qs);
         if (n511 > 64u) return team_median_bits<R>(keys, hi_team);
         // compact the keys of bucket b511 into cand, in team order
         uint32_t mine = 0;
 #pragma unroll
         for (int r = 0; r < 32; ++r) mine += ((keys[r] - kmin) >> shift) == b511 ? 1u : 0u;
-        uint32_t off = mine;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t v = __shfl_up(off, o, 32);
-            if (lane >= o) off += v;
-        }
-        off -= mine;
+        uint32_t off = team_incl_sum_u32(mine) - mine;
 #pragma unroll
         for (int r = 0; r < 32; ++r)
             if (((keys[r] - kmin) >> shift) == b511) cand[off++] = keys[r];
