@@ -356,7 +356,6 @@ template <typename R, int TEAMS, bool ALPHA_ONE>
 __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(const cplx<R> *__restrict__ grid,
                                                            cplx<R> *__restrict__ part,
                                                            const R *__restrict__ lp,
-                                                           const R *__restrict__ wfac,
                                                            ClapParams<R> prm) {
     const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const bool hi_team = (threadIdx.x & 63) >= 32;
@@ -390,14 +389,31 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
     const int lr = lane - rs;
     const cplx<R> *gsrc = grid + (int64_t)f * ni * nj + (int64_t)(i1 + lane) * nj;
     const int lrc = lr < 0 ? 0 : (lr > CLAP_WIN - 1 ? CLAP_WIN - 1 : lr);
+    // window weight of output row lrc (native_stage2.rs:1014-1037): r + c +
+    // 1e-6 on a quadrant, mirrored, so with a(x) = min(x, 23 - x) the weight
+    // is a(row) + a(col) + 1e-6, here with the inverse FFT's 1/32^2 folded in.
+    // The integer sum is exact; in f32, k + fl32(1e-6) rounds like the f64
+    // value k + 1e-6 for every k in [0, 22], so this equals the f64 table
+    // rounded to R.
+    R wv[CLAP_WIN / 2];
+    {
+        const int arow = lrc < CLAP_WIN / 2 ? lrc : CLAP_WIN - 1 - lrc;
+        const R bias = std::is_same<R, float>::value ? (R)1.0e-6f : (R)1.0e-6;
+#pragma unroll
+        for (int q = 0; q < CLAP_WIN / 2; ++q)
+            wv[q] = ((R)(arow + q) + bias) * (R)(1.0 / (CLAP_FFT * CLAP_FFT));
+    }
     cplx<R> *dst = part + ((((int64_t)f * prm.nwi + ix) * 2 + (seg & 1)) * CLAP_WIN + lrc) * nj;
 
     for (int jx = jx0; jx < jx1; ++jx) {
-        // re-read the low-pass and window-weight tables (L1-resident) every
-        // window instead of holding 56 loop-invariant values in registers
-        const R *lpw = lp;
-        const R *wfw = wfac;
-        asm volatile("" : "+s"(lpw), "+s"(wfw));
+        // re-read the low-pass table (L1-resident) every window instead of
+        // holding 32 loop-invariant values in registers
+        const R *lpg = lp;
+        asm volatile("" : "+s"(lpg));
+        // the asm hides the address space; restore it so these stay global
+        // (not flat) loads, which do not count against the LDS wait counter
+        const __attribute__((address_space(1))) R *lpw =
+            (const __attribute__((address_space(1))) R *)lpg;
         int j1 = jx * CLAP_INC, cs = 0;
         if (j1 + CLAP_WIN > nj) {
             cs = j1 + CLAP_WIN - nj;
@@ -406,12 +422,19 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         R xr[32], xi[32];
         // window row `lane`, zero padded to 32 (NaN -> 0, native_stage2.rs:852-858)
         if (lane < CLAP_WIN && active) {
+            // two cells per 16-byte load (rows are 8-byte aligned; gfx950
+            // global loads accept that)
+            typedef R pair4 __attribute__((ext_vector_type(4), aligned(sizeof(cplx<R>))));
+            const pair4 *src = reinterpret_cast<const pair4 *>(gsrc + j1);
 #pragma unroll
-            for (int c = 0; c < CLAP_WIN; ++c) {
-                cplx<R> v = gsrc[j1 + c];
-                if (RT<R>::isnan_(v.re) || RT<R>::isnan_(v.im)) v = {(R)0, (R)0};
-                xr[c] = v.re;
-                xi[c] = v.im;
+            for (int c = 0; c < CLAP_WIN; c += 2) {
+                const pair4 v = src[c / 2];
+                const bool z0 = RT<R>::isnan_(v.x) || RT<R>::isnan_(v.y);
+                const bool z1 = RT<R>::isnan_(v.z) || RT<R>::isnan_(v.w);
+                xr[c] = z0 ? (R)0 : v.x;
+                xi[c] = z0 ? (R)0 : v.y;
+                xr[c + 1] = z1 ? (R)0 : v.z;
+                xi[c + 1] = z1 ? (R)0 : v.w;
             }
 #pragma unroll
             for (int c = CLAP_WIN; c < 32; ++c) xr[c] = xi[c] = (R)0;
@@ -494,7 +517,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
             if (out_row) {
 #pragma unroll
                 for (int e = 0; e < CLAP_WIN; ++e) {
-                    const R fct = wfw[lr * CLAP_WIN + e];
+                    const R fct = wv[e < CLAP_WIN / 2 ? e : CLAP_WIN - 1 - e];
                     ar[e] = fma(xr[e], fct, ar[e]);
                     ai[e] = fma(xi[e], fct, ai[e]);
                 }
@@ -513,7 +536,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
 #pragma unroll
                 for (int e = 0; e < CLAP_WIN; ++e) {
                     if (e < CLAP_WIN - cs) {
-                        const R fct = wfw[lr * CLAP_WIN + e];
+                        const R fct = wv[e < CLAP_WIN / 2 ? e : CLAP_WIN - 1 - e];
                         ar[e] += tbr[lane][e + cs] * fct;
                         ai[e] += tbi[lane][e + cs] * fct;
                     }

@@ -123,7 +123,7 @@ struct ps2_session {
     std::vector<double> nr_base;
     HistRule hist_rule;
     // device state
-    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, wfac, rng, offset;
+    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, offset;
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
@@ -269,20 +269,6 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
         for (int q = 0; q < CLAP_FFT * CLAP_FFT; ++q) lp[q] = (R)in->low_pass[q];
         if ((rc = s->lp.alloc(sizeof(R) * lp.size()))) return rc;
         PS2_HIP(hipMemcpy(s->lp.p, lp.data(), sizeof(R) * lp.size(), hipMemcpyHostToDevice));
-        // clap_window_weight (native_stage2.rs:1014-1037): r + c + 1e-6 on a
-        // quadrant, mirrored to the four quadrants
-        const int half = CLAP_WIN / 2;
-        const double inv_scale = 1.0 / (double)(CLAP_FFT * CLAP_FFT);
-        std::vector<R> wf(CLAP_WIN * CLAP_WIN);
-        for (int r = 0; r < CLAP_WIN; ++r)
-            for (int c = 0; c < CLAP_WIN; ++c) {
-                const int qr = r < half ? r : CLAP_WIN - 1 - r;
-                const int qc = c < half ? c : CLAP_WIN - 1 - c;
-                const double w = (double)qr + (double)qc + 1.0e-6;
-                wf[r * CLAP_WIN + c] = (R)(w * inv_scale);
-            }
-        if ((rc = s->wfac.alloc(sizeof(R) * wf.size()))) return rc;
-        PS2_HIP(hipMemcpy(s->wfac.p, wf.data(), sizeof(R) * wf.size(), hipMemcpyHostToDevice));
     }
     return PS2_OK;
 }
@@ -410,12 +396,10 @@ int launch_clap(ps2_session *s, bool unit) {
         const dim3 g(s->nwi * prm.ngroups * prm.nseg);
         if (prm.alpha_is_one)
             hipLaunchKernelGGL((k_clap_strip<R, TEAMS, true>), g, dim3(32 * TEAMS), 0, s->stream,
-                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
-                               s->wfac.as<R>(), prm);
+                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm);
         else
             hipLaunchKernelGGL((k_clap_strip<R, TEAMS, false>), g, dim3(32 * TEAMS), 0, s->stream,
-                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(),
-                               s->wfac.as<R>(), prm);
+                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm);
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
