@@ -68,16 +68,14 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
     grid[gid] = {sre, sim};
 }
 
-// The same step with the phase weights computed cooperatively: a workgroup
-// covers FILL_CB consecutive cells and FILL_FG interferogram planes; the
-// weights of the cells' pixels (a contiguous sorted-pixel range) are computed
-// with coalesced loads into LDS, then each thread adds its cell's weights in
-// ascending row order, so the sums are the same as k_fill_grid's.
-constexpr int FILL_CB = 64;
-constexpr int FILL_FG = 4;
-constexpr int FILL_CHUNK = 1024;
-
-template <typename R>
+// The same step with the phase weights computed cooperatively: a 256-thread
+// workgroup covers CB consecutive cells and FG = 256 / CB interferogram
+// planes; the weights of the cells' pixels (a contiguous sorted-pixel range)
+// are computed with coalesced loads into LDS, CHUNK pixels per plane at a
+// time, then each thread adds its cell's weights in ascending row order, so
+// the sums are the same as k_fill_grid's.  CHUNK is sized to the mean pixel
+// count of CB cells: LDS per workgroup bounds occupancy.
+template <typename R, int CB, int CHUNK>
 __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restrict__ phnm,
                                                           const R *__restrict__ bp, int bp_rowinv,
                                                           const double *__restrict__ K,
@@ -85,29 +83,32 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
                                                           const int32_t *__restrict__ cell_start,
                                                           cplx<R> *__restrict__ grid, int64_t C,
                                                           int I, int64_t P) {
-    __shared__ cplx<R> vals[FILL_FG][FILL_CHUNK];
+    constexpr int FG = 256 / CB;
+    __shared__ cplx<R> vals[FG][CHUNK];
     const int t = threadIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.x * FILL_CB;
-    const int f0 = blockIdx.y * FILL_FG;
-    const int64_t cend = c0 + FILL_CB < C ? c0 + FILL_CB : C;
+    const int64_t c0 = (int64_t)blockIdx.x * CB;
+    const int f0 = blockIdx.y * FG;
+    const int64_t cend = c0 + CB < C ? c0 + CB : C;
     const int32_t ks = cell_start[c0], ke = cell_start[cend];
-    const int64_t c = c0 + (t % FILL_CB);
-    const int fs = t / FILL_CB;
+    const int64_t c = c0 + (t % CB);
+    const int fs = t / CB;
     const int f = f0 + fs;
     const bool mine = c < C && f < I;
     const int32_t my0 = mine ? cell_start[c] : 0, my1 = mine ? cell_start[c + 1] : 0;
     R sre = (R)0, sim = (R)0;
-    for (int32_t base = ks; base < ke; base += FILL_CHUNK) {
-        const int n = ke - base < FILL_CHUNK ? ke - base : FILL_CHUNK;
+    for (int32_t base = ks; base < ke; base += CHUNK) {
+        const int n = ke - base < CHUNK ? ke - base : CHUNK;
+        for (int j = t; j < n; j += 256) {
+            const int64_t k = base + j;
+            const double kk = K[k], ww = W[k];
 #pragma unroll
-        for (int q = 0; q < FILL_FG; ++q) {
-            const int ff = f0 + q;
-            if (ff >= I) break;
-            for (int j = t; j < n; j += 256) {
-                const int64_t k = base + j;
-                const int64_t e = (int64_t)ff * P + k;
-                const R b = bp_rowinv ? bp[ff] : bp[e];
-                vals[q][j] = weight_value<R>(phnm[e], b, K[k], W[k]);
+            for (int q = 0; q < FG; ++q) {
+                const int ff = f0 + q;
+                if (ff < I) {
+                    const int64_t e = (int64_t)ff * P + k;
+                    const R b = bp_rowinv ? bp[ff] : bp[e];
+                    vals[q][j] = weight_value<R>(phnm[e], b, kk, ww);
+                }
             }
         }
         __syncthreads();
@@ -647,6 +648,44 @@ struct FitParams {
     float rescore_bound;  // FP32 form only
 };
 
+// Loop-invariant inputs of the trial scan (native_stage2.rs:1215-1225),
+// computed once per session: the pixel's bperp range max - min (1 when zero)
+// and, for the FP32 form, the unit trial step e^{-i phi} with
+// phi = (bp / range) * pi/4 per interferogram, with the scan's own f32
+// arithmetic.  wt may be NULL (FP64 form).  With row-invariant bperp every
+// pixel has the same range and wt holds one step per interferogram.
+template <typename R>
+__global__ __launch_bounds__(256) void k_trial_setup(const R *__restrict__ bp, int rowinv,
+                                                     int64_t P, int I, R *__restrict__ rng,
+                                                     cplx<R> *__restrict__ wt) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    const R *b = rowinv ? bp : bp + k;
+    const int64_t bs = rowinv ? 1 : P;
+    R mn = (R)INFINITY, mx = -(R)INFINITY;
+    for (int i = 0; i < I; ++i) {
+        const R bi = b[i * bs];
+        if (bi < mn) mn = bi;
+        if (bi > mx) mx = bi;
+    }
+    R range = mx - mn;
+    if (range == (R)0) range = (R)1;
+    rng[k] = range;
+    if (!wt) return;
+    const R pi4 = RT<R>::pi4();
+    if (!rowinv) {
+        for (int i = 0; i < I; ++i) {
+            R sn, cs;
+            RT<R>::sincos_((b[i * bs] / range) * pi4, &sn, &cs);
+            wt[i * P + k] = {cs, -sn};
+        }
+    } else if (k < I) {
+        R sn, cs;
+        RT<R>::sincos_((bp[k] / range) * pi4, &sn, &cs);
+        wt[k] = {cs, -sn};
+    }
+}
+
 // Trial scan of topofit_row (native_stage2.rs:1207-1256) for one pixel per
 // thread.  Trials are processed in chunks of TC positive trials t and their
 // negatives -t.
@@ -656,13 +695,16 @@ struct FitParams {
 //     Rows whose best-vs-second margin is below rescore_bound are queued for
 //     the FP64 re-score.
 //   FP64 form (R=double): every rotation is an exact sincos, as the reference.
-// Output: tix (selected trial index, -1 for invalid rows), rng (bperp range).
+// Output: tix (selected trial index, -1 for invalid rows) and offset (the
+// trial sum at the selected trial).  rng and wt come from k_trial_setup.
 template <typename R, int TC>
 __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                                               const R *__restrict__ bp,
+                                              const cplx<R> *__restrict__ wt,
                                               const cplx<R> *__restrict__ filt,
                                               const int32_t *__restrict__ cell_of,
-                                              int32_t *__restrict__ tix, R *__restrict__ rng,
+                                              int32_t *__restrict__ tix,
+                                              const R *__restrict__ rng,
                                               cplx<R> *__restrict__ offset,
                                               int32_t *__restrict__ rescore_list,
                                               uint32_t *__restrict__ rescore_count,
@@ -675,16 +717,9 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     const cplx<R> *fl = filt + cell_of[k];             // fl[i * C]
     const R *b = prm.rowinv ? bp : bp + k;
     const int64_t bs = prm.rowinv ? 0 : P;            // b[i * bs] (rowinv: b[i])
+    const cplx<R> *w1 = wt ? (prm.rowinv ? wt : wt + k) : nullptr;  // FP32: w1[i * bs]
     const int64_t C = prm.C;
-
-    R mn = (R)INFINITY, mx = -(R)INFINITY;
-    for (int i = 0; i < I; ++i) {
-        const R bi = prm.rowinv ? b[i] : b[i * bs];
-        if (bi < mn) mn = bi;
-        if (bi > mx) mx = bi;
-    }
-    R range = mx - mn;
-    if (range == (R)0) range = (R)1;
+    const R range = rng[k];
     const R pi4 = RT<R>::pi4();
     const int tn = prm.tn;
 
@@ -718,17 +753,15 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                 if (v.re == (R)0 && v.im == (R)0) valid = false;
                 denom += RT<R>::abs_(v.re, v.im);
             }
-            const R bi = prm.rowinv ? b[i] : b[i * bs];
-            const R phi = (bi / range) * pi4;
             if constexpr (sizeof(R) == 4) {
-                R s1, c1;
-                RT<R>::sincos_(phi, &s1, &c1);
-                const R wr = c1, wi = -s1;  // e^{-i phi}
+                const cplx<R> e1 = w1[i * bs];  // e^{-i phi}
+                const R wr = e1.re, wi = e1.im;
                 R zr, zi;
                 if (t0 == 0) {
                     zr = (R)1;
                     zi = (R)0;
                 } else {
+                    const R phi = ((prm.rowinv ? b[i] : b[i * bs]) / range) * pi4;
                     R sa, ca;
                     RT<R>::sincos_(phi * (R)t0, &sa, &ca);
                     zr = ca;
@@ -749,6 +782,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                     }
                 }
             } else {
+                const R phi = ((prm.rowinv ? b[i] : b[i * bs]) / range) * pi4;
 #pragma unroll
                 for (int q = 0; q < TC; ++q) {
                     const int t = t0 + q;
@@ -768,7 +802,6 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
         if (t0 == 0) {
             if (!valid) {
                 tix[k] = -1;
-                rng[k] = (R)1;
                 return;
             }
             if (denom == (R)0) denom = (R)1;
@@ -783,7 +816,6 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
         }
     }
     tix[k] = best_ix;
-    rng[k] = range;
     offset[k] = {bre, bim};
     if constexpr (sizeof(R) == 4) {
         if (prm.T > 1 && best - second < (R)prm.rescore_bound) {

@@ -123,7 +123,7 @@ struct ps2_session {
     std::vector<double> nr_base;
     HistRule hist_rule;
     // device state
-    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, offset;
+    DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, wt, offset;
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
@@ -282,6 +282,12 @@ int alloc_state(ps2_session *s) {
     if (s->windows)
         if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * 2 * CLAP_WIN * s->nj * I))) return rc;
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
+    if (sizeof(R) == 4)
+        if ((rc = s->wt.alloc(sizeof(cplx<R>) * (s->rowinv ? I : P * I)))) return rc;
+    hipLaunchKernelGGL(k_trial_setup<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
+                       s->bp.as<R>(), s->rowinv, P, (int)I, s->rng.as<R>(),
+                       sizeof(R) == 4 ? s->wt.as<cplx<R>>() : nullptr);
+    PS2_LAUNCHED();
     if ((rc = s->offset.alloc(sizeof(cplx<R>) * P))) return rc;
     if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_list.alloc(sizeof(int32_t) * P))) return rc;
@@ -336,15 +342,34 @@ void collect_timers(ps2_session *s) {
 
 // --- kernel launch helpers ---------------------------------------------------
 
+#ifndef PS2_FILL_CB
+#define PS2_FILL_CB 128
+#endif
+
 template <typename R>
 int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
     const int64_t n = s->C * s->I;
     SlotTimer tm(s, SLOT_FILL);
     if (sizeof(R) == 4) {
-        const dim3 grid(blocks_for(s->C, FILL_CB), blocks_for(s->I, FILL_FG));
-        hipLaunchKernelGGL(k_fill_grid_staged<R>, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp.as<R>(), s->rowinv, K, W, s->cell_start.as<int32_t>(),
-                           s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P);
+        // 128 cells x 2 planes per workgroup (measured fastest of 32 x 8,
+        // 64 x 4, 128 x 2 on a 500k x 60 patch); the LDS chunk covers ~1.5x
+        // the mean pixel count of 128 cells (larger ranges take several chunks)
+        constexpr int CB = PS2_FILL_CB;
+        const double mean = (double)s->P / (double)std::max<int64_t>(s->C, 1) * CB;
+        const dim3 grid(blocks_for(s->C, CB), blocks_for(s->I, 256 / CB));
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
+                               s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
+                               (int)s->I, s->P);
+        };
+        if (mean * 1.5 <= 256.0)
+            go(k_fill_grid_staged<R, CB, 256>);
+        else if (mean * 1.5 <= 512.0)
+            go(k_fill_grid_staged<R, CB, 512>);
+        else if (mean * 1.5 <= 1024.0 || CB > 128)
+            go(k_fill_grid_staged<R, CB, 1024>);
+        else
+            go(k_fill_grid_staged<R, CB, 2048>);
     } else {
         hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
@@ -439,13 +464,13 @@ int launch_fit(ps2_session *s) {
     SlotTimer tm(s, SLOT_SCAN);
     if (s->tn < 8) {
         hipLaunchKernelGGL((k_scan<R, 8>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
                            s->rescore_list.as<int32_t>(),
                            s->rescore_count.as<uint32_t>(), fp);
     } else {
         hipLaunchKernelGGL((k_scan<R, 16>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
                            s->rescore_list.as<int32_t>(),
                            s->rescore_count.as<uint32_t>(), fp);
