@@ -46,27 +46,87 @@ def clap_windows(n):
 
 
 def kernel_work(prep, name):
-    """Algorithmic work per launch: (flops, bytes)."""
+    """Algorithmic work per launch: (flops, HBM bytes).
+
+    CLAP: 143,360 flop per window and interferogram (forward + inverse 32x32
+    FFT2 at 2 * 5 N^2 log2 N^2 = 102,400, 7-tap separable smoothing 28,672,
+    pointwise gain 12,288); bytes = grid read + partial-plane write
+    (complex64).  Trial scan: one complex multiply-add (8 flop) per pixel,
+    interferogram and trial; bytes = ph_nm + trial step (8 + 8 B) per pixel and
+    interferogram, the filtered grid read once.  Refine: two passes over
+    ph_nm + bperp (12 B) and the filtered grid.  Fill: ph_nm + bperp read,
+    grid written."""
     P, I, T = prep.n_ps, prep.n_ifg, len(prep.trial_values)
     C = prep.n_i * prep.n_j
     W = clap_windows(prep.n_i) * clap_windows(prep.n_j)
     if name == "clap_strip":
-        # 2 FFT2(32) at 5 N^2 log2(N^2) + 7-tap separable smoothing + pointwise
         return 143_360.0 * W * I, 16.0 * C * I
     if name == "scan":
-        return 8.0 * P * I * T, 12.0 * P * I
+        return 8.0 * P * I * T, 16.0 * P * I + 8.0 * C * I
     if name == "refine":
-        return 24.0 * P * I, 12.0 * P * I
+        return 2 * 20.0 * P * I, 2 * 12.0 * P * I + 8.0 * C * I
     if name == "fill_grid":
         return 16.0 * P * I, 12.0 * P * I + 8.0 * C * I
     if name == "clap_combine":
-        return 0.0, 8.0 * C * I + 8.0 * C * I
+        return 0.0, 16.0 * C * I + 8.0 * C * I
+    if name == "rms_hist":
+        return 0.0, 16.0 * P
+    if name == "weights":
+        return 0.0, 24.0 * P
     return 0.0, 0.0
+
+
+def aggregate_roofline(prep, kt, peak_tflops):
+    """Time-weighted roofline over the loop's kernels: sum of per-kernel ideal
+    times max(F / peak_VALU, B / peak_HBM) over the sum of measured times."""
+    ideal = meas = 0.0
+    for name, (ms, n) in kt.items():
+        if not n:
+            continue
+        f, b = kernel_work(prep, name)
+        ideal += max(f / (peak_tflops * 1e12), b / (PEAK_HBM_GBS * 1e9)) * n
+        meas += ms * 1e-3
+    return ideal / meas if meas > 0 else None
 
 
 KERNEL_SYMBOLS = {"clap_strip": "k_clap_strip", "scan": "k_scan", "refine": "k_refine",
                   "fill_grid": "k_fill_grid", "clap_combine": "k_clap_combine",
                   "rescore": "k_rescore", "rms_hist": "k_rms_hist", "weights": "k_weights"}
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage2_fp32_500k_60ifg_pmc_traffic.md")
+
+
+def summary_traffic(path, kernel):
+    """Calibrated HBM bytes per launch of `kernel` from a committed rocprofv3
+    --pmc summary table (| symbol | FETCH_SIZE KiB | WRITE_SIZE KiB | ...)."""
+    sym = KERNEL_SYMBOLS.get(kernel, kernel)
+    try:
+        with open(path) as fh:
+            for line in fh:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) >= 3 and cols[0].startswith(sym):
+                    try:
+                        return FETCH_SCALE * float(cols[1]) * 1024.0 + WRITE_SCALE * float(cols[2]) * 1024.0
+                    except ValueError:
+                        continue
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """A plain 128 MiB device-to-device copy, three times: the reference
+    kernel for scaling FETCH_SIZE / WRITE_SIZE to HBM bytes."""
+    import torch
+    n = 128 * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "copies": 3}), flush=True)
 
 
 def pmc_traffic(pmc_dir, kernel):
@@ -100,9 +160,14 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--nr-rows", type=int, default=300_000)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--pmc-dir", default="", help="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE outputs "
                     "of this workload, for the roofline traffic field")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain-copy kernel that calibrates FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -147,16 +212,20 @@ def main():
             dev_s += secs
         wall = time.perf_counter() - t0
         kt = s.kernel_times()
+        t0 = time.perf_counter()
         out = s.download()
+        download = time.perf_counter() - t0
         s.close()
         log(f"[bench] {form}: {steps} loops, {iters} iterations, wall {wall * 1e3:.2f} ms, "
             f"device {dev_s * 1e3:.2f} ms, setup {setup:.2f}s")
         for name, (ms, n) in kt.items():
             if n:
                 log(f"[bench]   {name:13s} {ms / n * 1e3:9.1f} us/launch x {n}")
-        return {"wall": wall, "iters": iters, "dev": dev_s, "kt": kt, "out": out, "setup": setup}
+        return {"wall": wall, "iters": iters, "dev": dev_s, "kt": kt, "out": out, "setup": setup,
+                "download": download}
 
     r32 = time_form("fp32", args.steps, args.warmup)
+    e2e = prep_s + r32["setup"] + r32["wall"] / args.steps + r32["download"]
     value = P * I * r32["iters"] / r32["wall"]
     ms_per_step = r32["wall"] / args.steps * 1e3
 
@@ -171,9 +240,21 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": nbytes / avg_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = pmc_traffic(args.pmc_dir, dom) if args.pmc_dir else None
+    if roof["unit"] == "TFLOP/s":
+        # the library is built without packed FP32 instructions: one FP32
+        # FMA per lane per cycle, half the published packed peak
+        roof["frac_of_unpacked_fp32_peak"] = roof["achieved"] / (PEAK_FP32_TFLOPS / 2)
+    if args.pmc_dir:
+        roof["traffic"] = pmc_traffic(args.pmc_dir, dom)
+        roof["traffic_source"] = "rocprofv3 --pmc run given by --pmc-dir"
+    elif args.config == 2 and not args.n_ps:
+        roof["traffic"] = summary_traffic(TRAFFIC_SUMMARY, dom)
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_SUMMARY, ROOT) if roof["traffic"] else None
+    else:
+        roof["traffic"] = None
     roof["kernel"] = dom
     roof["avg_launch_us"] = avg_s * 1e6
+    roof["loop_time_weighted_frac"] = aggregate_roofline(prep, kt, PEAK_FP32_TFLOPS)
 
     result = {
         "metric": "stage-2 PS·ifg·iter/s, FP32+batched vs FP64 GPU at matched K/gamma error",
@@ -194,6 +275,9 @@ def main():
                    "form": "fp32 bulk + fp64 near-tie re-score"},
         "roofline": roof,
         "rescored_rows_per_loop": r32["out"]["n_rescored"],
+        "end_to_end_seconds": {"total": e2e, "prep_incl_gpu_nr": prep_s, "upload": r32["setup"],
+                               "loop": r32["wall"] / args.steps, "download": r32["download"]},
+        "kernel_us_per_launch": {k: round(v[0] / v[1] * 1e3, 1) for k, v in r32["kt"].items() if v[1]},
     }
 
     if not args.no_fp64:
@@ -202,30 +286,45 @@ def main():
         o32, o64 = r32["out"], r64["out"]
         agree = float(np.mean(o32["trial_ix"] == o64["trial_ix"]))
         dcoh = np.abs(o32["coh_ps"] - o64["coh_ps"])
+        ok = np.isfinite(o32["K_ps"]) & np.isfinite(o64["K_ps"])
+        dk = np.abs(o32["K_ps"][ok] - o64["K_ps"][ok])
+        dc = np.abs(np.angle(np.exp(1j * (o32["C_ps"] - o64["C_ps"]))))
+        dpp = np.abs(np.angle(o32["ph_patch"] * np.conj(o64["ph_patch"])))
         result["fp64_plain"] = {"value": v64, "ms_per_step": r64["wall"] / args.fp64_steps * 1e3,
                                 "fp32_speedup": value / v64,
                                 "trial_ix_agreement": agree,
                                 "coh_abs_err_max": float(dcoh.max()),
                                 "coh_abs_err_p999": float(np.quantile(dcoh, 0.999)),
                                 "coh_frac_above_1e-5": float(np.mean(dcoh > 1e-5)),
+                                "K_abs_err_p999": float(np.quantile(dk, 0.999)) if dk.size else None,
+                                "C_abs_err_p999": float(np.quantile(dc, 0.999)),
+                                "ph_patch_phase_err_p999": float(np.quantile(dpp, 0.999)),
+                                "rescored_rows_per_loop": r32["out"]["n_rescored"],
                                 "i_loop_equal": bool(o32["i_loop"] == o64["i_loop"])}
 
     if not args.no_cpu:
         from oracle import oracle as O
         threads = min(os.cpu_count() or 1, 16)
-        t0 = time.perf_counter()
-        ro = O.run(prep, mode=O.REF32, threads=threads, diagnostics=False)
-        cpu_s = time.perf_counter() - t0
+        # one warm-up, then the median of args.cpu_runs timed loops
+        O.run(prep, mode=O.REF32, threads=threads, diagnostics=False)
+        runs = []
+        for _ in range(args.cpu_runs):
+            t0 = time.perf_counter()
+            ro = O.run(prep, mode=O.REF32, threads=threads, diagnostics=False)
+            runs.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(runs))
         cpu_v = P * I * ro["i_loop"] / cpu_s
         result["cpu_baseline"] = {
             "value": cpu_v, "unit": "PS·ifg·iter/s", "cores": threads, "kind": "port",
             "sample": f"one whole gamma loop ({ro['i_loop']} iterations) of the same {P} x {I} "
-                      f"workload, oracle in the reference's precision map, OpenMP over rows/ifgs",
+                      f"workload, median of {args.cpu_runs} after a warm-up; oracle in the "
+                      f"reference's precision map, OpenMP over rows/ifgs",
             "seconds": cpu_s,
             "cpu": _cpu_model(),
         }
         result["gpu_vs_cpu"] = value / cpu_v
-        log(f"[bench] cpu ref32: {cpu_s:.2f}s for {ro['i_loop']} iterations on {threads} threads")
+        log(f"[bench] cpu ref32: {cpu_s:.2f}s (median of {args.cpu_runs}) for {ro['i_loop']} "
+            f"iterations on {threads} threads")
     print(json.dumps(result), flush=True)
 
 

@@ -27,6 +27,9 @@ constexpr int CLAP_FFT = 32;   // n_win_ex
 constexpr int CLAP_WIN = 24;   // n_win
 constexpr int CLAP_INC = 6;    // n_inc = n_win / 4
 
+// gfx950 dispatches workgroup b to XCD b % 8 (8 XCDs, each with its own L2)
+#define PS2_NUM_XCD 8
+
 // ---------------------------------------------------------------------------
 // fill_phase_weight value (native_stage2.rs:648-652):
 //   phase = -(bp * K); src * (cos, sin) * w   (all in R)
@@ -82,12 +85,20 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
                                                           const double *__restrict__ W,
                                                           const int32_t *__restrict__ cell_start,
                                                           cplx<R> *__restrict__ grid, int64_t C,
-                                                          int I, int64_t P) {
+                                                          int I, int64_t P, int npg) {
     constexpr int FG = 256 / CB;
     __shared__ cplx<R> vals[FG][CHUNK];
     const int t = threadIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.x * CB;
-    const int f0 = blockIdx.y * FG;
+    // XCD-aware order: cell group cg belongs to XCD cg % 8, and its npg
+    // plane groups are consecutive workgroups there, so the group's K, W and
+    // cell_start reads are shared through that XCD's L2
+    const int xcd = blockIdx.x % PS2_NUM_XCD;
+    const int loc = blockIdx.x / PS2_NUM_XCD;
+    const int cgl = loc / npg;
+    const int pg = loc - cgl * npg;
+    const int64_t c0 = ((int64_t)cgl * PS2_NUM_XCD + xcd) * CB;
+    if (c0 >= C) return;
+    const int f0 = pg * FG;
     const int64_t cend = c0 + CB < C ? c0 + CB : C;
     const int32_t ks = cell_start[c0], ke = cell_start[cend];
     const int64_t c = c0 + (t % CB);
@@ -163,6 +174,8 @@ struct ClapParams {
     int alpha_is_one;
     int ni, nj, I, nwi, nwj, ngroups;
     int nseg, wseg;   // window segments per row, windows per segment (>= 3)
+    int units;        // strips: ngroups * nseg (interferogram pair x segment)
+    int upx;          // strips per XCD: ceil(units / PS2_NUM_XCD)
 };
 
 // Wave-level LDS synchronisation: a team is half a wavefront and never shares
@@ -360,10 +373,17 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
                                                            ClapParams<R> prm) {
     const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const bool hi_team = (threadIdx.x & 63) >= 32;
-    const int seg = blockIdx.x % prm.nseg;
-    const int rowgrp = blockIdx.x / prm.nseg;
-    const int ix = rowgrp / prm.ngroups;
-    const int g = rowgrp - ix * prm.ngroups;
+    // XCD-aware order: workgroup b runs on XCD b % 8.  Strip u (interferogram
+    // pair g, segment seg) belongs to XCD u % 8, which walks its strips' window
+    // rows top to bottom, so the 4 window rows that read a grid row run close
+    // together on the XCD whose L2 holds it.
+    const int xcd = blockIdx.x % PS2_NUM_XCD;
+    const int loc = blockIdx.x / PS2_NUM_XCD;
+    const int ix = loc / prm.upx;
+    const int u = (loc - ix * prm.upx) * PS2_NUM_XCD + xcd;
+    if (u >= prm.units) return;
+    const int g = u / prm.nseg;
+    const int seg = u - g * prm.nseg;
     const int f = g * TEAMS + team;
     const int jx0 = seg * prm.wseg;
     const int jx1 = jx0 + prm.wseg < prm.nwj ? jx0 + prm.wseg : prm.nwj;

@@ -352,24 +352,26 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
     SlotTimer tm(s, SLOT_FILL);
     if (sizeof(R) == 4) {
         // 128 cells x 2 planes per workgroup (measured fastest of 32 x 8,
-        // 64 x 4, 128 x 2 on a 500k x 60 patch); the LDS chunk covers ~1.5x
-        // the mean pixel count of 128 cells (larger ranges take several chunks)
+        // 64 x 4, 128 x 2, 256 x 1 on a 500k x 60 patch); the LDS chunk is the
+        // smallest of 256 / 512 / 1024 pixels covering the mean pixel count of
+        // 128 cells (larger ranges take several chunks: LDS beyond 16 KB per
+        // workgroup costs more in occupancy than the extra passes)
         constexpr int CB = PS2_FILL_CB;
         const double mean = (double)s->P / (double)std::max<int64_t>(s->C, 1) * CB;
-        const dim3 grid(blocks_for(s->C, CB), blocks_for(s->I, 256 / CB));
+        const int64_t ncg = blocks_for(s->C, CB);
+        const int npg = (int)blocks_for(s->I, 256 / CB);
+        const dim3 grid((unsigned)(blocks_for(ncg, PS2_NUM_XCD) * PS2_NUM_XCD * npg));
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
                                s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
-                               (int)s->I, s->P);
+                               (int)s->I, s->P, npg);
         };
-        if (mean * 1.5 <= 256.0)
+        if (mean <= 256.0)
             go(k_fill_grid_staged<R, CB, 256>);
-        else if (mean * 1.5 <= 512.0)
+        else if (mean <= 512.0)
             go(k_fill_grid_staged<R, CB, 512>);
-        else if (mean * 1.5 <= 1024.0 || CB > 128)
-            go(k_fill_grid_staged<R, CB, 1024>);
         else
-            go(k_fill_grid_staged<R, CB, 2048>);
+            go(k_fill_grid_staged<R, CB, 1024>);
     } else {
         hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
@@ -404,6 +406,8 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
     nseg = std::min(nseg, std::max(1, s->nwj / 3));
     p.wseg = (s->nwj + nseg - 1) / nseg;
     p.nseg = (s->nwj + p.wseg - 1) / p.wseg;
+    p.units = p.ngroups * p.nseg;
+    p.upx = (p.units + PS2_NUM_XCD - 1) / PS2_NUM_XCD;
     return p;
 }
 
@@ -418,7 +422,7 @@ int launch_clap(ps2_session *s, bool unit) {
     const ClapParams<R> prm = clap_params<R>(s, TEAMS);
     {
         SlotTimer tm(s, SLOT_CLAP_STRIP);
-        const dim3 g(s->nwi * prm.ngroups * prm.nseg);
+        const dim3 g((unsigned)s->nwi * prm.upx * PS2_NUM_XCD);
         if (prm.alpha_is_one)
             hipLaunchKernelGGL((k_clap_strip<R, TEAMS, true>), g, dim3(32 * TEAMS), 0, s->stream,
                                s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm);
