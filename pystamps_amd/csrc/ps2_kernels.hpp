@@ -762,6 +762,66 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
         }
     };
 
+    if constexpr (sizeof(R) == 4) {
+        if (tn < TC) {
+            // one chunk: trials 0..tn and their negatives.  Trial 0 adds v
+            // itself and the recurrence starts at z = e^{-i phi}, which is what
+            // the general loop computes from z = 1 (exact multiplies by 1, 0).
+            // The next interferogram's three loads are issued before the
+            // current one is processed.
+            R pre[TC], pim[TC], nre[TC], nim[TC];
+#pragma unroll
+            for (int q = 0; q < TC; ++q) pre[q] = pim[q] = nre[q] = nim[q] = (R)0;
+            cplx<R> cf = fl[0], cn = nm[0], ce = w1[0];
+            for (int i = 0; i < I; ++i) {
+                const int i1 = i + 1 < I ? i + 1 : i;
+                const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P],
+                              ne = w1[(int64_t)i1 * bs];
+                const cplx<R> v = psdph_value<R>(cf, cn);
+                if (v.re == (R)0 && v.im == (R)0) valid = false;
+                denom += RT<R>::abs_(v.re, v.im);
+                pre[0] += v.re;
+                pim[0] += v.im;
+                const R wr = ce.re, wi = ce.im;
+                R zr = wr, zi = wi;
+#pragma unroll
+                for (int q = 1; q < TC; ++q) {
+                    if (q <= tn) {
+                        pre[q] = fmaf(v.re, zr, fmaf(-v.im, zi, pre[q]));
+                        pim[q] = fmaf(v.re, zi, fmaf(v.im, zr, pim[q]));
+                        nre[q] = fmaf(v.re, zr, fmaf(v.im, zi, nre[q]));
+                        nim[q] = fmaf(v.im, zr, fmaf(-v.re, zi, nim[q]));
+                        const R nzr = fmaf(zr, wr, -zi * wi);
+                        const R nzi = fmaf(zr, wi, zi * wr);
+                        zr = nzr;
+                        zi = nzi;
+                    }
+                }
+                cf = nf;
+                cn = nn;
+                ce = ne;
+            }
+            if (!valid) {
+                tix[k] = -1;
+                return;
+            }
+            if (denom == (R)0) denom = (R)1;
+#pragma unroll
+            for (int q = 0; q < TC; ++q) {
+                if (q <= tn) {
+                    offer(RT<R>::abs_(pre[q], pim[q]) / denom, q + tn, pre[q], pim[q]);
+                    if (q != 0) offer(RT<R>::abs_(nre[q], nim[q]) / denom, tn - q, nre[q], nim[q]);
+                }
+            }
+            tix[k] = best_ix;
+            offset[k] = {bre, bim};
+            if (prm.T > 1 && best - second < (R)prm.rescore_bound) {
+                const uint32_t slot = atomicAdd(rescore_count, 1u);
+                rescore_list[slot] = (int32_t)k;
+            }
+            return;
+        }
+    }
     for (int t0 = 0; t0 <= tn; t0 += TC) {
         // chunk: positive trials t0 .. t0+TC-1 (t0 = 0 holds trial 0) and their negatives
         R pre[TC], pim[TC], nre[TC], nim[TC];
