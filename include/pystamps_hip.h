@@ -138,6 +138,17 @@ int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
                                        int32_t device, double *out_nr,
                                        double *out_nr_max_nz_ix);
 
+/* Small-baseline branch of the same histogram (random_coherence_histogram,
+ * native_stage2.rs:1321-1334 with small_baseline_random_indices :1358-1403):
+ * n_rows x n_image uniform image phases, interferogram i's phase is
+ * image_b[i] - image_a[i] (0-based image indices, from ps1.ifgday_ix - 1). */
+int ps2_hip_random_coherence_histogram_sb(const double *bperp_nm, int64_t n_ifg,
+                                          const int32_t *image_a, const int32_t *image_b,
+                                          int64_t n_image, const double *trial_values,
+                                          int64_t n_trial, const double *coh_bins, int64_t n_rows,
+                                          uint32_t seed, int32_t device, double *out_nr,
+                                          double *out_nr_max_nz_ix);
+
 /* ---- Sessions: device-resident inputs, for repeated runs and per-step use ----
  * A session uploads a patch's constant inputs once.  ps2_hip_session_run then
  * runs the whole loop from the initial state (K = 0, weighting = 1/D_A) with

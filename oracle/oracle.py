@@ -90,6 +90,9 @@ def lib():
         L.ora_clap_window_weight.argtypes = [C.c_int64, _f64p]
         L.ora_random_coherence_histogram.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p,
                                                      C.c_int64, C.c_uint32, _f64p, _f64p, C.c_int]
+        L.ora_random_coherence_histogram_sb.argtypes = [_f64p, C.c_int64, _i32p, _i32p, C.c_int64,
+                                                        _f64p, C.c_int64, _f64p, C.c_int64,
+                                                        C.c_uint32, _f64p, _f64p, C.c_int]
         L.ora_topofit_phase_row_coh.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64]
         L.ora_topofit_phase_row_coh.restype = C.c_double
         L.ora_fft32_2d.argtypes = [_f64p, C.c_int]
@@ -318,6 +321,22 @@ def random_coherence_histogram(bperp_nm, trial_vals, coh_bins, n_rows=300_000, s
     lib().ora_random_coherence_histogram(_p(bp, _f64p), bp.size, _p(tv, _f64p), tv.size,
                                          _p(cb, _f64p), n_rows, seed, _p(nr, _f64p), C.byref(mx),
                                          threads)
+    return nr, mx.value
+
+
+def random_coherence_histogram_sb(bperp_nm, image_a, image_b, n_image, trial_vals, coh_bins,
+                                  n_rows=300_000, seed=2005, threads=0):
+    """Small-baseline Nr: image_a / image_b are 0-based image indices per ifg."""
+    bp = _c(bperp_nm, np.float64)
+    ia = _c(image_a, np.int32)
+    ib = _c(image_b, np.int32)
+    tv = _c(trial_vals, np.float64)
+    cb = _c(coh_bins, np.float64)
+    nr = np.zeros(COH_BINS)
+    mx = C.c_double()
+    lib().ora_random_coherence_histogram_sb(_p(bp, _f64p), bp.size, _p(ia, _i32p), _p(ib, _i32p),
+                                            int(n_image), _p(tv, _f64p), tv.size, _p(cb, _f64p),
+                                            n_rows, seed, _p(nr, _f64p), C.byref(mx), threads)
     return nr, mx.value
 
 

@@ -731,6 +731,11 @@ double ora_topofit_phase_row_coh(const double *phase, const double *bperp, int64
     return hypot(sr, si) / (double)(n_col > 1 ? n_col : 1);
 }
 
+static void nr_histogram_of_rows(const double *ph, const double *bperp_nm, int64_t n_ifg,
+                                 const double *trial_values, int64_t n_trial,
+                                 const double *coh_bins, int64_t n_rows, double *out_nr,
+                                 double *out_nr_max_nz_ix, int threads);
+
 void ora_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
                                     const double *trial_values, int64_t n_trial,
                                     const double *coh_bins, int64_t n_rows, uint32_t seed,
@@ -739,6 +744,41 @@ void ora_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
     double *ph = (double *)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
     ora_rng_uniform(seed, total, ph);
     for (int64_t k = 0; k < total; ++k) ph[k] *= 2.0 * ORA_PI;
+    nr_histogram_of_rows(ph, bperp_nm, n_ifg, trial_values, n_trial, coh_bins, n_rows, out_nr,
+                         out_nr_max_nz_ix, threads);
+    free(ph);
+}
+
+/* Small-baseline branch (native_stage2.rs:1321-1334): n_rows x n_image uniform
+ * phases (column-major draws, x 2 pi), interferogram phase = image b - image a
+ * (0-based image indices). */
+void ora_random_coherence_histogram_sb(const double *bperp_nm, int64_t n_ifg,
+                                       const int32_t *image_a, const int32_t *image_b,
+                                       int64_t n_image, const double *trial_values,
+                                       int64_t n_trial, const double *coh_bins, int64_t n_rows,
+                                       uint32_t seed, double *out_nr, double *out_nr_max_nz_ix,
+                                       int threads) {
+    int64_t nimg = n_rows * n_image;
+    double *img = (double *)malloc(sizeof(double) * (size_t)(nimg > 0 ? nimg : 1));
+    ora_rng_uniform(seed, nimg, img);
+    for (int64_t k = 0; k < nimg; ++k) img[k] *= 2.0 * ORA_PI;
+    int64_t total = n_rows * n_ifg;
+    double *ph = (double *)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+    for (int64_t i = 0; i < n_ifg; ++i)
+        for (int64_t r = 0; r < n_rows; ++r)
+            ph[r + i * n_rows] = img[r + (int64_t)image_b[i] * n_rows] - img[r + (int64_t)image_a[i] * n_rows];
+    nr_histogram_of_rows(ph, bperp_nm, n_ifg, trial_values, n_trial, coh_bins, n_rows, out_nr,
+                         out_nr_max_nz_ix, threads);
+    free(ph);
+    free(img);
+}
+
+/* histogram_random_rows (native_stage2.rs:1405-1436) over rows r of the
+ * column-major phase matrix ph[r + i * n_rows]. */
+static void nr_histogram_of_rows(const double *ph, const double *bperp_nm, int64_t n_ifg,
+                                 const double *trial_values, int64_t n_trial,
+                                 const double *coh_bins, int64_t n_rows, double *out_nr,
+                                 double *out_nr_max_nz_ix, int threads) {
     int eq = centers_equal_spacing(coh_bins, COH_BIN_COUNT);
     int64_t counts[COH_BIN_COUNT];
     memset(counts, 0, sizeof(counts));
@@ -764,7 +804,6 @@ void ora_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
         if (counts[k] > 0) last = k;
     }
     *out_nr_max_nz_ix = last >= 0 ? (double)(last + 1) : 1.0;
-    free(ph);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -110,6 +110,14 @@ void ora_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
                                     const double *trial_values, int64_t n_trial,
                                     const double *coh_bins, int64_t n_rows, uint32_t seed,
                                     double *out_nr, double *out_nr_max_nz_ix, int threads);
+/* small-baseline branch: interferogram phases are image-phase differences
+ * (image_a / image_b 0-based, native_stage2.rs:1321-1334) */
+void ora_random_coherence_histogram_sb(const double *bperp_nm, int64_t n_ifg,
+                                       const int32_t *image_a, const int32_t *image_b,
+                                       int64_t n_image, const double *trial_values,
+                                       int64_t n_trial, const double *coh_bins, int64_t n_rows,
+                                       uint32_t seed, double *out_nr, double *out_nr_max_nz_ix,
+                                       int threads);
 /* FP64 coherence of one phase row (native_stage2.rs:1438-1512). */
 double ora_topofit_phase_row_coh(const double *phase, const double *bperp, int64_t n_col,
                                  const double *trial_values, int64_t n_trial);

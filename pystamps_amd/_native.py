@@ -85,6 +85,10 @@ def lib() -> C.CDLL:
                                     C.POINTER(RunOpts)]
     L.ps2_hip_random_coherence_histogram.argtypes = [
         _f64p, C.c_int64, _f64p, C.c_int64, _f64p, C.c_int64, C.c_uint32, C.c_int32, _f64p, _f64p]
+    _i32p = C.POINTER(C.c_int32)
+    L.ps2_hip_random_coherence_histogram_sb.argtypes = [
+        _f64p, C.c_int64, _i32p, _i32p, C.c_int64, _f64p, C.c_int64, _f64p, C.c_int64, C.c_uint32,
+        C.c_int32, _f64p, _f64p]
     L.ps2_hip_session_create.argtypes = [C.POINTER(PatchIn), C.POINTER(RunOpts), C.POINTER(C.c_void_p)]
     L.ps2_hip_session_run.argtypes = [C.c_void_p, _i64p, _f64p]
     L.ps2_hip_session_download.argtypes = [C.c_void_p, C.POINTER(PatchOut)]

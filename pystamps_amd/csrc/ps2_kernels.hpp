@@ -1019,9 +1019,19 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     const cplx<R> off = offset[k];
     const R ocr = off.re, oci = -off.im;
     double num = 0.0, den = 0.0;
+    // both passes issue the next interferogram's loads before processing the
+    // current one
+    cplx<R> cf = fl[0], cn = nm[0];
+    R cb = bat(0);
     for (int i = 0; i < I; ++i) {
-        const cplx<R> v = psdph_value<R>(fl[i * C], nm[i * P]);
-        const R bi = bat(i);
+        const int i1 = i + 1 < I ? i + 1 : i;
+        const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P];
+        const R nb = bat(i1);
+        const cplx<R> v = psdph_value<R>(cf, cn);
+        const R bi = cb;
+        cf = nf;
+        cn = nn;
+        cb = nb;
         const double wgt = (double)RT<R>::abs_(v.re, v.im);
         const double wb = wgt * (double)bi;
         den += wb * wb;
@@ -1032,8 +1042,17 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     if (den == 0.0) den = 1.0;
     k0 = (R)((double)k0 + num / den);
     R mre = (R)0, mim = (R)0, dn = (R)0;
+    cf = fl[0];
+    cn = nm[0];
+    cb = bat(0);
     for (int i = 0; i < I; ++i) {
-        const cplx<R> r = rotate_neg<R>(psdph_value<R>(fl[i * C], nm[i * P]), k0 * bat(i));
+        const int i1 = i + 1 < I ? i + 1 : i;
+        const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P];
+        const R nb = bat(i1);
+        const cplx<R> r = rotate_neg<R>(psdph_value<R>(cf, cn), k0 * cb);
+        cf = nf;
+        cn = nn;
+        cb = nb;
         mre += r.re;
         mim += r.im;
         dn += RT<R>::abs_(r.re, r.im);

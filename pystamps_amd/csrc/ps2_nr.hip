@@ -139,11 +139,14 @@ using namespace ps2;
         }                                                                                  \
     } while (0)
 
-extern "C" int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
-                                                  const double *trial_values, int64_t n_trial,
-                                                  const double *coh_bins, int64_t n_rows,
-                                                  uint32_t seed, int32_t device, double *out_nr,
-                                                  double *out_nr_max_nz_ix) {
+namespace {
+
+// uniform phases, column-major [n_rows][n_ifg] (x 2 pi): single master draws
+// the interferogram columns directly (native_stage2.rs:1336-1349); small
+// baseline draws image columns and takes b - a (:1321-1334)
+int nr_check(const double *bperp_nm, int64_t n_ifg, const double *trial_values, int64_t n_trial,
+             const double *coh_bins, int64_t n_rows, const double *out_nr,
+             const double *out_nr_max_nz_ix) {
     if (!bperp_nm || !trial_values || !coh_bins || !out_nr || !out_nr_max_nz_ix || n_ifg <= 0 ||
         n_rows < 0) {
         set_error("random_coherence_histogram: invalid argument");
@@ -159,6 +162,13 @@ extern "C" int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_
             set_error("trial_values must be the consecutive integers -tn..tn");
             return PS2_E_ARG;
         }
+    return PS2_OK;
+}
+
+// FP64 fit of every row of ph on the device, histogram, Nr_max_nz_ix
+int nr_histogram(const std::vector<double> &ph, const double *bperp_nm, int64_t n_ifg,
+                 int64_t n_trial, const double *coh_bins, int64_t n_rows, int32_t device,
+                 double *out_nr, double *out_nr_max_nz_ix) {
     int rc = PS2_OK;
     double *d_ph = nullptr, *d_bp = nullptr, *d_mids = nullptr;
     uint32_t *d_hist = nullptr;
@@ -173,12 +183,6 @@ extern "C" int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_
             goto done;
         }
         NR_HIP(hipSetDevice(device));
-        // uniform draws, column-major [n_rows][n_ifg], scaled by 2 pi
-        const int64_t total = n_rows * n_ifg;
-        std::vector<double> ph((size_t)(total > 0 ? total : 1));
-        MatlabV5Uniform rng(seed);
-        const double two_pi = 2.0 * 3.14159265358979323846;
-        for (int64_t q = 0; q < total; ++q) ph[q] = rng.next() * two_pi;
         double mn = INFINITY, mx = -INFINITY, den = 0.0;
         for (int64_t i = 0; i < n_ifg; ++i) {
             mn = std::fmin(mn, bperp_nm[i]);
@@ -187,7 +191,7 @@ extern "C" int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_
         }
         p.n_rows = n_rows;
         p.I = (int)n_ifg;
-        p.tn = (int)tn;
+        p.tn = (int)(n_trial / 2);
         p.range = (mx - mn) == 0.0 ? 1.0 : (mx - mn);
         p.den_lin = den == 0.0 ? 1.0 : den;
         const HistRule hr = make_hist_rule(coh_bins, PS2_COH_BINS);
@@ -227,4 +231,52 @@ done:
     if (d_mids) (void)hipFree(d_mids);
     if (d_hist) (void)hipFree(d_hist);
     return rc;
+}
+
+constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+
+}  // namespace
+
+extern "C" int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_t n_ifg,
+                                                  const double *trial_values, int64_t n_trial,
+                                                  const double *coh_bins, int64_t n_rows,
+                                                  uint32_t seed, int32_t device, double *out_nr,
+                                                  double *out_nr_max_nz_ix) {
+    int rc = nr_check(bperp_nm, n_ifg, trial_values, n_trial, coh_bins, n_rows, out_nr, out_nr_max_nz_ix);
+    if (rc) return rc;
+    const int64_t total = n_rows * n_ifg;
+    std::vector<double> ph((size_t)(total > 0 ? total : 1));
+    MatlabV5Uniform rng(seed);
+    for (int64_t q = 0; q < total; ++q) ph[q] = rng.next() * kTwoPi;
+    return nr_histogram(ph, bperp_nm, n_ifg, n_trial, coh_bins, n_rows, device, out_nr, out_nr_max_nz_ix);
+}
+
+extern "C" int ps2_hip_random_coherence_histogram_sb(const double *bperp_nm, int64_t n_ifg,
+                                                     const int32_t *image_a, const int32_t *image_b,
+                                                     int64_t n_image, const double *trial_values,
+                                                     int64_t n_trial, const double *coh_bins,
+                                                     int64_t n_rows, uint32_t seed, int32_t device,
+                                                     double *out_nr, double *out_nr_max_nz_ix) {
+    int rc = nr_check(bperp_nm, n_ifg, trial_values, n_trial, coh_bins, n_rows, out_nr, out_nr_max_nz_ix);
+    if (rc) return rc;
+    if (!image_a || !image_b || n_image <= 0) {
+        set_error("random_coherence_histogram_sb: image indices are required");
+        return PS2_E_ARG;
+    }
+    for (int64_t i = 0; i < n_ifg; ++i)
+        if (image_a[i] < 0 || image_a[i] >= n_image || image_b[i] < 0 || image_b[i] >= n_image) {
+            set_error("random_coherence_histogram_sb: image index out of range at ifg " +
+                      std::to_string(i));
+            return PS2_E_ARG;
+        }
+    const int64_t nimg = n_rows * n_image;
+    std::vector<double> img((size_t)(nimg > 0 ? nimg : 1));
+    MatlabV5Uniform rng(seed);
+    for (int64_t q = 0; q < nimg; ++q) img[q] = rng.next() * kTwoPi;
+    const int64_t total = n_rows * n_ifg;
+    std::vector<double> ph((size_t)(total > 0 ? total : 1));
+    for (int64_t i = 0; i < n_ifg; ++i)
+        for (int64_t r = 0; r < n_rows; ++r)
+            ph[r + i * n_rows] = img[r + (int64_t)image_b[i] * n_rows] - img[r + (int64_t)image_a[i] * n_rows];
+    return nr_histogram(ph, bperp_nm, n_ifg, n_trial, coh_bins, n_rows, device, out_nr, out_nr_max_nz_ix);
 }

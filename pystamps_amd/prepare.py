@@ -237,15 +237,63 @@ def nr_hip(bperp_nm, trials, bins, n_rows=STAGE2_RANDOM_COUNT, seed=STAGE2_RANDO
     return nr, mx.value
 
 
+def nr_hip_sb(bperp_nm, image_a, image_b, n_image, trials, bins, n_rows=STAGE2_RANDOM_COUNT,
+              seed=STAGE2_RANDOM_SEED, device=0):
+    """Small-baseline Nr on the GPU (0-based image indices per interferogram)."""
+    L = _native.lib()
+    bp = np.ascontiguousarray(bperp_nm, dtype=np.float64)
+    ia = np.ascontiguousarray(image_a, dtype=np.int32)
+    ib = np.ascontiguousarray(image_b, dtype=np.int32)
+    tv = np.ascontiguousarray(trials, dtype=np.float64)
+    cb = np.ascontiguousarray(bins, dtype=np.float64)
+    nr = np.zeros(COH_BIN_COUNT)
+    mx = C.c_double()
+    i32 = C.POINTER(C.c_int32)
+    f64 = C.POINTER(C.c_double)
+    rc = L.ps2_hip_random_coherence_histogram_sb(
+        bp.ctypes.data_as(f64), bp.size, ia.ctypes.data_as(i32), ib.ctypes.data_as(i32),
+        int(n_image), tv.ctypes.data_as(f64), tv.size, cb.ctypes.data_as(f64), int(n_rows),
+        int(seed), int(device), nr.ctypes.data_as(f64), C.byref(mx))
+    if rc != 0:
+        raise Stage2Error(_native.last_error())
+    return nr, mx.value
+
+
+def small_baseline_random_indices(ifgday_ix, n_ifg: int):
+    """small_baseline_random_indices, native_stage2.rs:1358-1403: 0-based image
+    indices (a, b) per interferogram and the image count from ps1.ifgday_ix."""
+    if ifgday_ix is None:
+        raise Stage2Error("ps1.ifgday_ix is missing or invalid: variable not found")
+    m = np.asarray(ifgday_ix, dtype=np.float64)
+    if m.ndim == 1:
+        m = m.reshape(1, -1)
+    if m.shape == (n_ifg, 2):
+        pass
+    elif m.shape == (2, n_ifg):
+        m = m.T
+    else:
+        raise Stage2Error(f"ps1.ifgday_ix has incompatible shape {m.shape[0]}x{m.shape[1]} "
+                          f"for n_ifg={n_ifg}")
+    a = np.array([_round_half_away(v) for v in m[:, 0]], dtype=np.int64)
+    b = np.array([_round_half_away(v) for v in m[:, 1]], dtype=np.int64)
+    if np.any(a <= 0) or np.any(b <= 0):
+        raise Stage2Error("ps1.ifgday_ix must contain positive one-based image ids")
+    n_image = int(max(a.max(initial=0), b.max(initial=0)))
+    return (a - 1).astype(np.int32), (b - 1).astype(np.int32), n_image
+
+
 NrFn = Callable[..., tuple]
 
 
 def prepare_stage2_inputs(raw: RawPatch, nr_fn: Optional[NrFn] = None,
-                          nr_rows: int = STAGE2_RANDOM_COUNT, device: int = 0) -> Stage2Prepared:
+                          nr_rows: int = STAGE2_RANDOM_COUNT, device: int = 0,
+                          nr_sb_fn: Optional[NrFn] = None) -> Stage2Prepared:
     """prepare_stage2_inputs (native_stage2.rs:415-561) on in-memory arrays.
 
     ``nr_fn(bperp_nm, trial_values, coh_bins, n_rows)`` computes the random
-    coherence histogram; the default runs it on the GPU.
+    coherence histogram and ``nr_sb_fn(bperp_nm, image_a, image_b, n_image,
+    trial_values, coh_bins, n_rows)`` its small-baseline branch; the defaults
+    run them on the GPU.
     """
     parms = load_stage2_parms(raw.parms)
     options = load_stage2_options(raw.parms)
@@ -331,10 +379,13 @@ def prepare_stage2_inputs(raw: RawPatch, nr_fn: Optional[NrFn] = None,
     trials = trial_values(n_trial_wraps)
     low_coh_thresh = 15 if small_baseline else 31
     if small_baseline:
-        raise Stage2Error("small-baseline Nr (ps1.ifgday_ix image differences) is not implemented "
-                          "by the HIP backend")
-    fn = nr_fn if nr_fn is not None else (lambda bp, tv, cb, n: nr_hip(bp, tv, cb, n, device=device))
-    nr_base, nr_max_nz_ix = fn(bperp_nm, trials, bins, nr_rows)
+        ia, ib, n_image = small_baseline_random_indices(raw.ifgday_ix, n_ifg)
+        fn = nr_sb_fn if nr_sb_fn is not None else (
+            lambda bp, a, b, ni, tv, cb, n: nr_hip_sb(bp, a, b, ni, tv, cb, n, device=device))
+        nr_base, nr_max_nz_ix = fn(bperp_nm, ia, ib, n_image, trials, bins, nr_rows)
+    else:
+        fn = nr_fn if nr_fn is not None else (lambda bp, tv, cb, n: nr_hip(bp, tv, cb, n, device=device))
+        nr_base, nr_max_nz_ix = fn(bperp_nm, trials, bins, nr_rows)
 
     return Stage2Prepared(
         n_ps=n_ps, n_ifg=n_ifg, ph_nm=ph_nm, amp=amp, bperp_mat=bperp_mat,

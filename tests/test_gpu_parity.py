@@ -333,3 +333,39 @@ def test_sessions_run_concurrently_matches_sequential(S, prep_small):
     finally:
         for s in ss:
             s.close()
+
+
+# --- small-baseline branch ---------------------------------------------------------
+
+def test_gpu_nr_small_baseline_matches_oracle(ora):
+    from pystamps_amd.prepare import coh_bins, nr_hip_sb, trial_values
+    bp = np.linspace(-180.0, 210.0, 24)
+    rng = np.random.default_rng(5)
+    ia = rng.integers(0, 12, 24).astype(np.int32)
+    ib = ((ia + 1 + rng.integers(0, 6, 24)) % 13).astype(np.int32)
+    tv = trial_values(0.8)
+    nr_g, mx_g = nr_hip_sb(bp, ia, ib, 13, tv, coh_bins(), n_rows=20_000)
+    nr_o, mx_o = ora.random_coherence_histogram_sb(bp, ia, ib, 13, tv, coh_bins(), n_rows=20_000)
+    assert np.array_equal(nr_g, nr_o)
+    assert mx_g == mx_o
+
+
+def test_small_baseline_loop_fp64_vs_ref64(S, ora):
+    """Small-baseline patch: no master column dropped, image-difference Nr,
+    low-coherence threshold 15; the FP64 loop matches ref64."""
+    from pystamps_amd import synth
+    from pystamps_amd.prepare import prepare_stage2_inputs
+    cfg = dataclasses.replace(synth.scaled(synth.CONFIGS[1], 6000, 16), seed=0x5EED1300)
+    raw = synth.generate(cfg)
+    n_full = raw.ph.shape[1]
+    raw.parms = dict(raw.parms, small_baseline_flag="y")
+    raw.ifgday_ix = np.stack([np.arange(1, n_full + 1), np.arange(2, n_full + 2)], axis=1)
+    prep = prepare_stage2_inputs(
+        raw, nr_rows=20_000,
+        nr_sb_fn=lambda bp, a, b, ni, tv, cb, n: ora.random_coherence_histogram_sb(bp, a, b, ni, tv, cb, n_rows=n))
+    assert prep.small_baseline and prep.low_coh_thresh == 15 and prep.n_ifg == n_full
+    g = S.run_stage2_hip(prep, form="fp64")
+    r = ora.run(prep, mode=ora.REF64)
+    assert g["i_loop"] == r["i_loop"]
+    assert np.array_equal(g["trial_ix"], r["trial_ix"])
+    np.testing.assert_allclose(g["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)

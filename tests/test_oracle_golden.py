@@ -304,3 +304,38 @@ def test_ref32_and_ref64_loops_agree(ora):
     assert a["i_loop"] == b["i_loop"]
     assert np.mean(a["trial_ix"] == b["trial_ix"]) > 0.999
     assert np.median(np.abs(a["coh_ps"] - b["coh_ps"])) < 1e-6
+
+
+def test_small_baseline_nr_brute_force(ora):
+    """The small-baseline Nr composes pinned pieces (RNG, FP64 row fit,
+    histogram) on image-difference phases (native_stage2.rs:1321-1334): restate
+    the composition with numpy and compare counts exactly."""
+    from pystamps_amd.prepare import coh_bins, trial_values
+    bp = np.array([-31.0, -12.5, 4.0, 17.0, 40.0, 55.5])
+    ia = np.array([0, 0, 1, 2, 3, 1], np.int32)     # 0-based image ids
+    ib = np.array([1, 2, 3, 4, 4, 4], np.int32)
+    n_image, n_rows = 5, 400
+    tv = trial_values(0.9)
+    nr, mx = ora.random_coherence_histogram_sb(bp, ia, ib, n_image, tv, coh_bins(), n_rows=n_rows)
+    img = ora.rng_uniform(n_rows * n_image).reshape(n_image, n_rows).T * (2.0 * np.pi)  # column-major
+    ph = img[:, ib] - img[:, ia]
+    coh = np.array([ora.topofit_phase_row_coh(ph[r], bp, tv) for r in range(n_rows)])
+    ref = ora.hist_with_centers(coh, coh_bins())
+    assert np.array_equal(nr, ref)
+    nz = np.nonzero(ref)[0]
+    assert mx == (nz[-1] + 1 if nz.size else 1)
+    assert nr.sum() == n_rows
+
+
+def test_small_baseline_indices_and_errors():
+    from pystamps_amd.prepare import Stage2Error, small_baseline_random_indices
+    a, b, n = small_baseline_random_indices(np.array([[1, 2], [2, 3], [1, 4]]), 3)
+    assert list(a) == [0, 1, 0] and list(b) == [1, 2, 3] and n == 4
+    a2, b2, n2 = small_baseline_random_indices(np.array([[1, 2, 1], [2, 3, 4]]), 3)  # 2 x n_ifg
+    assert list(a2) == [0, 1, 0] and list(b2) == [1, 2, 3] and n2 == 4
+    with pytest.raises(Stage2Error, match="positive one-based"):
+        small_baseline_random_indices(np.array([[0, 2], [2, 3]]), 2)
+    with pytest.raises(Stage2Error, match="incompatible shape"):
+        small_baseline_random_indices(np.ones((3, 3)), 2)
+    with pytest.raises(Stage2Error, match="missing"):
+        small_baseline_random_indices(None, 2)
