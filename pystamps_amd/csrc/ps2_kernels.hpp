@@ -925,6 +925,9 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
                                                 const uint32_t *__restrict__ rescore_count,
                                                 int32_t *__restrict__ tix,
                                                 cplx<RF> *__restrict__ offset, FitParams prm) {
+    // lane l holds interferograms l, l + 64, ... (up to RESCORE_PER_LANE of
+    // them) for every trial; larger I re-reads the rest per trial
+    constexpr int RESCORE_PER_LANE = 4;
     const uint32_t n = *rescore_count;
     const int lane = threadIdx.x;
     for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
@@ -934,11 +937,33 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
         const cplx<RF> *nm = phnm + k;
         const cplx<RF> *fl = filt + cell_of[k];
         auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
+        double vr[RESCORE_PER_LANE], vi[RESCORE_PER_LANE], vb[RESCORE_PER_LANE];
         double denom = 0.0, mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+        for (int u = 0; u < RESCORE_PER_LANE; ++u) {
+            const int i = lane + 64 * u;
+            vr[u] = vi[u] = vb[u] = 0.0;
+            if (i < I) {
+                const cplx<RF> v = psdph_value<RF>(fl[(int64_t)i * C], nm[(int64_t)i * P]);
+                vr[u] = (double)v.re;
+                vi[u] = (double)v.im;
+                vb[u] = bat(i);
+            }
+        }
         for (int i = lane; i < I; i += 64) {
-            const cplx<RF> v = psdph_value<RF>(fl[i * C], nm[i * P]);
-            denom += hypot((double)v.re, (double)v.im);
-            const double bi = bat(i);
+            double re, im, bi;
+            const int u = i / 64;
+            if (u < RESCORE_PER_LANE) {
+                re = u == 0 ? vr[0] : u == 1 ? vr[1] : u == 2 ? vr[2] : vr[3];
+                im = u == 0 ? vi[0] : u == 1 ? vi[1] : u == 2 ? vi[2] : vi[3];
+                bi = u == 0 ? vb[0] : u == 1 ? vb[1] : u == 2 ? vb[2] : vb[3];
+            } else {
+                const cplx<RF> v = psdph_value<RF>(fl[(int64_t)i * C], nm[(int64_t)i * P]);
+                re = (double)v.re;
+                im = (double)v.im;
+                bi = bat(i);
+            }
+            denom += hypot(re, im);
             mn = bi < mn ? bi : mn;
             mx = bi > mx ? bi : mx;
         }
@@ -957,14 +982,24 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
             const double tv = (double)(t - prm.tn);
             double sr = 0.0, si = 0.0;
             for (int i = lane; i < I; i += 64) {
-                const cplx<RF> v = psdph_value<RF>(fl[i * C], nm[i * P]);
-                const double ph = ((bat(i) / range) * 0.78539816339744830962) * tv;
+                double re, im, bi;
+                const int u = i / 64;
+                if (u < RESCORE_PER_LANE) {
+                    re = u == 0 ? vr[0] : u == 1 ? vr[1] : u == 2 ? vr[2] : vr[3];
+                    im = u == 0 ? vi[0] : u == 1 ? vi[1] : u == 2 ? vi[2] : vi[3];
+                    bi = u == 0 ? vb[0] : u == 1 ? vb[1] : u == 2 ? vb[2] : vb[3];
+                } else {
+                    const cplx<RF> v = psdph_value<RF>(fl[(int64_t)i * C], nm[(int64_t)i * P]);
+                    re = (double)v.re;
+                    im = (double)v.im;
+                    bi = bat(i);
+                }
+                const double ph = ((bi / range) * 0.78539816339744830962) * tv;
                 double sn, cs;
                 sincos(ph, &sn, &cs);
                 const double nsn = -sn;
-                const double vr = (double)v.re, vi = (double)v.im;
-                sr += vr * cs - vi * nsn;
-                si += vr * nsn + vi * cs;
+                sr += re * cs - im * nsn;
+                si += re * nsn + im * cs;
             }
             sr = wave_sum_f64(sr);
             si = wave_sum_f64(si);
