@@ -50,8 +50,9 @@ def kernel_work(prep, name):
 
     CLAP: 143,360 flop per window and interferogram (forward + inverse 32x32
     FFT2 at 2 * 5 N^2 log2 N^2 = 102,400, 7-tap separable smoothing 28,672,
-    pointwise gain 12,288); bytes = grid read + partial-plane write
-    (complex64).  Trial scan: one complex multiply-add (8 flop) per pixel,
+    pointwise gain 12,288); bytes = grid read + the partial planes written
+    (each window row writes its 24 output rows: ~4x the grid, complex64); the
+    combine reads those planes and writes the filtered grid.  Trial scan: one complex multiply-add (8 flop) per pixel,
     interferogram and trial; bytes = ph_nm + trial step (8 + 8 B) per pixel and
     interferogram, the filtered grid read once.  Refine: two passes over
     ph_nm + bperp (12 B) and the filtered grid.  Fill: ph_nm + bperp read,
@@ -59,8 +60,9 @@ def kernel_work(prep, name):
     P, I, T = prep.n_ps, prep.n_ifg, len(prep.trial_values)
     C = prep.n_i * prep.n_j
     W = clap_windows(prep.n_i) * clap_windows(prep.n_j)
+    part = 8.0 * clap_windows(prep.n_i) * 24 * prep.n_j * I
     if name == "clap_strip":
-        return 143_360.0 * W * I, 16.0 * C * I
+        return 143_360.0 * W * I, 8.0 * C * I + part
     if name == "scan":
         return 8.0 * P * I * T, 16.0 * P * I + 8.0 * C * I
     if name == "refine":
@@ -68,7 +70,7 @@ def kernel_work(prep, name):
     if name == "fill_grid":
         return 16.0 * P * I, 12.0 * P * I + 8.0 * C * I
     if name == "clap_combine":
-        return 0.0, 16.0 * C * I + 8.0 * C * I
+        return 0.0, part + 8.0 * C * I
     if name == "rms_hist":
         return 0.0, 16.0 * P
     if name == "weights":
@@ -105,7 +107,7 @@ def summary_traffic(path, kernel):
         with open(path) as fh:
             for line in fh:
                 cols = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cols) >= 3 and cols[0].startswith(sym):
+                if len(cols) >= 3 and sym in cols[0]:
                     try:
                         return FETCH_SCALE * float(cols[1]) * 1024.0 + WRITE_SCALE * float(cols[2]) * 1024.0
                     except ValueError:
