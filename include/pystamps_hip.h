@@ -149,6 +149,37 @@ int ps2_hip_random_coherence_histogram_sb(const double *bperp_nm, int64_t n_ifg,
                                           uint32_t seed, int32_t device, double *out_nr,
                                           double *out_nr_max_nz_ix);
 
+/* ---- Kernels of the pySTAMPS kernel registry (a `hip` provider for
+ * pystamps/kernels/accelerated.py:392-426) ----
+ *
+ * Python-pipeline topofit of n_rows rows (stage2_topofit,
+ * stage2_topofit_row_invariant, stage2_topofit_coh_row_invariant; the fit is
+ * _ps_topofit_single, pystamps/pipeline/ported.py:2355-2430), in FP64: zero
+ * elements are left out of the fit, the local maxima of the trial coherence
+ * within 2e-4 of the maximum are refined, and the refined coherence selects
+ * among several (ported.py:2433-2477).  cpx [n_rows][n_ifg] complex128;
+ * bperp [n_rows][n_ifg], or [n_ifg] when row_invariant; k/c/coh [n_rows]
+ * (NaN for an all-zero row); res_out [n_rows][n_ifg] complex64 residual
+ * phasors (0 at zero elements), may be NULL. */
+int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int32_t row_invariant,
+                         int64_t n_rows, int64_t n_ifg, double n_trial_wraps, int32_t device,
+                         double *k_out, double *c_out, double *coh_out, float *res_out);
+
+/* stage2_grid_accumulate (_stage2_grid_accumulate_cpu,
+ * pystamps/kernels/accelerated.py:183-200: np.add.at per ifg, rows in index
+ * order, complex64): ph_weight [n_rows][n_ifg] complex64, grid_lin [n_rows]
+ * in [0, n_i*n_j); out [n_i*n_j][n_ifg] complex64.  An index out of range is
+ * PS2_E_ARG (numpy raises IndexError; negative indices are not wrapped). */
+int ps2_hip_grid_accumulate(const float *ph_weight, const int64_t *grid_lin, int64_t n_rows,
+                            int64_t n_ifg, int64_t n_i, int64_t n_j, int32_t device, float *out);
+
+/* stage2_histogram (_stage2_histogram_with_centers_cpu,
+ * pystamps/kernels/accelerated.py:240-264; the rule of hist_with_centers,
+ * native_stage2.rs:1538-1622): counts of the finite values per center;
+ * host code, no device use. */
+int ps2_hip_histogram_with_centers(const double *values, int64_t n, const double *centers,
+                                   int64_t n_centers, double *out);
+
 /* ---- Sessions: device-resident inputs, for repeated runs and per-step use ----
  * A session uploads a patch's constant inputs once.  ps2_hip_session_run then
  * runs the whole loop from the initial state (K = 0, weighting = 1/D_A) with

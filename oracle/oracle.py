@@ -93,6 +93,14 @@ def lib():
         L.ora_random_coherence_histogram_sb.argtypes = [_f64p, C.c_int64, _i32p, _i32p, C.c_int64,
                                                         _f64p, C.c_int64, _f64p, C.c_int64,
                                                         C.c_uint32, _f64p, _f64p, C.c_int]
+        L.ora_py_near_max_candidates.argtypes = [_f64p, C.c_int64, _i64p]
+        L.ora_py_near_max_candidates.restype = C.c_int64
+        L.ora_py_select_candidate.argtypes = [_i64p, _f64p, _f64p, C.c_int64, C.c_int64]
+        L.ora_py_select_candidate.restype = C.c_int64
+        L.ora_py_topofit_rows.argtypes = [_f64p, _f64p, C.c_int, C.c_int64, C.c_int64, C.c_double,
+                                          _f64p, _f64p, _f64p, _f64p]
+        L.ora_py_grid_accumulate.argtypes = [_f32p, _i64p, C.c_int64, C.c_int64, C.c_int64, _f32p]
+        L.ora_py_grid_accumulate.restype = C.c_int
         L.ora_topofit_phase_row_coh.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64]
         L.ora_topofit_phase_row_coh.restype = C.c_double
         L.ora_fft32_2d.argtypes = [_f64p, C.c_int]
@@ -356,3 +364,47 @@ def fft32_2d(x, inverse=False):
 
 def cpu_threads_default():
     return os.cpu_count() or 1
+
+
+# --- Python-pipeline registry kernels (pytopofit_oracle.c) ---------------------
+
+def py_near_max_candidates(coh_trial):
+    c = _c(coh_trial, np.float64)
+    out = np.zeros(max(c.size, 1), np.int64)
+    n = lib().ora_py_near_max_candidates(_p(c, _f64p), c.size, _p(out, _i64p))
+    return out[:n]
+
+
+def py_select_candidate(cand, coarse, refined, trial_count):
+    ca = _c(cand, np.int64)
+    co = _c(coarse, np.float64)
+    rf = _c(refined, np.float64)
+    return int(lib().ora_py_select_candidate(_p(ca, _i64p), _p(co, _f64p), _p(rf, _f64p), ca.size,
+                                             int(trial_count)))
+
+
+def py_topofit_rows(cpxphase, bperp, n_trial_wraps):
+    """stage2_topofit / _row_invariant semantics in FP64: (K, C, coh, residual)."""
+    cpx = np.ascontiguousarray(cpxphase, dtype=np.complex128)
+    P, I = cpx.shape
+    bp = np.asarray(bperp, dtype=np.float64)
+    rowinv = bp.ndim == 1
+    bp = np.ascontiguousarray(bp)
+    K, Cc, coh = np.zeros(P), np.zeros(P), np.zeros(P)
+    res = np.zeros((P, I), np.complex128)
+    lib().ora_py_topofit_rows(cpx.ctypes.data_as(_f64p), _p(bp, _f64p), int(rowinv), P, I,
+                              float(n_trial_wraps), _p(K, _f64p), _p(Cc, _f64p), _p(coh, _f64p),
+                              res.ctypes.data_as(_f64p))
+    return K, Cc, coh, res
+
+
+def py_grid_accumulate(ph_weight, grid_lin, n_i, n_j):
+    ph = np.ascontiguousarray(ph_weight, dtype=np.complex64)
+    gl = _c(grid_lin, np.int64)
+    P, I = ph.shape
+    out = np.zeros((n_i * n_j, I), np.complex64)
+    rc = lib().ora_py_grid_accumulate(ph.ctypes.data_as(_f32p), _p(gl, _i64p), P, I, n_i * n_j,
+                                      out.ctypes.data_as(_f32p))
+    if rc != 0:
+        raise IndexError("grid index out of range")
+    return out.reshape(n_i, n_j, I)

@@ -89,6 +89,11 @@ def lib() -> C.CDLL:
     L.ps2_hip_random_coherence_histogram_sb.argtypes = [
         _f64p, C.c_int64, _i32p, _i32p, C.c_int64, _f64p, C.c_int64, _f64p, C.c_int64, C.c_uint32,
         C.c_int32, _f64p, _f64p]
+    L.ps2_hip_topofit_rows.argtypes = [_f64p, _f64p, C.c_int32, C.c_int64, C.c_int64, C.c_double,
+                                       C.c_int32, _f64p, _f64p, _f64p, C.POINTER(C.c_float)]
+    L.ps2_hip_grid_accumulate.argtypes = [C.POINTER(C.c_float), _i64p, C.c_int64, C.c_int64,
+                                          C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_float)]
+    L.ps2_hip_histogram_with_centers.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p]
     L.ps2_hip_session_create.argtypes = [C.POINTER(PatchIn), C.POINTER(RunOpts), C.POINTER(C.c_void_p)]
     L.ps2_hip_session_run.argtypes = [C.c_void_p, _i64p, _f64p]
     L.ps2_hip_session_download.argtypes = [C.c_void_p, C.POINTER(PatchOut)]

@@ -1,0 +1,382 @@
+// Kernels of the pySTAMPS kernel registry on the GPU (gfx950): the `hip`
+// provider of stage2_topofit / stage2_topofit_row_invariant /
+// stage2_topofit_coh_row_invariant, stage2_grid_accumulate and
+// stage2_histogram (pystamps/kernels/accelerated.py:392-426).
+//
+// The topofit kernels follow the Python pipeline's single-row fit
+// (_ps_topofit_single, pystamps/pipeline/ported.py:2355-2430), in FP64:
+// zero elements are left out of the fit, every local maximum of the trial
+// coherence within 2e-4 of the maximum is refined, and the refined coherence
+// picks among several candidates (endpoint pairs keep the coarse argmax).
+#include "pystamps_hip.h"
+#include "ps2_host_util.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace ps2 {
+namespace {
+
+constexpr double kPi4 = 0.78539816339744830962;
+constexpr double kNearMaxTol = 2.0e-4;
+
+struct PyFitParams {
+    int64_t P;
+    int I, T, tn, rowinv;
+};
+
+// exp(-i ph) * (re + i im)
+__device__ inline void rot(double re, double im, double ph, double &orr, double &oi) {
+    double s, c;
+    sincos(ph, &s, &c);
+    orr = re * c + im * s;
+    oi = im * c - re * s;
+}
+
+// _ps_topofit_refine_candidate (ported.py:2480-2513) of row k; writes the
+// residual when res != nullptr.
+__device__ void py_refine(const double2 *cpx, const double *bp, int64_t bs, int64_t P, int64_t k,
+                          int I, double den_lin, double k0, double &K, double &C, double &coh,
+                          float2 *res, int64_t rs) {
+    double ore = 0.0, oim = 0.0;
+    for (int i = 0; i < I; ++i) {
+        const double2 v = cpx[(int64_t)i * P + k];
+        if (v.x == 0.0 && v.y == 0.0) continue;
+        double rr, ri;
+        rot(v.x, v.y, k0 * bp[i * bs], rr, ri);
+        ore += rr;
+        oim += ri;
+    }
+    double num = 0.0;
+    for (int i = 0; i < I; ++i) {
+        const double2 v = cpx[(int64_t)i * P + k];
+        if (v.x == 0.0 && v.y == 0.0) continue;
+        const double b = bp[i * bs];
+        double rr, ri;
+        rot(v.x, v.y, k0 * b, rr, ri);
+        const double w = hypot(v.x, v.y);
+        const double pr = rr * ore + ri * oim, pi = ri * ore - rr * oim;
+        num += (w * b) * (w * atan2(pi, pr));
+    }
+    const double kk = k0 + num / den_lin;
+    double mr = 0.0, mi = 0.0, d2 = 0.0;
+    for (int i = 0; i < I; ++i) {
+        const double2 v = cpx[(int64_t)i * P + k];
+        double rr = 0.0, ri = 0.0;
+        if (v.x != 0.0 || v.y != 0.0) {
+            rot(v.x, v.y, kk * bp[i * bs], rr, ri);
+            mr += rr;
+            mi += ri;
+            d2 += hypot(rr, ri);
+        }
+        if (res) res[(int64_t)i * rs + k] = make_float2((float)rr, (float)ri);
+    }
+    if (d2 == 0.0) d2 = 1.0;
+    K = kk;
+    C = atan2(mi, mr);
+    coh = hypot(mr, mi) / d2;
+}
+
+// one thread per row; cpx, bp ifg-major ([I][P]; bp [I] when rowinv);
+// cohs [T][P] scratch; res [I][P] complex64 (may be null)
+__global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ cpx,
+                                                    const double *__restrict__ bpv,
+                                                    double *__restrict__ cohs,
+                                                    double *__restrict__ Kout,
+                                                    double *__restrict__ Cout,
+                                                    double *__restrict__ cohout,
+                                                    float2 *__restrict__ res, PyFitParams p) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= p.P) return;
+    const int I = p.I, T = p.T;
+    const double *bp = p.rowinv ? bpv : bpv + k;
+    const int64_t bs = p.rowinv ? 1 : p.P;
+    double mn = INFINITY, mx = -INFINITY, denom = 0.0, den_lin = 0.0;
+    int m = 0;
+    for (int i = 0; i < I; ++i) {
+        const double2 v = cpx[(int64_t)i * p.P + k];
+        if (v.x == 0.0 && v.y == 0.0) continue;
+        const double b = bp[i * bs];
+        mn = fmin(mn, b);
+        mx = fmax(mx, b);
+        const double w = hypot(v.x, v.y);
+        denom += w;
+        den_lin += (w * b) * (w * b);
+        ++m;
+    }
+    if (m == 0) {
+        Kout[k] = Cout[k] = cohout[k] = __builtin_nan("");
+        if (res)
+            for (int i = 0; i < I; ++i) res[(int64_t)i * p.P + k] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    double range = mx - mn;
+    if (range == 0.0) range = 1.0;
+    if (denom == 0.0) denom = 1.0;
+    if (den_lin == 0.0) den_lin = 1.0;
+    double cmax = -INFINITY;
+    int amax = 0;
+    for (int t = 0; t < T; ++t) {
+        const double tm = (double)(t - p.tn);
+        double sr = 0.0, si = 0.0;
+        for (int i = 0; i < I; ++i) {
+            const double2 v = cpx[(int64_t)i * p.P + k];
+            if (v.x == 0.0 && v.y == 0.0) continue;
+            double rr, ri;
+            rot(v.x, v.y, (bp[i * bs] / range * kPi4) * tm, rr, ri);
+            sr += rr;
+            si += ri;
+        }
+        const double c = hypot(sr, si) / denom;
+        cohs[(int64_t)t * p.P + k] = c;
+        if (c > cmax) {
+            cmax = c;
+            amax = t;
+        }
+    }
+    // candidates in ascending trial order (ported.py:2433-2448) and the
+    // selection rule (:2451-2477), tracked on the fly
+    int n = 0, first = -1, last = -1, coarse_best = -1, refined_best = -1;
+    double coarse_v = -INFINITY, refined_v = -INFINITY;
+    for (int t = 0; t < T; ++t) {
+        const double c = cohs[(int64_t)t * p.P + k];
+        bool lm;
+        if (T == 1)
+            lm = true;
+        else if (t == 0)
+            lm = c >= cohs[(int64_t)1 * p.P + k];
+        else if (t == T - 1)
+            lm = c >= cohs[(int64_t)(T - 2) * p.P + k];
+        else
+            lm = c >= cohs[(int64_t)(t - 1) * p.P + k] && c >= cohs[(int64_t)(t + 1) * p.P + k];
+        if (!(lm && c >= cmax - kNearMaxTol)) continue;
+        if (n == 0) first = t;
+        last = t;
+        ++n;
+        if (c > coarse_v) {
+            coarse_v = c;
+            coarse_best = t;
+        }
+        double Kc, Cc, rc;
+        py_refine(cpx, bp, bs, p.P, k, I, den_lin, kPi4 / range * (double)(t - p.tn), Kc, Cc, rc,
+                  nullptr, 0);
+        if (rc > refined_v) {
+            refined_v = rc;
+            refined_best = t;
+        }
+    }
+    int sel;
+    if (n == 0)
+        sel = amax;
+    else if (n == 1 || (n == 2 && first == 0 && last == T - 1))
+        sel = coarse_best;
+    else
+        sel = refined_best;
+    double K, C, coh;
+    py_refine(cpx, bp, bs, p.P, k, I, den_lin, kPi4 / range * (double)(sel - p.tn), K, C, coh, res,
+              p.P);
+    Kout[k] = K;
+    Cout[k] = C;
+    cohout[k] = coh;
+}
+
+// np.add.at order: thread per (cell, ifg), ifg fastest; the cell's rows in
+// ascending row order, complex64 sums
+__global__ __launch_bounds__(256) void k_py_grid_accumulate(const float2 *__restrict__ ph,
+                                                            const int64_t *__restrict__ order,
+                                                            const int64_t *__restrict__ cell_start,
+                                                            float2 *__restrict__ out, int64_t C,
+                                                            int I) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= C * I) return;
+    const int64_t c = gid / I;
+    const int i = (int)(gid - c * I);
+    float sr = 0.0f, si = 0.0f;
+    for (int64_t q = cell_start[c]; q < cell_start[c + 1]; ++q) {
+        const float2 v = ph[order[q] * I + i];
+        sr += v.x;
+        si += v.y;
+    }
+    out[gid] = make_float2(sr, si);
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)); }
+};
+
+#define REG_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_error(std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)); \
+            return e_ == hipErrorOutOfMemory ? PS2_E_NOMEM : PS2_E_DEVICE;                \
+        }                                                                                 \
+    } while (0)
+
+int set_device(int32_t device) {
+    int n = 0;
+    REG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        set_error("device " + std::to_string(device) + " not present (" + std::to_string(n) +
+                  " visible)");
+        return PS2_E_ARG;
+    }
+    REG_HIP(hipSetDevice(device));
+    return PS2_OK;
+}
+
+}  // namespace
+}  // namespace ps2
+
+using namespace ps2;
+
+extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int32_t row_invariant,
+                                    int64_t n_rows, int64_t n_ifg, double n_trial_wraps,
+                                    int32_t device, double *k_out, double *c_out,
+                                    double *coh_out, float *res_out) {
+    if (n_rows < 0 || n_ifg <= 0 || (n_rows > 0 && (!cpx || !bperp || !k_out || !c_out || !coh_out))) {
+        set_error("topofit_rows: invalid argument");
+        return PS2_E_ARG;
+    }
+    if (!(n_trial_wraps >= 0.0) || !std::isfinite(n_trial_wraps) || n_trial_wraps > 1.0e6) {
+        set_error("topofit_rows: n_trial_wraps must be finite and >= 0");
+        return PS2_E_ARG;
+    }
+    if (n_rows == 0) return PS2_OK;
+    if (n_rows * n_ifg >= ((int64_t)1 << 31) / 2) {
+        set_error("topofit_rows: n_rows * n_ifg too large for one call");
+        return PS2_E_ARG;
+    }
+    int rc = set_device(device);
+    if (rc) return rc;
+    const int64_t P = n_rows, I = n_ifg;
+    const int tn = (int)std::ceil(8.0 * n_trial_wraps);
+    const int T = 2 * tn + 1;
+    // ifg-major copies for coalesced per-row threads
+    std::vector<double> hc(2 * P * I), hb(row_invariant ? I : P * I);
+    for (int64_t k = 0; k < P; ++k)
+        for (int64_t i = 0; i < I; ++i) {
+            hc[2 * (i * P + k)] = cpx[2 * (k * I + i)];
+            hc[2 * (i * P + k) + 1] = cpx[2 * (k * I + i) + 1];
+            if (!row_invariant) hb[i * P + k] = bperp[k * I + i];
+        }
+    if (row_invariant)
+        for (int64_t i = 0; i < I; ++i) hb[i] = bperp[i];
+    DevBuf<double2> dc;
+    DevBuf<double> db, dcoh_t, dK, dC, dcoh;
+    DevBuf<float2> dres;
+    REG_HIP(dc.alloc(P * I));
+    REG_HIP(db.alloc(hb.size()));
+    REG_HIP(dcoh_t.alloc((size_t)T * P));
+    REG_HIP(dK.alloc(P));
+    REG_HIP(dC.alloc(P));
+    REG_HIP(dcoh.alloc(P));
+    if (res_out) REG_HIP(dres.alloc(P * I));
+    REG_HIP(hipMemcpy(dc.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice));
+    REG_HIP(hipMemcpy(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
+    PyFitParams prm{P, (int)I, T, tn, row_invariant ? 1 : 0};
+    hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, dc.p, db.p,
+                       dcoh_t.p, dK.p, dC.p, dcoh.p, res_out ? dres.p : nullptr, prm);
+    REG_HIP(hipGetLastError());
+    REG_HIP(hipDeviceSynchronize());
+    REG_HIP(hipMemcpy(k_out, dK.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    REG_HIP(hipMemcpy(c_out, dC.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    REG_HIP(hipMemcpy(coh_out, dcoh.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    if (res_out) {
+        std::vector<float2> hr(P * I);
+        REG_HIP(hipMemcpy(hr.data(), dres.p, sizeof(float2) * hr.size(), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < P; ++k)
+            for (int64_t i = 0; i < I; ++i) {
+                res_out[2 * (k * I + i)] = hr[i * P + k].x;
+                res_out[2 * (k * I + i) + 1] = hr[i * P + k].y;
+            }
+    }
+    return PS2_OK;
+}
+
+extern "C" int ps2_hip_grid_accumulate(const float *ph_weight, const int64_t *grid_lin,
+                                       int64_t n_rows, int64_t n_ifg, int64_t n_i, int64_t n_j,
+                                       int32_t device, float *out) {
+    if (n_rows < 0 || n_ifg <= 0 || n_i <= 0 || n_j <= 0 || !out ||
+        (n_rows > 0 && (!ph_weight || !grid_lin))) {
+        set_error("grid_accumulate: invalid argument");
+        return PS2_E_ARG;
+    }
+    const int64_t C = n_i * n_j, P = n_rows, I = n_ifg;
+    std::vector<int64_t> start(C + 1, 0), order(P > 0 ? P : 1);
+    for (int64_t k = 0; k < P; ++k) {
+        if (grid_lin[k] < 0 || grid_lin[k] >= C) {
+            set_error("grid_accumulate: grid index " + std::to_string(grid_lin[k]) + " of row " +
+                      std::to_string(k) + " is out of range for " + std::to_string(n_i) + "x" +
+                      std::to_string(n_j) + " cells");
+            return PS2_E_ARG;
+        }
+        ++start[grid_lin[k] + 1];
+    }
+    for (int64_t c = 0; c < C; ++c) start[c + 1] += start[c];
+    {
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int64_t k = 0; k < P; ++k) order[fill[grid_lin[k]]++] = k;
+    }
+    int rc = set_device(device);
+    if (rc) return rc;
+    DevBuf<float2> dph, dout;
+    DevBuf<int64_t> dorder, dstart;
+    REG_HIP(dph.alloc(P * I));
+    REG_HIP(dout.alloc(C * I));
+    REG_HIP(dorder.alloc(P));
+    REG_HIP(dstart.alloc(C + 1));
+    if (P > 0) {
+        REG_HIP(hipMemcpy(dph.p, ph_weight, sizeof(float2) * P * I, hipMemcpyHostToDevice));
+        REG_HIP(hipMemcpy(dorder.p, order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice));
+    }
+    REG_HIP(hipMemcpy(dstart.p, start.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_py_grid_accumulate, dim3((unsigned)((C * I + 255) / 256)), dim3(256), 0, 0,
+                       dph.p, dorder.p, dstart.p, dout.p, C, (int)I);
+    REG_HIP(hipGetLastError());
+    REG_HIP(hipDeviceSynchronize());
+    REG_HIP(hipMemcpy(out, dout.p, sizeof(float2) * C * I, hipMemcpyDeviceToHost));
+    return PS2_OK;
+}
+
+extern "C" int ps2_hip_histogram_with_centers(const double *values, int64_t n,
+                                              const double *centers, int64_t n_centers,
+                                              double *out) {
+    if (n < 0 || n_centers < 0 || (n > 0 && !values) || (n_centers > 0 && (!centers || !out))) {
+        set_error("histogram_with_centers: invalid argument");
+        return PS2_E_ARG;
+    }
+    if (n_centers == 0) return PS2_OK;
+    for (int64_t b = 0; b < n_centers; ++b) out[b] = 0.0;
+    if (n_centers == 1) {
+        for (int64_t q = 0; q < n; ++q)
+            if (std::isfinite(values[q])) out[0] += 1.0;
+        return PS2_OK;
+    }
+    const HistRule hr = make_hist_rule(centers, n_centers);
+    for (int64_t q = 0; q < n; ++q) {
+        const double v = values[q];
+        if (!std::isfinite(v)) continue;
+        int64_t b;
+        if (hr.equal) {
+            double a = std::ceil((v - hr.cutoff0) / hr.d);
+            if (!(a > 0.0)) a = 0.0;
+            if (a > (double)(n_centers - 1)) a = (double)(n_centers - 1);
+            b = (int64_t)a;
+        } else {
+            b = std::lower_bound(hr.mids.begin(), hr.mids.end(), v) - hr.mids.begin();
+            if (b > n_centers - 1) b = n_centers - 1;
+        }
+        out[b] += 1.0;
+    }
+    return PS2_OK;
+}

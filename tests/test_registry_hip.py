@@ -1,0 +1,176 @@
+"""The `hip` provider of the Python pipeline's kernel registry.
+
+CPU tests pin the oracle's restatement of the Python-path topofit
+(pystamps/pipeline/ported.py:2355-2513): the near-max candidate and selection
+rules on hand-made coherence vectors, the noise-free on-node fit, the subset
+fit of rows with zero elements; the grid accumulation against numpy's
+``np.add.at`` (the routine the reference calls); the library's histogram
+(host code) against the oracle's golden-pinned rule; and the registry glue.
+GPU tests compare the provider's kernels with the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+# --- rules of the Python-path topofit ------------------------------------------
+
+def test_near_max_candidate_rule(ora):
+    f = ora.py_near_max_candidates
+    # interior local maxima within 2e-4 of the max; 0.4999 is not a local max
+    assert list(f([0.1, 0.5, 0.4999, 0.2, 0.49995, 0.3])) == [1, 4]
+    # ends compare with their one neighbour (>=)
+    assert list(f([0.5, 0.3, 0.49985])) == [0, 2]
+    # a plateau: both cells are local maxima (>=)
+    assert list(f([0.3, 0.5, 0.5, 0.3])) == [1, 2]
+    # outside the tolerance
+    assert list(f([0.5, 0.2, 0.4997])) == [0]
+    assert list(f([0.7])) == [0]
+
+
+def test_select_candidate_rule(ora):
+    g = ora.py_select_candidate
+    # one candidate: itself
+    assert g([3], [0.4], [0.9], 7) == 3
+    # the two endpoints: the coarse argmax even when refinement prefers the other
+    assert g([0, 6], [0.5, 0.49995], [0.7, 0.8], 7) == 0
+    # otherwise the refined argmax, first on ties
+    assert g([1, 4], [0.5, 0.49995], [0.7, 0.8], 7) == 4
+    assert g([1, 4, 5], [0.5, 0.49995, 0.4999], [0.8, 0.8, 0.1], 7) == 1
+    # endpoints but three candidates: refined
+    assert g([0, 3, 6], [0.5, 0.4, 0.49995], [0.1, 0.2, 0.3], 7) == 6
+
+
+def _on_node_rows(n_rows=40, n_ifg=25, wraps=0.7, seed=1):
+    rng = np.random.default_rng(seed)
+    tn = math.ceil(8 * wraps)
+    bp = rng.uniform(-200, 200, (n_rows, n_ifg))
+    t = rng.integers(-tn, tn + 1, n_rows)
+    c0 = rng.uniform(-np.pi, np.pi, n_rows)
+    rng_b = bp.max(1) - bp.min(1)
+    k = (np.pi / 4) / rng_b * t
+    cpx = np.exp(1j * (k[:, None] * bp + c0[:, None]))
+    return cpx, bp, k, c0, wraps
+
+
+def test_py_topofit_noise_free_on_node(ora):
+    cpx, bp, k, c0, wraps = _on_node_rows()
+    K, Cc, coh, res = ora.py_topofit_rows(cpx, bp, wraps)
+    np.testing.assert_allclose(K, k, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(coh, 1.0, rtol=0, atol=1e-12)
+    d = np.angle(np.exp(1j * (Cc - c0)))
+    np.testing.assert_allclose(d, 0.0, atol=1e-9)
+    np.testing.assert_allclose(np.abs(res), 1.0, atol=1e-12)
+
+
+def test_py_topofit_zero_elements_fit_the_subset(ora):
+    rng = np.random.default_rng(2)
+    cpx = np.exp(1j * rng.uniform(-np.pi, np.pi, (30, 20)))
+    bp = rng.uniform(-150, 150, (30, 20))
+    holes = cpx.copy()
+    holes[:, [3, 11, 17]] = 0.0
+    K, Cc, coh, res = ora.py_topofit_rows(holes, bp, 0.6)
+    keep = [j for j in range(20) if j not in (3, 11, 17)]
+    K2, C2, coh2, res2 = ora.py_topofit_rows(cpx[:, keep], bp[:, keep], 0.6)
+    np.testing.assert_array_equal(K, K2)
+    np.testing.assert_array_equal(coh, coh2)
+    assert not np.any(res[:, [3, 11, 17]])
+    np.testing.assert_array_equal(res[:, keep], res2)
+    Kz, Cz, cz, rz = ora.py_topofit_rows(np.zeros((2, 5)), np.ones((2, 5)), 0.5)
+    assert np.all(np.isnan(Kz)) and np.all(np.isnan(cz)) and not np.any(rz)
+
+
+def test_py_grid_accumulate_matches_numpy_add_at(ora):
+    rng = np.random.default_rng(3)
+    P, I, ni, nj = 700, 9, 6, 7
+    ph = (rng.standard_normal((P, I)) + 1j * rng.standard_normal((P, I))).astype(np.complex64)
+    gl = rng.integers(0, ni * nj, P)
+    ref = np.zeros((ni, nj, I), np.complex64)
+    for i in range(I):
+        flat = ref[:, :, i].reshape(-1)
+        np.add.at(flat, gl, ph[:, i])
+    out = ora.py_grid_accumulate(ph, gl, ni, nj)
+    assert np.array_equal(out, ref)
+
+
+# --- library host code and registry glue (no GPU) ---------------------------------------
+
+def test_library_histogram_matches_oracle(ora):
+    from pystamps_amd import registry_hip
+    from pystamps_amd.prepare import coh_bins
+    rng = np.random.default_rng(4)
+    v = np.concatenate([rng.uniform(-0.2, 1.2, 5000), [np.nan, np.inf, -np.inf, 0.005, 0.995]])
+    for centers in (coh_bins(), np.array([0.0, 0.1, 0.35, 0.4, 1.0]), np.array([0.5])):
+        got = registry_hip.stage2_histogram(v, centers)
+        ref = ora.hist_with_centers(v[np.isfinite(v)], centers)
+        assert np.array_equal(got, ref), centers
+    assert registry_hip.stage2_histogram(v, np.array([])).size == 0
+
+
+def test_register_adds_provider_and_kernels():
+    from pystamps_amd import registry_hip
+
+    class FakeRegistry:
+        def __init__(self):
+            self.providers, self.kernels = {}, {}
+
+        def register_provider(self, backend_id, *, description, aliases=(), availability_probe=None,
+                              unavailable_reason=""):
+            self.providers[backend_id] = (description, availability_probe, unavailable_reason)
+
+        def register(self, name, cpu=None, gpu=None, native=None, **impls):
+            self.kernels.setdefault(name, {}).update(impls)
+
+    reg = registry_hip.register(FakeRegistry())
+    assert "hip" in reg.providers and callable(reg.providers["hip"][1])
+    assert set(reg.kernels) == set(registry_hip.KERNELS)
+    for name in registry_hip.KERNELS:
+        assert reg.kernels[name]["hip"] is registry_hip.IMPLEMENTATIONS[name]
+
+
+# --- GPU -------------------------------------------------------------------------
+
+@pytest.mark.gpu
+def test_gpu_topofit_matches_oracle(ora):
+    from pystamps_amd import registry_hip
+    rng = np.random.default_rng(5)
+    P, I = 3000, 33
+    bp = rng.uniform(-250, 250, (P, I))
+    k = rng.uniform(-0.01, 0.01, P)
+    noise = rng.uniform(0.0, 1.5, P)
+    cpx = np.exp(1j * (k[:, None] * bp + noise[:, None] * rng.standard_normal((P, I))))
+    cpx[rng.random((P, I)) < 0.03] = 0.0                     # missing elements
+    cpx[7] = 0.0                                             # an all-zero row
+    # two near-equal peaks: +t and -t mixtures on a symmetric baseline
+    sym = np.linspace(-200, 200, I)
+    bp[11] = sym
+    kk = (np.pi / 4) / 400 * 3
+    cpx[11] = np.exp(1j * kk * sym) + 0.9998 * np.exp(-1j * kk * sym)
+    K, Cc, coh, res = registry_hip.stage2_topofit(cpx, bp, 0.7)
+    Ko, Co, co, ro = ora.py_topofit_rows(cpx, bp, 0.7)
+    np.testing.assert_allclose(K, Ko, rtol=0, atol=1e-9, equal_nan=True)
+    np.testing.assert_allclose(coh, co, rtol=0, atol=1e-9, equal_nan=True)
+    dC = np.angle(np.exp(1j * (Cc - Co)))
+    assert np.nanmax(np.abs(dC)) < 1e-8
+    np.testing.assert_allclose(res, ro.astype(np.complex64), rtol=0, atol=2e-6)
+    assert np.isnan(K[7]) and not np.any(res[7])
+    # row invariant, and the coherence-only kernel
+    Kr, Cr, cr, rr = registry_hip.stage2_topofit_row_invariant(cpx, np.tile(sym, (P, 1)), 0.7)
+    Ko2, Co2, co2, ro2 = ora.py_topofit_rows(cpx, sym, 0.7)
+    np.testing.assert_allclose(Kr, Ko2, rtol=0, atol=1e-9, equal_nan=True)
+    c_only = registry_hip.stage2_topofit_coh_row_invariant(cpx, sym, 0.7)
+    np.testing.assert_array_equal(c_only, cr)
+
+
+@pytest.mark.gpu
+def test_gpu_grid_accumulate_matches_add_at(ora):
+    from pystamps_amd import registry_hip
+    rng = np.random.default_rng(6)
+    P, I, ni, nj = 20000, 17, 40, 55
+    ph = (rng.standard_normal((P, I)) + 1j * rng.standard_normal((P, I))).astype(np.complex64)
+    gl = rng.integers(0, ni * nj, P)
+    out = registry_hip.stage2_grid_accumulate(ph, gl, ni, nj)
+    assert np.array_equal(out, ora.py_grid_accumulate(ph, gl, ni, nj))
+    with pytest.raises(registry_hip.BackendError, match="out of range"):
+        registry_hip.stage2_grid_accumulate(ph[:3], np.array([0, 1, ni * nj]), ni, nj)
