@@ -50,6 +50,10 @@ struct RT<float> {
     // CLAP gain normalisation h / median: one reciprocal per window
     __device__ static inline float gain_scale(float med) { return med != 0.0f ? 1.0f / med : 1.0f; }
     __device__ static inline float gain_div(float h, float, float scl) { return h * scl; }
+    // h / median - 1 in one rounding
+    __device__ static inline float gain_rel_minus_one(float h, float, float scl) {
+        return fmaf(h, scl, -1.0f);
+    }
     __device__ static inline key_t key(float v) { return __float_as_uint(v); }
     __device__ static inline float from_key(key_t k) { return __uint_as_float(k); }
     __device__ static inline bool isnan_(float v) { return __builtin_isnan(v); }
@@ -69,6 +73,9 @@ struct RT<double> {
     __device__ static inline double gain_scale(double) { return 1.0; }
     __device__ static inline double gain_div(double h, double med, double) {
         return med != 0.0 ? h / med : h;
+    }
+    __device__ static inline double gain_rel_minus_one(double h, double med, double s) {
+        return gain_div(h, med, s) - 1.0;
     }
     __device__ static inline key_t key(double v) { return (uint64_t)__double_as_longlong(v); }
     __device__ static inline double from_key(key_t k) { return __longlong_as_double((long long)k); }
@@ -112,7 +119,10 @@ __device__ constexpr int bitrev5(int v) {
 // 32-point radix-2 decimation-in-time FFT held in registers, unnormalised:
 // forward X[k] = sum x[n] e^{-2 pi i nk/32}, inverse with e^{+...} (the
 // rustfft conventions of native_stage2.rs:1055-1095).
-template <typename R, bool INV>
+// NZ: inputs n >= NZ are zero (the zero-padded window: their first-stage
+// butterflies reduce to copies).  NOUT: outputs k >= NOUT are not needed
+// (their last-stage halves are left uncomputed; xr/xi[k] are unspecified).
+template <typename R, bool INV, int NZ = 32, int NOUT = 32>
 __device__ inline void fft32(R (&xr)[32], R (&xi)[32]) {
     R ar[32], ai[32];
 #pragma unroll
@@ -149,8 +159,15 @@ __device__ inline void fft32(R (&xr)[32], R (&xi)[32]) {
                     vi = fma(br, wi, bi * wr);
                 }
                 R ur = ar[b + j], ui = ai[b + j];
+                if (s == 1 && bitrev5(b + j + half) >= NZ) {
+                    // the pair's second input is a padding zero
+                    ar[b + j + half] = ur;
+                    ai[b + j + half] = ui;
+                    continue;
+                }
                 ar[b + j] = ur + vr;
                 ai[b + j] = ui + vi;
+                if (s == 5 && b + j + half >= NOUT) continue;
                 ar[b + j + half] = ur - vr;
                 ai[b + j + half] = ui - vi;
             }

@@ -459,7 +459,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
             }
 #pragma unroll
             for (int c = CLAP_WIN; c < 32; ++c) xr[c] = xi[c] = (R)0;
-            fft32<R, false>(xr, xi);
+            fft32<R, false, CLAP_WIN>(xr, xi);
         } else {
 #pragma unroll
             for (int c = 0; c < 32; ++c) xr[c] = xi[c] = (R)0;
@@ -471,12 +471,15 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
             tbi[lane][c] = xi[c];
         }
         wave_sync();
+        // rows 24..31 of the row transforms are zero (padding rows)
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
+        for (int r = 0; r < CLAP_WIN; ++r) {
             xr[r] = tbr[r][lane];
             xi[r] = tbi[r][lane];
         }
-        fft32<R, false>(xr, xi);  // lane c now holds column c of FFT2
+#pragma unroll
+        for (int r = CLAP_WIN; r < 32; ++r) xr[r] = xi[r] = (R)0;
+        fft32<R, false, CLAP_WIN>(xr, xi);  // lane c now holds column c of FFT2
         wave_sync();
         // |F| at fftshifted position (re plane)
 #pragma unroll
@@ -490,19 +493,17 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
             for (int c = 0; c < 32; ++c) tbi[lane][c] = smooth7<R>(h, c, prm.k7);
         }
         wave_sync();
-        {   // columns (lane = shifted column) -> re plane
+        // columns: lane c takes its shifted column cS and smooths it in
+        // registers; ifftshift back, H[r][c] = Hf[(r+16)%32][cS], kept as
+        // median keys (exact: H >= 0)
+        typename RT<R>::key_t hk[32];
+        {
             R t[32];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) t[r] = tbi[r][lane];
+            for (int r = 0; r < 32; ++r) t[r] = tbi[r][cS];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) tbr[r][lane] = smooth7<R>(t, r, prm.k7);
+            for (int r = 0; r < 32; ++r) hk[r] = median_key<R>(smooth7<R>(t, (r + 16) & 31, prm.k7));
         }
-        wave_sync();
-        // ifftshift back: lane c takes H[r][c] = Hf[(r+16)%32][(c+16)%32],
-        // kept as median keys (exact: H >= 0)
-        typename RT<R>::key_t hk[32];
-#pragma unroll
-        for (int r = 0; r < 32; ++r) hk[r] = median_key<R>(tbr[(r + 16) & 31][cS]);
         wave_sync();  // the tile is scratch for the median from here to the gain
 #if PS2_CLAP_MEDIAN_HIST
         const R med = team_median<R>(hk, hi_team, &tbr[0][0], &tbi[0][0]);
@@ -512,17 +513,18 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         const R scl = RT<R>::gain_scale(med);
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            R v = RT<R>::gain_div(RT<R>::from_key(hk[r]), med, scl);
-            v = ALPHA_ONE ? v - (R)1 : RT<R>::pow_(v, prm.alpha) - (R)1;
+            R v = ALPHA_ONE ? RT<R>::gain_rel_minus_one(RT<R>::from_key(hk[r]), med, scl)
+                            : RT<R>::pow_(RT<R>::gain_div(RT<R>::from_key(hk[r]), med, scl), prm.alpha) - (R)1;
             if (v < (R)0) v = (R)0;
-            const R gsc = v * prm.beta + lpw[r * 32 + lane];
+            const R gsc = fma(v, prm.beta, lpw[r * 32 + lane]);
             xr[r] *= gsc;
             xi[r] *= gsc;
         }
-        fft32<R, true>(xr, xi);  // inverse along columns
+        // inverse along columns; only the window's 24 output rows go on
+        fft32<R, true, 32, CLAP_WIN>(xr, xi);
         wave_sync();
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
+        for (int r = 0; r < CLAP_WIN; ++r) {
             tbr[r][lane] = xr[r];
             tbi[r][lane] = xi[r];
         }
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
             xr[c] = tbr[lane][c];
             xi[c] = tbi[lane][c];
         }
-        fft32<R, true>(xr, xi);  // inverse along rows: spatial row `lane`
+        fft32<R, true, 32, CLAP_WIN>(xr, xi);  // inverse along rows: spatial row `lane`
         const bool last = jx == jx1 - 1;
         if (cs == 0) {
             if (out_row) {
