@@ -670,6 +670,45 @@ struct FitParams {
     float rescore_bound;  // FP32 form only
 };
 
+// Session upload: the caller's row-major [P][I] arrays, copied as they are,
+// become ifg-major planes [I][P] in cell-sorted pixel order.  Thread per
+// (sorted pixel k, ifg f), f fastest, so each source row is read contiguously.
+template <typename R>
+__global__ __launch_bounds__(256) void k_planes_cplx(const float2 *__restrict__ src,
+                                                     const int64_t *__restrict__ order,
+                                                     cplx<R> *__restrict__ dst, int64_t P, int I) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= P * I) return;
+    const int64_t k = gid / I;
+    const int f = (int)(gid - k * I);
+    const float2 v = src[order[k] * I + f];
+    dst[(int64_t)f * P + k] = {(R)v.x, (R)v.y};
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_planes_real(const double *__restrict__ src,
+                                                     const int64_t *__restrict__ order,
+                                                     R *__restrict__ dst, double *__restrict__ dst64,
+                                                     int64_t P, int I) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= P * I) return;
+    const int64_t k = gid / I;
+    const int f = (int)(gid - k * I);
+    const double v = src[order[k] * I + f];
+    dst[(int64_t)f * P + k] = (R)v;
+    if (dst64) dst64[(int64_t)f * P + k] = v;
+}
+
+__global__ __launch_bounds__(256) void k_planes_f32(const float *__restrict__ src,
+                                                    const int64_t *__restrict__ order,
+                                                    float *__restrict__ dst, int64_t P, int I) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= P * I) return;
+    const int64_t k = gid / I;
+    const int f = (int)(gid - k * I);
+    dst[(int64_t)f * P + k] = src[order[k] * I + f];
+}
+
 // Loop-invariant inputs of the trial scan (native_stage2.rs:1215-1225),
 // computed once per session: the pixel's bperp range max - min (1 when zero)
 // and, for the FP32 form, the unit trial step e^{-i phi} with

@@ -214,47 +214,47 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     PS2_HIP(hipMemcpy(s->cell_of.p, cell_of.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
     PS2_HIP(hipMemcpy(s->d_order.p, s->order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice));
 
-    {   // ph_nm planes [I][P] in sorted pixel order
-        std::vector<cplx<R>> h((size_t)(P * I));
-        for (int64_t k = 0; k < P; ++k) {
-            const float *src = in->ph_nm + 2 * s->order[k] * I;
-            for (int64_t f = 0; f < I; ++f) h[f * P + k] = {(R)src[2 * f], (R)src[2 * f + 1]};
-        }
+    // the row-major inputs go up as they are; the device permutes them into
+    // sorted ifg-major planes (one staging buffer, reused)
+    DBuf stage;
+    const unsigned nb = blocks_for(P * I, 256);
+    if ((rc = stage.alloc(sizeof(double) * P * I))) return rc;
+    {   // ph_nm planes [I][P]
         if ((rc = s->phnm.alloc(sizeof(cplx<R>) * P * I))) return rc;
-        PS2_HIP(hipMemcpy(s->phnm.p, h.data(), sizeof(cplx<R>) * P * I, hipMemcpyHostToDevice));
+        PS2_HIP(hipMemcpy(stage.p, in->ph_nm, sizeof(float) * 2 * P * I, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_planes_cplx<R>, dim3(nb), dim3(256), 0, s->stream, stage.as<float2>(),
+                           s->d_order.as<int64_t>(), s->phnm.as<cplx<R>>(), P, (int)I);
+        PS2_LAUNCHED();
     }
     {   // baselines: R copy for the bulk, f64 copy for the FP64 re-score
         const int64_t n = s->rowinv ? I : P * I;
-        std::vector<R> h((size_t)n);
-        std::vector<double> h64((size_t)n);
-        if (s->rowinv) {
-            for (int64_t f = 0; f < I; ++f) {
-                h64[f] = in->bperp_row[f];
-                h[f] = (R)in->bperp_row[f];
-            }
-        } else {
-            for (int64_t k = 0; k < P; ++k) {
-                const double *src = in->bperp_mat + s->order[k] * I;
-                for (int64_t f = 0; f < I; ++f) {
-                    h64[f * P + k] = src[f];
-                    h[f * P + k] = (R)src[f];
-                }
-            }
-        }
         if ((rc = s->bp.alloc(sizeof(R) * n))) return rc;
-        PS2_HIP(hipMemcpy(s->bp.p, h.data(), sizeof(R) * n, hipMemcpyHostToDevice));
-        if (sizeof(R) == 4) {
+        if (sizeof(R) == 4)
             if ((rc = s->bp64.alloc(sizeof(double) * n))) return rc;
-            PS2_HIP(hipMemcpy(s->bp64.p, h64.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        if (s->rowinv) {
+            std::vector<R> h((size_t)n);
+            for (int64_t f = 0; f < I; ++f) h[f] = (R)in->bperp_row[f];
+            PS2_HIP(hipMemcpy(s->bp.p, h.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+            if (sizeof(R) == 4)
+                PS2_HIP(hipMemcpy(s->bp64.p, in->bperp_row, sizeof(double) * n, hipMemcpyHostToDevice));
+        } else {
+            PS2_HIP(hipStreamSynchronize(s->stream));  // the staging buffer is free again
+            PS2_HIP(hipMemcpy(stage.p, in->bperp_mat, sizeof(double) * P * I, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_planes_real<R>, dim3(nb), dim3(256), 0, s->stream, stage.as<double>(),
+                               s->d_order.as<int64_t>(), s->bp.as<R>(),
+                               sizeof(R) == 4 ? s->bp64.as<double>() : nullptr, P, (int)I);
+            PS2_LAUNCHED();
         }
     }
     if (in->amp && s->weighting == PS2_WEIGHT_SNR) {
-        std::vector<float> h((size_t)(P * I));
-        for (int64_t k = 0; k < P; ++k)
-            for (int64_t f = 0; f < I; ++f) h[f * P + k] = in->amp[s->order[k] * I + f];
         if ((rc = s->amp.alloc(sizeof(float) * P * I))) return rc;
-        PS2_HIP(hipMemcpy(s->amp.p, h.data(), sizeof(float) * P * I, hipMemcpyHostToDevice));
+        PS2_HIP(hipStreamSynchronize(s->stream));
+        PS2_HIP(hipMemcpy(stage.p, in->amp, sizeof(float) * P * I, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_planes_f32, dim3(nb), dim3(256), 0, s->stream, stage.as<float>(),
+                           s->d_order.as<int64_t>(), s->amp.as<float>(), P, (int)I);
+        PS2_LAUNCHED();
     }
+    PS2_HIP(hipStreamSynchronize(s->stream));  // before the staging buffer is freed
     {   // initial weighting 1/D_A (0 where D_A == 0), native_stage2.rs:242-246
         std::vector<double> w0(P);
         for (int64_t k = 0; k < P; ++k) {
