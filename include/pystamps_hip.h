@@ -200,6 +200,23 @@ void ps2_hip_session_destroy(ps2_session *s);
 int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loops,
                          double *seconds);
 
+/* Stage-2 results left on the device for a GPU consumer (the stage-3 inputs
+ * native_stage3.rs:40-105 reads from pm1.mat: K_ps, C_ps, coh_ps, N_opt,
+ * ph_patch, ph_res, Nr), in the caller's row order.  The device arrays belong
+ * to the session: they stay valid until the session's next run, export or
+ * destroy.  The call returns after they are complete. */
+typedef struct {
+    int32_t device;
+    int64_t n_ps, n_ifg;
+    const double *k_ps, *c_ps, *coh_ps, *n_opt; /* device, [n_ps] */
+    const float *ph_patch;                      /* device, [n_ps][n_ifg] complex64 */
+    const float *ph_res;                        /* device, [n_ps][n_ifg] */
+    double nr[PS2_COH_BINS];                    /* host copy: last scaled Nr */
+    double gamma_change_save;
+    int64_t i_loop;
+} ps2_device_outputs;
+int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
+
 /* Per-kernel device time of the session's runs and single steps so far (the reference prints
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,

@@ -53,6 +53,15 @@ class PatchOut(C.Structure):
     ]
 
 
+class DeviceOutputs(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("n_ps", C.c_int64), ("n_ifg", C.c_int64),
+        ("k_ps", C.c_void_p), ("c_ps", C.c_void_p), ("coh_ps", C.c_void_p), ("n_opt", C.c_void_p),
+        ("ph_patch", C.c_void_p), ("ph_res", C.c_void_p), ("nr", C.c_double * COH_BINS),
+        ("gamma_change_save", C.c_double), ("i_loop", C.c_int64),
+    ]
+
+
 class RunOpts(C.Structure):
     _fields_ = [
         ("form", C.c_int32), ("device", C.c_int32), ("rescore_bound", C.c_double),
@@ -94,6 +103,7 @@ def lib() -> C.CDLL:
     L.ps2_hip_grid_accumulate.argtypes = [C.POINTER(C.c_float), _i64p, C.c_int64, C.c_int64,
                                           C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_float)]
     L.ps2_hip_histogram_with_centers.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p]
+    L.ps2_hip_session_export_device.argtypes = [C.c_void_p, C.POINTER(DeviceOutputs)]
     L.ps2_hip_session_create.argtypes = [C.POINTER(PatchIn), C.POINTER(RunOpts), C.POINTER(C.c_void_p)]
     L.ps2_hip_session_run.argtypes = [C.c_void_p, _i64p, _f64p]
     L.ps2_hip_session_download.argtypes = [C.c_void_p, C.POINTER(PatchOut)]

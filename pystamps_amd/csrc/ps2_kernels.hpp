@@ -709,6 +709,14 @@ __global__ __launch_bounds__(256) void k_planes_f32(const float *__restrict__ sr
     dst[(int64_t)f * P + k] = src[order[k] * I + f];
 }
 
+// sorted pixel k -> the caller's row order[k]
+__global__ __launch_bounds__(256) void k_unpermute_f64(const double *__restrict__ src,
+                                                       const int64_t *__restrict__ order,
+                                                       double *__restrict__ dst, int64_t P) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < P) dst[order[k]] = src[k];
+}
+
 // Loop-invariant inputs of the trial scan (native_stage2.rs:1215-1225),
 // computed once per session: the pixel's bperp range max - min (1 when zero)
 // and, for the FP32 form, the unit trial step e^{-i phi} with

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <utility>
 #include <string>
 #include <thread>
 #include <vector>
@@ -124,6 +125,7 @@ struct ps2_session {
     HistRule hist_rule;
     // device state
     DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, wt, offset;
+    DBuf ex_k, ex_c, ex_coh, ex_nopt, ex_patch, ex_res;  // device export (row order)
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
@@ -796,6 +798,57 @@ int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out) {
 }
 
 void ps2_hip_session_destroy(ps2_session *s) { delete s; }
+
+int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out) {
+    if (!s || !out) return arg_error("session or output is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->ran) {
+        set_error("session_export_device before session_run");
+        return PS2_E_STATE;
+    }
+    PS2_HIP(hipSetDevice(s->device));
+    const int64_t P = s->P, I = s->I;
+    int rc;
+    for (DBuf *b : {&s->ex_k, &s->ex_c, &s->ex_coh, &s->ex_nopt})
+        if (!b->p && (rc = b->alloc(sizeof(double) * P))) return rc;
+    if (!s->ex_patch.p && (rc = s->ex_patch.alloc(sizeof(float2) * P * I))) return rc;
+    if (!s->ex_res.p && (rc = s->ex_res.alloc(sizeof(float) * P * I))) return rc;
+    const unsigned nb = blocks_for(P, 256);
+    const std::pair<DBuf *, DBuf *> vecs[] = {
+        {&s->K, &s->ex_k}, {&s->Cc, &s->ex_c}, {&s->coh, &s->ex_coh}, {&s->nopt, &s->ex_nopt}};
+    for (const auto &pr : vecs) {
+        hipLaunchKernelGGL(k_unpermute_f64, dim3(nb), dim3(256), 0, s->stream, pr.first->as<double>(),
+                           s->d_order.as<int64_t>(), pr.second->as<double>(), P);
+        PS2_LAUNCHED();
+    }
+    if (s->form == PS2_FORM_FP64)
+        hipLaunchKernelGGL(k_final_patch_res<double>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+                           s->phnm.as<cplx<double>>(), s->bp.as<double>(), s->filt.as<cplx<double>>(),
+                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
+                           s->d_order.as<int64_t>(), s->ex_patch.as<float2>(), s->ex_res.as<float>(), P,
+                           (int)I, s->rowinv, s->C);
+    else
+        hipLaunchKernelGGL(k_final_patch_res<float>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+                           s->phnm.as<cplx<float>>(), s->bp.as<float>(), s->filt.as<cplx<float>>(),
+                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
+                           s->d_order.as<int64_t>(), s->ex_patch.as<float2>(), s->ex_res.as<float>(), P,
+                           (int)I, s->rowinv, s->C);
+    PS2_LAUNCHED();
+    PS2_HIP(hipStreamSynchronize(s->stream));
+    out->device = s->device;
+    out->n_ps = P;
+    out->n_ifg = I;
+    out->k_ps = s->ex_k.as<double>();
+    out->c_ps = s->ex_c.as<double>();
+    out->coh_ps = s->ex_coh.as<double>();
+    out->n_opt = s->ex_nopt.as<double>();
+    out->ph_patch = s->ex_patch.as<float>();
+    out->ph_res = s->ex_res.as<float>();
+    for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
+    out->gamma_change_save = s->gamma_change_save;
+    out->i_loop = s->i_loop;
+    return PS2_OK;
+}
 
 int ps2_hip_session_set_timing(ps2_session *s, int32_t enable) {
     if (!s) return arg_error("session is NULL");

@@ -221,6 +221,16 @@ class Stage2Session:
         _check(self.L.ps2_hip_session_download(self.h, C.byref(out.s)))
         return out.finish(self.prep)
 
+    def export_device(self) -> dict:
+        """Stage-3 inputs left on the device in row order (device pointers as
+        ints, valid until the session's next run, export or close)."""
+        d = _native.DeviceOutputs()
+        _check(self.L.ps2_hip_session_export_device(self.h, C.byref(d)))
+        return {"device": int(d.device), "n_ps": int(d.n_ps), "n_ifg": int(d.n_ifg),
+                "K_ps": d.k_ps, "C_ps": d.c_ps, "coh_ps": d.coh_ps, "N_opt": d.n_opt,
+                "ph_patch": d.ph_patch, "ph_res": d.ph_res, "Nr": np.array(d.nr[:]),
+                "gamma_change_save": float(d.gamma_change_save), "i_loop": int(d.i_loop)}
+
     def fill_grid(self, k_ps, weighting) -> np.ndarray:
         k = np.ascontiguousarray(k_ps, dtype=np.float64)
         w = np.ascontiguousarray(weighting, dtype=np.float64)

@@ -369,3 +369,30 @@ def test_small_baseline_loop_fp64_vs_ref64(S, ora):
     assert g["i_loop"] == r["i_loop"]
     assert np.array_equal(g["trial_ix"], r["trial_ix"])
     np.testing.assert_allclose(g["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
+
+
+# --- device-resident outputs --------------------------------------------------------
+
+def _hip_to_host(ptr, shape, dtype):
+    """Copies a device buffer to a new host array with the HIP runtime."""
+    import ctypes
+    rt = ctypes.CDLL("libamdhip64.so")
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(shape, dtype)
+    assert rt.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0   # hipMemcpyDeviceToHost
+    return out
+
+
+def test_export_device_matches_download(S, prep_small):
+    with S.Stage2Session(prep_small, form="fp32") as s:
+        s.run()
+        host = s.download()
+        dev = s.export_device()
+        P, I = prep_small.n_ps, prep_small.n_ifg
+        assert (dev["n_ps"], dev["n_ifg"], dev["i_loop"]) == (P, I, host["i_loop"])
+        for key in ("K_ps", "C_ps", "coh_ps", "N_opt"):
+            assert np.array_equal(_hip_to_host(dev[key], (P,), np.float64), host[key],
+                                  equal_nan=True), key
+        assert np.array_equal(_hip_to_host(dev["ph_patch"], (P, I), np.complex64), host["ph_patch"])
+        assert np.array_equal(_hip_to_host(dev["ph_res"], (P, I), np.float32), host["ph_res"])
+        assert np.array_equal(dev["Nr"], host["Nr"])
