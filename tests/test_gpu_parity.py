@@ -396,3 +396,50 @@ def test_export_device_matches_download(S, prep_small):
         assert np.array_equal(_hip_to_host(dev["ph_patch"], (P, I), np.complex64), host["ph_patch"])
         assert np.array_equal(_hip_to_host(dev["ph_res"], (P, I), np.float32), host["ph_res"])
         assert np.array_equal(dev["Nr"], host["Nr"])
+
+
+# --- inputs the reference treats specially --------------------------------------------
+
+def _edge_prep(ora, n_ps=5000, n_ifg=12, seed=0x5EED1500, nan_frac=0.0, da_zero_frac=0.0):
+    from pystamps_amd import synth
+    from pystamps_amd.prepare import prepare_stage2_inputs
+    from conftest import oracle_nr
+    cfg = dataclasses.replace(synth.scaled(synth.CONFIGS[1], n_ps, n_ifg), seed=seed)
+    raw = synth.generate(cfg)
+    rng = np.random.default_rng(seed)
+    if nan_frac:
+        m = rng.random(raw.ph.shape) < nan_frac
+        m[:, raw.master_ix - 1] = False
+        raw.ph = raw.ph.copy()
+        raw.ph[m] = np.complex64(complex(np.nan, np.nan))
+    if da_zero_frac:
+        d = np.ones(n_ps) if raw.d_a is None else np.array(raw.d_a, dtype=np.float64)
+        d[rng.random(n_ps) < da_zero_frac] = 0.0
+        raw.d_a = d
+    return prepare_stage2_inputs(raw, nr_fn=oracle_nr, nr_rows=5000)
+
+
+@pytest.mark.parametrize("case", ["nan_phases", "zero_da", "two_ifgs"])
+def test_edge_inputs_fp64_vs_ref64(S, ora, case):
+    """NaN phases (NaN grid cells, read as 0 by CLAP, native_stage2.rs:852-858),
+    D_A = 0 (initial weight 0, :242-246) and a 2-interferogram patch: the FP64
+    loop equals ref64; the FP32 loop selects ref64's trials almost everywhere."""
+    kw = {"nan_phases": dict(nan_frac=0.002), "zero_da": dict(da_zero_frac=0.05),
+          "two_ifgs": dict(n_ifg=2)}[case]
+    prep = _edge_prep(ora, **kw)
+    g = S.run_stage2_hip(prep, form="fp64")
+    r = ora.run(prep, mode=ora.REF64)
+    assert g["i_loop"] == r["i_loop"]
+    assert np.array_equal(g["trial_ix"], r["trial_ix"])
+    assert np.array_equal(g["N_opt"], r["N_opt"])
+    np.testing.assert_allclose(g["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(g["K_ps"], r["K_ps"], rtol=0, atol=1e-9, equal_nan=True)
+    gw, rw = g["ph_weight"], r["ph_weight"].astype(np.complex64)
+    assert np.array_equal(np.isnan(gw), np.isnan(rw))
+    # FP32 form: ref64's trials after one iteration; over three, P-square
+    # lookup flips may cascade (with 2 interferograms the coherence is coarse)
+    one = dataclasses.replace(prep, gamma_max_iterations=1)
+    assert np.mean(S.run_stage2_hip(one, form="fp32")["trial_ix"] ==
+                   ora.run(one, mode=ora.REF64)["trial_ix"]) >= 0.9995
+    g32 = S.run_stage2_hip(prep, form="fp32")
+    assert np.mean(g32["trial_ix"] == r["trial_ix"]) > 0.995
