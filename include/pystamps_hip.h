@@ -184,7 +184,12 @@ int ps2_hip_histogram_with_centers(const double *values, int64_t n, const double
  * A session uploads a patch's constant inputs once.  ps2_hip_session_run then
  * runs the whole loop from the initial state (K = 0, weighting = 1/D_A) with
  * every input already in HBM, and returns after the stop decision; outputs
- * stay on the device until ps2_hip_session_download. */
+ * stay on the device until ps2_hip_session_download.  The loop stays on the
+ * device: a control kernel takes the stop decision and builds the P-square
+ * table, one iteration is captured as a HIP graph on the session's first run
+ * and replayed gamma_max_iterations times (iterations after the stop return at
+ * once), and the host waits once per loop.  With timing or verbose output the
+ * same kernels are launched iteration by iteration instead. */
 typedef struct ps2_session ps2_session;
 
 int ps2_hip_session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session **out);
@@ -222,7 +227,8 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
  * rescore, refine, rms_hist, weights.  ms[k] is the summed HIP-event time of
  * slot k over launches[k] launches.  Timing events are recorded only when
- * ps2_hip_session_set_timing(s, 1) was called before the runs. */
+ * ps2_hip_session_set_timing(s, 1) was called before the runs; such runs
+ * launch the iterations one by one (no graph), with the same results. */
 #define PS2_NUM_KERNEL_SLOTS 8
 int ps2_hip_session_set_timing(ps2_session *s, int32_t enable);
 int ps2_hip_session_kernel_times(ps2_session *s, double *ms, int64_t *launches);

@@ -1,6 +1,6 @@
 // Host-side helpers of the stage-2 library: error reporting and the
-// host-sized pieces of the gamma loop (P-square weighting, histogram rule,
-// MATLAB v5 uniform generator for the random-coherence histogram).
+// host-sized pieces of the prep (histogram rule, MATLAB v5 uniform generator
+// for the random-coherence histogram).
 #pragma once
 
 #include <cmath>
@@ -35,13 +35,6 @@ inline HistRule make_hist_rule(const double *c, int64_t n) {
     for (int64_t k = 0; k + 1 < n; ++k) h.mids.push_back((c[k] + c[k + 1]) / 2.0);
     return h;
 }
-
-// ---------------------------------------------------------------------------
-// psquare_weighting (native_stage2.rs:1642-1748): returns prand_hi (the
-// 1001-point interpolated false-positive probability); the per-pixel lookup
-// w = (1 - prand_hi[round(1000 coh)])^2 runs on the device.
-std::vector<double> psquare_prand_hi(const std::vector<double> &nr, const std::vector<double> &na,
-                                     int64_t low_coh_thresh, double nr_max_nz_ix);
 
 // ---------------------------------------------------------------------------
 // MATLAB v5 uniform generator ('state' method; native_stage2.rs:1785-1873).

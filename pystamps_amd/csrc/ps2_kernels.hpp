@@ -25,6 +25,12 @@ namespace ps2 {
 
 constexpr int CLAP_FFT = 32;   // n_win_ex
 constexpr int CLAP_WIN = 24;   // n_win
+// Loop gate: the device-resident gamma loop launches every iteration's
+// kernels up front; once its control kernel has taken the stop decision the
+// flag is set and each loop kernel returns at entry.  nullptr = ungated
+// (single-step use).
+__device__ inline bool loop_stopped(const int32_t *gate) { return gate && *gate; }
+
 constexpr int CLAP_INC = 6;    // n_inc = n_win / 4
 
 // gfx950 dispatches workgroup b to XCD b % 8 (8 XCDs, each with its own L2)
@@ -54,7 +60,9 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
                                                    const double *__restrict__ W,
                                                    const int32_t *__restrict__ cell_start,
                                                    cplx<R> *__restrict__ grid, int64_t C, int I,
-                                                   int64_t P) {
+                                                   int64_t P,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= C * I) return;
     const int f = (int)(gid / C);
@@ -85,7 +93,9 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
                                                           const double *__restrict__ W,
                                                           const int32_t *__restrict__ cell_start,
                                                           cplx<R> *__restrict__ grid, int64_t C,
-                                                          int I, int64_t P, int npg) {
+                                                          int I, int64_t P, int npg,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     constexpr int FG = 256 / CB;
     __shared__ cplx<R> vals[FG][CHUNK];
     const int t = threadIdx.x;
@@ -370,7 +380,9 @@ template <typename R, int TEAMS, bool ALPHA_ONE>
 __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(const cplx<R> *__restrict__ grid,
                                                            cplx<R> *__restrict__ part,
                                                            const R *__restrict__ lp,
-                                                           ClapParams<R> prm) {
+                                                           ClapParams<R> prm,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const bool hi_team = (threadIdx.x & 63) >= 32;
     // XCD-aware order: workgroup b runs on XCD b % 8.  Strip u (interferogram
@@ -607,7 +619,9 @@ __device__ inline cplx<R> unit_phase(cplx<R> v) {
 template <typename R, bool UNIT>
 __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
                                                       cplx<R> *__restrict__ filt, int ni, int nj,
-                                                      int I, int nwi, int nseg, int wseg) {
+                                                      int I, int nwi, int nseg, int wseg,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t C = (int64_t)ni * nj;
     if (gid >= C * I) return;
@@ -777,7 +791,9 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                                               cplx<R> *__restrict__ offset,
                                               int32_t *__restrict__ rescore_list,
                                               uint32_t *__restrict__ rescore_count,
-                                              FitParams prm) {
+                                              FitParams prm,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= prm.P) return;
     const int I = prm.I;
@@ -973,7 +989,9 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
                                                 const int32_t *__restrict__ rescore_list,
                                                 const uint32_t *__restrict__ rescore_count,
                                                 int32_t *__restrict__ tix,
-                                                cplx<RF> *__restrict__ offset, FitParams prm) {
+                                                cplx<RF> *__restrict__ offset, FitParams prm,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     // lane l holds interferograms l, l + 64, ... (up to RESCORE_PER_LANE of
     // them) for every trial; larger I re-reads the rest per trial
     constexpr int RESCORE_PER_LANE = 4;
@@ -1080,7 +1098,9 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
                                                 double *__restrict__ Kout,
                                                 double *__restrict__ Cout,
                                                 double *__restrict__ cohout,
-                                                double *__restrict__ nopt, FitParams prm) {
+                                                double *__restrict__ nopt, FitParams prm,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= prm.P) return;
     const int t = tix[k];
@@ -1213,7 +1233,9 @@ __global__ __launch_bounds__(RMS_BLOCK) void k_rms_hist(const double *__restrict
                                                         double *__restrict__ part,
                                                         uint32_t *__restrict__ hist,
                                                         const double *__restrict__ mids,
-                                                        HistParams hp) {
+                                                        HistParams hp,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     __shared__ double red[RMS_BLOCK];
     __shared__ uint32_t lh[128];
     for (int q = threadIdx.x; q < 128; q += RMS_BLOCK) lh[q] = 0;
@@ -1243,8 +1265,9 @@ __global__ __launch_bounds__(RMS_BLOCK) void k_rms_hist(const double *__restrict
         if (lh[q]) atomicAdd(&hist[q], lh[q]);
 }
 
-__global__ void k_zero_u32(uint32_t *p, int n) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_zero_u32(uint32_t *p, int64_t n, const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) p[q] = 0u;
 }
 
@@ -1253,7 +1276,9 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 __global__ __launch_bounds__(256) void k_weights_psquare(const double *__restrict__ coh,
                                                          const double *__restrict__ prand_hi,
                                                          int nhi, double *__restrict__ W,
-                                                         int64_t P) {
+                                                         int64_t P,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P) return;
     const double v = coh[k] * 1000.0;
@@ -1276,7 +1301,9 @@ __global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__
                                                      const double *__restrict__ K,
                                                      const double *__restrict__ nopt,
                                                      double *__restrict__ W, int64_t P, int I,
-                                                     int rowinv, int64_t C) {
+                                                     int rowinv, int64_t C,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P) return;
     double g = 0.0, a2 = 0.0;
@@ -1299,6 +1326,180 @@ __global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__
     a2 /= dd;
     const double sigma = sqrt(0.5 * (a2 - g * g));
     W[k] = sigma != 0.0 ? g / sigma : 0.0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Device-resident loop control.  One control kernel per iteration takes the
+// stop decision (native_stage2.rs:307-314) and, when the loop goes on with
+// P-square weighting, rebuilds the lookup table prand_hi (:364-381 and
+// psquare_weighting :1642-1748) with the host restatement's operation order;
+// later kernels of the iteration and of further iterations return at entry
+// once ctl->stopped is set.
+constexpr int CTL_BINS = 100;
+constexpr int CTL_HI = 1001;      // interp([1, prand], 10) keeps 1010 - 9 samples
+constexpr int CTL_UP = 1049;      // 101 * 10 + 39-sample delay
+
+struct LoopCtl {
+    int32_t stopped;
+    int32_t i_loop;       // current iteration (1-based), final count once stopped
+    uint64_t rescored;    // FP32 form: rows re-scored in FP64, summed
+    double gcs;           // gamma_change_save
+    double nr[CTL_BINS];  // Nr, scaled each P-square update
+};
+
+struct CtlParams {
+    int64_t P;
+    int n_rms_blocks;
+    int max_iter;
+    int psquare;          // 1: P-square weights follow (else SNR)
+    int lct;              // low_coh_thresh
+    double conv;          // gamma_change_convergence
+    double nr_max_nz_ix;
+    double g7[7];         // gausswin(7, 2.5) taps and their sum
+    double gsum;
+};
+
+// MATLAB interp(x, 10) FIR: the reference's 79 audited taps (native_stage2.rs:33-113)
+__constant__ double kCtlInterpTaps[79] = {
+    -7.92925708923623813e-05, -1.73293349766172333e-04, -3.00248431516961407e-04,
+    -4.48605550383722462e-04, -5.95835452295412349e-04, -7.08759204842826507e-04,
+    -7.46231412934915406e-04, -6.64375214040318490e-04, -4.24121369705182660e-04,
+    -3.01053961794528480e-07, 6.08939001185030554e-04,  1.37399755086099230e-03,
+    2.22872001918483081e-03,  3.06999955141492101e-03,  3.76333143346069959e-03,
+    4.15488007985555935e-03,  4.08970016389962471e-03,  3.43472424182160452e-03,
+    2.10413493213892910e-03,  8.39421246326618742e-05,  -2.54786253685349538e-03,
+    -5.60918261221970971e-03, -8.81278723412589476e-03, -1.17783753398490003e-02,
+    -1.40582499688596801e-02, -1.51753768689126753e-02, -1.46708218626883501e-02,
+    -1.21559904304523667e-02, -7.36396963079587880e-03, -1.93792008491816548e-04,
+    9.25827682864269544e-03,  2.06852593184593814e-02,  3.35768644427044682e-02,
+    4.72485589714582085e-02,  6.08905923193367796e-02,  7.36328154139928304e-02,
+    8.46190871379102622e-02,  9.30836882813813049e-02,  9.84216619119140101e-02,
+    1.00245459871706619e-01,  9.84216619119134828e-02,  9.30836882813811800e-02,
+    8.46190871379124271e-02,  7.36328154139975766e-02,  6.08905923193391457e-02,
+    4.72485589714637388e-02,  3.35768644427109145e-02,  2.06852593184574732e-02,
+    9.25827682864464527e-03,  -1.93792008489998774e-04, -7.36396963079247527e-03,
+    -1.21559904304541604e-02, -1.46708218626876996e-02, -1.51753768689126579e-02,
+    -1.40582499688608267e-02, -1.17783753398508096e-02, -8.81278723413165405e-03,
+    -5.60918261221661496e-03, -2.54786253685826761e-03, 8.39421246353382272e-05,
+    2.10413493213411785e-03,  3.43472424182329154e-03,  4.08970016390024574e-03,
+    4.15488007985538154e-03,  3.76333143346146634e-03,  3.06999955141576842e-03,
+    2.22872001918783102e-03,  1.37399755086213202e-03,  6.08939001188346478e-04,
+    -3.01053965413812172e-07, -4.24121369703597340e-04, -6.64375214039929912e-04,
+    -7.46231412935480926e-04, -7.08759204842192465e-04, -5.95835452295058465e-04,
+    -4.48605550383040716e-04, -3.00248431517695195e-04, -1.73293349768269614e-04,
+    -7.92925708931989381e-05,
+};
+
+// one workgroup of CTL_THREADS threads
+constexpr int CTL_THREADS = 1024;
+__global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ ctl,
+                                                        const double *__restrict__ rms_part,
+                                                        const uint32_t *__restrict__ hist,
+                                                        const uint32_t *__restrict__ rescore_count,
+                                                        double *__restrict__ prand_hi,
+                                                        CtlParams cp) {
+    __shared__ double red[CTL_THREADS];
+    __shared__ double na[128], nr[128], prand[128], sm[128];
+    __shared__ double up[CTL_UP];
+    __shared__ int s_stop;
+    __shared__ double s_scale;
+    if (ctl->stopped) return;
+    const int t = threadIdx.x;
+    // gamma-change RMS: fixed assignment of block partials to threads, then a
+    // fixed tree (deterministic)
+    double acc = 0.0;
+    for (int q = t; q < cp.n_rms_blocks; q += CTL_THREADS) acc += rms_part[q];
+    red[t] = acc;
+    __syncthreads();
+    for (int o = CTL_THREADS / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double rms = sqrt(red[0] / (double)(cp.P > 1 ? cp.P : 1));
+        const double change = rms - ctl->gcs;
+        ctl->gcs = rms;
+        if (rescore_count) ctl->rescored += *rescore_count;
+        s_stop = (fabs(change) < cp.conv || ctl->i_loop >= cp.max_iter) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_stop) {
+        if (t == 0) ctl->stopped = 1;
+        return;
+    }
+    if (!cp.psquare) {
+        if (t == 0) ctl->i_loop += 1;
+        return;
+    }
+    // Nr scaling on the low-coherence bins (native_stage2.rs:365-374)
+    const int lct = cp.lct < 0 ? 0 : (cp.lct > CTL_BINS ? CTL_BINS : cp.lct);
+    if (t < CTL_BINS) {
+        na[t] = (double)hist[t];
+        nr[t] = ctl->nr[t];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double denom = 0.0, num = 0.0;
+        for (int b = 0; b < lct; ++b) denom += nr[b];
+        for (int b = 0; b < lct; ++b) num += na[b];
+        s_scale = denom > 0.0 ? num / denom : 1.0;
+    }
+    __syncthreads();
+    // prand = min(Nr / Na, 1), [:lct] = 1, [Nr_max_nz_ix:] = 0 (:1642-1683)
+    const int zero_from = cp.nr_max_nz_ix > 0.0 ? (int)cp.nr_max_nz_ix : 0;
+    if (t < CTL_BINS) {
+        nr[t] *= s_scale;
+        ctl->nr[t] = nr[t];
+        const double q = nr[t] / (na[t] == 0.0 ? 1.0 : na[t]);
+        double pr = (__builtin_isnan(q) || q > 1.0) ? 1.0 : q;
+        if (t < lct) pr = 1.0;
+        if (t >= zero_from) pr = 0.0;
+        prand[t] = pr;
+    }
+    __syncthreads();
+    // gausswin(7) FIR over [1 x 7, prand], normalised (:1685-1713)
+    if (t < CTL_BINS) {
+        const int n = t + 7;
+        double sacc = 0.0;
+        for (int k = 0; k < 7; ++k) {
+            const int m = n - k;
+            sacc += cp.g7[k] * (m < 7 ? 1.0 : prand[m - 7]);
+        }
+        sm[t] = sacc / cp.gsum;
+    }
+    __syncthreads();
+    // interp([1, prand], 10) (:1715-1748)
+    for (int q = t; q < CTL_UP; q += CTL_THREADS) {
+        double v = 0.0;
+        if (q == 0)
+            v = 1.0;
+        else if (q % 10 == 0 && q / 10 - 1 < CTL_BINS)
+            v = sm[q / 10 - 1];
+        up[q] = v;
+    }
+    __syncthreads();
+    for (int q = t; q < CTL_HI; q += CTL_THREADS) {
+        const int n = 39 + q;
+        double sacc = 0.0;
+        for (int k = 0; k < 79 && k <= n; ++k) sacc += kCtlInterpTaps[k] * up[n - k];
+        prand_hi[q] = sacc * 10.0;
+    }
+    if (t == 0) ctl->i_loop += 1;
+}
+
+// K and weighting of this iteration's fill, kept for the ph_weight output
+__global__ __launch_bounds__(256) void k_copy_state(const double *__restrict__ K,
+                                                    const double *__restrict__ W,
+                                                    double *__restrict__ Kprev,
+                                                    double *__restrict__ Wprev, int64_t P,
+                                                    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < P) {
+        Kprev[k] = K[k];
+        Wprev[k] = W[k];
+    }
 }
 
 }  // namespace ps2

@@ -129,7 +129,14 @@ struct ps2_session {
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
-    HostPinned h_rms, h_hist, h_cnt, h_prand;
+    // device-resident loop: control block, gate of the loop kernels (set
+    // while a loop is being launched), one iteration captured as a graph
+    DBuf ctl;
+    HostPinned h_ctl;
+    const int32_t *gate = nullptr;
+    CtlParams ctl_params{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     // per-kernel timing (ps2_hip_session_set_timing)
     bool timing = false;
     std::vector<hipEvent_t> tev;
@@ -148,6 +155,8 @@ struct ps2_session {
         if (stream) {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
+            if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+            if (graph) (void)hipGraphDestroy(graph);
             for (auto &e : ev)
                 if (e) (void)hipEventDestroy(e);
             for (auto &e : tev)
@@ -298,16 +307,31 @@ int alloc_state(ps2_session *s) {
     for (DBuf *b : {&s->K, &s->Kprev, &s->W, &s->Wprev, &s->Cc, &s->coh, &s->cohprev, &s->nopt})
         if ((rc = b->alloc(sizeof(double) * P))) return rc;
     if ((rc = s->prand_hi.alloc(sizeof(double) * 2048))) return rc;
+    if ((rc = s->ctl.alloc(sizeof(LoopCtl)))) return rc;
+    if ((rc = s->h_ctl.alloc(sizeof(LoopCtl)))) return rc;
     s->n_rms_blocks = (int)blocks_for(P, RMS_BLOCK * RMS_PER_THREAD);
     if ((rc = s->rms_part.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
+    {   // control-kernel constants: gausswin(7, 2.5) taps and their sum (native_stage2.rs:1685-1699)
+        CtlParams &cp = s->ctl_params;
+        cp.P = P;
+        cp.n_rms_blocks = s->n_rms_blocks;
+        cp.max_iter = (int)std::min<int64_t>(std::max<int64_t>(1, s->max_iter), 1 << 30);
+        cp.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
+        cp.lct = (int)std::min<int64_t>(std::max<int64_t>(s->lct, -1), 1 << 30);
+        cp.conv = s->conv;
+        cp.nr_max_nz_ix = s->nr_max_nz_ix;
+        const double sd = 6.0 / 5.0;
+        cp.gsum = 0.0;
+        for (int k = 0; k < 7; ++k) {
+            const double x = (double)k - 3.0;
+            cp.g7[k] = std::exp(-0.5 * (x / sd) * (x / sd));
+        }
+        for (int k = 0; k < 7; ++k) cp.gsum += cp.g7[k];
+    }
     if ((rc = s->mids.alloc(sizeof(double) * 128))) return rc;
     std::vector<double> mids(128, 0.0);
     for (size_t k = 0; k < s->hist_rule.mids.size() && k < 128; ++k) mids[k] = s->hist_rule.mids[k];
     PS2_HIP(hipMemcpy(s->mids.p, mids.data(), sizeof(double) * 128, hipMemcpyHostToDevice));
-    if ((rc = s->h_rms.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
-    if ((rc = s->h_hist.alloc(sizeof(uint32_t) * 128))) return rc;
-    if ((rc = s->h_cnt.alloc(sizeof(uint32_t) * 4))) return rc;
-    if ((rc = s->h_prand.alloc(sizeof(double) * 2048))) return rc;
     return PS2_OK;
 }
 
@@ -366,7 +390,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
                                s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
-                               (int)s->I, s->P, npg);
+                               (int)s->I, s->P, npg, s->gate);
         };
         if (mean <= 256.0)
             go(k_fill_grid_staged<R, CB, 256>);
@@ -377,7 +401,8 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
     } else {
         hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
-                           s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P);
+                           s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P,
+                           s->gate);
     }
     PS2_LAUNCHED();
     return PS2_OK;
@@ -416,8 +441,11 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
 template <typename R>
 int launch_clap(ps2_session *s, bool unit) {
     const int64_t n = s->C * s->I;
-    if (!s->windows) {
-        PS2_HIP(hipMemsetAsync(s->filt.p, 0, sizeof(cplx<R>) * n, s->stream));
+    if (!s->windows) {   // empty CLAP: zero filtered grid (a kernel: this may be graph-captured)
+        const int64_t words = (int64_t)(sizeof(cplx<R>) / 4) * n;
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(words, 256)), dim3(256), 0, s->stream,
+                           s->filt.as<uint32_t>(), words, s->gate);
+        PS2_LAUNCHED();
         return PS2_OK;
     }
     constexpr int TEAMS = 2;  // one wavefront per workgroup
@@ -427,21 +455,21 @@ int launch_clap(ps2_session *s, bool unit) {
         const dim3 g((unsigned)s->nwi * prm.upx * PS2_NUM_XCD);
         if (prm.alpha_is_one)
             hipLaunchKernelGGL((k_clap_strip<R, TEAMS, true>), g, dim3(32 * TEAMS), 0, s->stream,
-                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm);
+                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm, s->gate);
         else
             hipLaunchKernelGGL((k_clap_strip<R, TEAMS, false>), g, dim3(32 * TEAMS), 0, s->stream,
-                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm);
+                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm, s->gate);
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
     if (unit)
         hipLaunchKernelGGL((k_clap_combine<R, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                           (int)s->I, s->nwi, prm.nseg, prm.wseg);
+                           (int)s->I, s->nwi, prm.nseg, prm.wseg, s->gate);
     else
         hipLaunchKernelGGL((k_clap_combine<R, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                           (int)s->I, s->nwi, prm.nseg, prm.wseg);
+                           (int)s->I, s->nwi, prm.nseg, prm.wseg, s->gate);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -463,7 +491,8 @@ int launch_fit(ps2_session *s) {
     const FitParams fp = fit_params(s);
     const unsigned nb = blocks_for(s->P, 256);
     if (sizeof(R) == 4) {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4,
+                           s->gate);
         PS2_LAUNCHED();
     }
     {
@@ -473,13 +502,13 @@ int launch_fit(ps2_session *s) {
                            s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
                            s->rescore_list.as<int32_t>(),
-                           s->rescore_count.as<uint32_t>(), fp);
+                           s->rescore_count.as<uint32_t>(), fp, s->gate);
     } else {
         hipLaunchKernelGGL((k_scan<R, 16>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                            s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
                            s->rescore_list.as<int32_t>(),
-                           s->rescore_count.as<uint32_t>(), fp);
+                           s->rescore_count.as<uint32_t>(), fp, s->gate);
     }
     PS2_LAUNCHED();
     }
@@ -488,7 +517,7 @@ int launch_fit(ps2_session *s) {
         hipLaunchKernelGGL(k_rescore<R>, dim3(std::min<int64_t>(s->P, 16384)), dim3(64), 0, s->stream, s->phnm.as<cplx<R>>(),
                            s->bp64.as<double>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(),
-                           s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp);
+                           s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp, s->gate);
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_REFINE);
@@ -496,7 +525,7 @@ int launch_fit(ps2_session *s) {
                        s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                        s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
                        s->Cc.as<double>(),
-                       s->coh.as<double>(), s->nopt.as<double>(), fp);
+                       s->coh.as<double>(), s->nopt.as<double>(), fp, s->gate);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -508,111 +537,134 @@ int launch_rms_hist(ps2_session *s) {
     hp.cutoff0 = s->hist_rule.cutoff0;
     hp.d = s->hist_rule.d;
     hp.nbins = PS2_COH_BINS;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s->stream, s->hist.as<uint32_t>(), 128);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s->stream, s->hist.as<uint32_t>(), 128, s->gate);
     PS2_LAUNCHED();
     SlotTimer tm(s, SLOT_RMS);
     hipLaunchKernelGGL(k_rms_hist, dim3(s->n_rms_blocks), dim3(RMS_BLOCK), 0, s->stream,
                        s->coh.as<double>(), s->cohprev.as<double>(), s->rms_part.as<double>(),
-                       s->hist.as<uint32_t>(), s->mids.as<double>(), hp);
+                       s->hist.as<uint32_t>(), s->mids.as<double>(), hp, s->gate);
     PS2_LAUNCHED();
     return PS2_OK;
 }
 
+// One gamma iteration (native_stage2.rs:261-390): every launch is gated by
+// the control block, so a whole loop can be launched without the host seeing
+// the stop decision.
+template <typename R>
+int launch_iteration(ps2_session *s) {
+    const int64_t P = s->P;
+    int rc;
+    // K and weighting used by this fill (for the ph_weight output)
+    hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream, s->K.as<double>(),
+                       s->W.as<double>(), s->Kprev.as<double>(), s->Wprev.as<double>(), P, s->gate);
+    PS2_LAUNCHED();
+    if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
+    if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>()))) return rc;
+    if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
+    if ((rc = launch_clap<R>(s, true))) return rc;
+    if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
+    if ((rc = launch_fit<R>(s))) return rc;
+    if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
+    if ((rc = launch_rms_hist(s))) return rc;
+    // gamma-change RMS, stop test, Nr scaling and the P-square table
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(CTL_THREADS), 0, s->stream, s->ctl.as<LoopCtl>(),
+                       s->rms_part.as<double>(), s->hist.as<uint32_t>(),
+                       sizeof(R) == 4 ? s->rescore_count.as<uint32_t>() : nullptr,
+                       s->prand_hi.as<double>(), s->ctl_params);
+    PS2_LAUNCHED();
+    SlotTimer tm(s, SLOT_WEIGHTS);
+    if (s->weighting == PS2_WEIGHT_PSQUARE)
+        hipLaunchKernelGGL(k_weights_psquare, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
+                           s->coh.as<double>(), s->prand_hi.as<double>(), CTL_HI, s->W.as<double>(), P,
+                           s->gate);
+    else
+        hipLaunchKernelGGL(k_weights_snr<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
+                           s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
+                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
+                           s->W.as<double>(), P, (int)s->I, s->rowinv, s->C, s->gate);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+// The gamma loop from the initial state (native_stage2.rs:242-258) to the
+// stop decision.  Default: one iteration is captured as a HIP graph once per
+// session and launched gamma_max_iterations times back to back; the control
+// kernel's stop flag turns the remaining launches into early returns, and the
+// host waits once.  With per-kernel timing or verbose output the iterations
+// are launched one by one and the host reads the control block after each.
 template <typename R>
 int run_loop(ps2_session *s) {
     const int64_t P = s->P;
     int rc;
+    LoopCtl *hc = s->h_ctl.as<LoopCtl>();
     PS2_HIP(hipEventRecord(s->ev[0], s->stream));
-    // initial state (native_stage2.rs:242-258)
     PS2_HIP(hipMemsetAsync(s->K.p, 0, sizeof(double) * P, s->stream));
     PS2_HIP(hipMemsetAsync(s->cohprev.p, 0, sizeof(double) * P, s->stream));
     PS2_HIP(hipMemcpyAsync(s->W.p, s->W0.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
-    std::vector<double> nr = s->nr_base;
-    double gcs = 0.0;
-    int64_t i_loop = 1;
-    const int64_t max_iter = std::max<int64_t>(1, s->max_iter);
-    int64_t rescored = 0;
-    for (;;) {
-        const auto t_it = std::chrono::steady_clock::now();
-        // K and weighting used by this fill (for the ph_weight output)
-        PS2_HIP(hipMemcpyAsync(s->Kprev.p, s->K.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
-        PS2_HIP(hipMemcpyAsync(s->Wprev.p, s->W.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
-        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
-        if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>()))) return rc;
-        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
-        if ((rc = launch_clap<R>(s, true))) return rc;
-        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
-        if ((rc = launch_fit<R>(s))) return rc;
-        if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
-        if ((rc = launch_rms_hist(s))) return rc;
-        PS2_HIP(hipMemcpyAsync(s->h_rms.p, s->rms_part.p, sizeof(double) * s->n_rms_blocks,
-                               hipMemcpyDeviceToHost, s->stream));
-        PS2_HIP(hipMemcpyAsync(s->h_hist.p, s->hist.p, sizeof(uint32_t) * 128, hipMemcpyDeviceToHost, s->stream));
-        if (sizeof(R) == 4)
-            PS2_HIP(hipMemcpyAsync(s->h_cnt.p, s->rescore_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                   s->stream));
-        PS2_HIP(hipStreamSynchronize(s->stream));
-        collect_timers(s);
-        if (sizeof(R) == 4) rescored += s->h_cnt.as<uint32_t>()[0];
-        // gamma-change RMS and stop test (native_stage2.rs:307-314)
-        double sum = 0.0;
-        for (int b = 0; b < s->n_rms_blocks; ++b) sum += s->h_rms.as<double>()[b];
-        const double rms = std::sqrt(sum / (double)std::max<int64_t>(P, 1));
-        const double change = rms - gcs;
-        gcs = rms;
-        const bool stop = std::fabs(change) < s->conv || i_loop >= max_iter;
-        if (s->verbose) {
-            float t_fill = 0, t_clap = 0, t_fit = 0;
-            (void)hipEventElapsedTime(&t_fill, s->ev[2], s->ev[3]);
-            (void)hipEventElapsedTime(&t_clap, s->ev[3], s->ev[4]);
-            (void)hipEventElapsedTime(&t_fit, s->ev[4], s->ev[5]);
-            const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
-            std::fprintf(stderr,
-                         "stage2 hip iteration %lld: fill+grid=%.3fms clap=%.3fms topofit=%.3fms "
-                         "wall=%.3fms gamma_change=%.6e gamma_delta=%.6e stop=%s\n",
-                         (long long)i_loop, t_fill, t_clap, t_fit, wall * 1e3, gcs, change,
-                         stop ? "true" : "false");
+    std::memset(hc, 0, sizeof(LoopCtl));
+    hc->i_loop = 1;
+    for (int b = 0; b < CTL_BINS; ++b) hc->nr[b] = s->nr_base[b];
+    PS2_HIP(hipMemcpyAsync(s->ctl.p, hc, sizeof(LoopCtl), hipMemcpyHostToDevice, s->stream));
+    s->gate = &s->ctl.as<LoopCtl>()->stopped;
+    const int64_t max_iter = s->ctl_params.max_iter;
+    const bool stepwise = s->timing || s->verbose;
+    if (!stepwise) {
+        if (!s->graph_exec) {
+            PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            rc = launch_iteration<R>(s);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                s->gate = nullptr;
+                return rc;
+            }
+            if (e != hipSuccess) {
+                set_error(std::string("graph capture of the gamma iteration failed: ") + hipGetErrorString(e));
+                s->gate = nullptr;
+                return PS2_E_DEVICE;
+            }
+            s->graph = g;
+            PS2_HIP(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
         }
-        if (stop) break;
-        if (s->weighting == PS2_WEIGHT_PSQUARE) {
-            // native_stage2.rs:364-381
-            std::vector<double> na(PS2_COH_BINS);
-            for (int b = 0; b < PS2_COH_BINS; ++b) na[b] = (double)s->h_hist.as<uint32_t>()[b];
-            const int64_t lct = std::min<int64_t>(s->lct, PS2_COH_BINS);
-            double denom = 0.0, num = 0.0;
-            for (int64_t b = 0; b < lct; ++b) denom += nr[b];
-            for (int64_t b = 0; b < lct; ++b) num += na[b];
-            const double scale = denom > 0.0 ? num / denom : 1.0;
-            for (auto &v : nr) v *= scale;
-            const std::vector<double> hi = psquare_prand_hi(nr, na, s->lct, s->nr_max_nz_ix);
-            std::memcpy(s->h_prand.p, hi.data(), sizeof(double) * hi.size());
-            PS2_HIP(hipMemcpyAsync(s->prand_hi.p, s->h_prand.p, sizeof(double) * hi.size(),
-                                   hipMemcpyHostToDevice, s->stream));
-            SlotTimer tm(s, SLOT_WEIGHTS);
-            hipLaunchKernelGGL(k_weights_psquare, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
-                               s->coh.as<double>(), s->prand_hi.as<double>(), (int)hi.size(),
-                               s->W.as<double>(), P);
-            PS2_LAUNCHED();
-        } else {
-            SlotTimer tm(s, SLOT_WEIGHTS);
-            hipLaunchKernelGGL(k_weights_snr<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
-                               s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(),
-                               s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->K.as<double>(),
-                               s->nopt.as<double>(), s->W.as<double>(), P, (int)s->I, s->rowinv, s->C);
-            PS2_LAUNCHED();
+        for (int64_t it = 0; it < max_iter; ++it) PS2_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+    } else {
+        for (int64_t it = 0; it < max_iter; ++it) {
+            const auto t_it = std::chrono::steady_clock::now();
+            if ((rc = launch_iteration<R>(s))) {
+                s->gate = nullptr;
+                return rc;
+            }
+            PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
+            PS2_HIP(hipStreamSynchronize(s->stream));
+            collect_timers(s);
+            if (s->verbose) {
+                float t_fill = 0, t_clap = 0, t_fit = 0;
+                (void)hipEventElapsedTime(&t_fill, s->ev[2], s->ev[3]);
+                (void)hipEventElapsedTime(&t_clap, s->ev[3], s->ev[4]);
+                (void)hipEventElapsedTime(&t_fit, s->ev[4], s->ev[5]);
+                const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
+                std::fprintf(stderr,
+                             "stage2 hip iteration %lld: fill+grid=%.3fms clap=%.3fms topofit=%.3fms "
+                             "wall=%.3fms gamma_change=%.6e stop=%s\n",
+                             (long long)(it + 1), t_fill, t_clap, t_fit, wall * 1e3, hc->gcs,
+                             hc->stopped ? "true" : "false");
+            }
+            if (hc->stopped) break;
         }
-        ++i_loop;
     }
     PS2_HIP(hipEventRecord(s->ev[1], s->stream));
-    PS2_HIP(hipEventSynchronize(s->ev[1]));
+    PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
+    PS2_HIP(hipStreamSynchronize(s->stream));
+    s->gate = nullptr;
     collect_timers(s);
     float ms = 0.0f;
     PS2_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
     s->loop_seconds = (double)ms * 1e-3;
-    s->i_loop = i_loop;
-    s->gamma_change_save = gcs;
-    s->nr = nr;
-    s->n_rescored = rescored;
+    s->i_loop = hc->i_loop;
+    s->gamma_change_save = hc->gcs;
+    s->nr.assign(hc->nr, hc->nr + CTL_BINS);
+    s->n_rescored = (int64_t)hc->rescored;
     s->ran = true;
     return PS2_OK;
 }

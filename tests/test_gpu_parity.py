@@ -443,3 +443,28 @@ def test_edge_inputs_fp64_vs_ref64(S, ora, case):
                    ora.run(one, mode=ora.REF64)["trial_ix"]) >= 0.9995
     g32 = S.run_stage2_hip(prep, form="fp32")
     assert np.mean(g32["trial_ix"] == r["trial_ix"]) > 0.995
+
+
+def test_graph_replays_equal_stepwise_launches(S, prep_small):
+    """The default loop replays one captured iteration; with timing on the same
+    kernels are launched one by one with host reads between iterations.  Every
+    replay, with eager work and host synchronisation in between, gives the
+    step-wise result bit for bit."""
+    keys = ("K_ps", "C_ps", "coh_ps", "ph_grid", "ph_weight", "ph_res", "Nr", "trial_ix")
+    with S.Stage2Session(prep_small, form="fp32") as s:
+        s.run()
+        first = s.download()
+        s.set_timing(True)
+        s.run()
+        stepwise = s.download()
+        s.set_timing(False)
+        for _ in range(3):
+            _ = s.fill_grid(np.zeros(prep_small.n_ps), np.ones(prep_small.n_ps))  # eager work between replays
+            il, _ = s.run()
+            again = s.download()
+            assert il == stepwise["i_loop"]
+            for key in keys:
+                assert np.array_equal(again[key], stepwise[key], equal_nan=True), key
+        for key in keys:
+            assert np.array_equal(first[key], stepwise[key], equal_nan=True), key
+        assert first["gamma_change_save"] == stepwise["gamma_change_save"]
