@@ -486,6 +486,16 @@ FitParams fit_params(ps2_session *s) {
     return f;
 }
 
+// positive trials (incl. 0) per scan chunk: the smallest of 8/16/24/32/48
+// holding 0..tn in the FP32 form (one pass over the data), 8/16 in the FP64 form
+int scan_chunk(const ps2_session *s) {
+    if (s->tn < 8) return 8;
+    if (s->form != PS2_FORM_FP32 || s->tn < 16) return 16;
+    if (s->tn < 24) return 24;
+    if (s->tn < 32) return 32;
+    return 48;
+}
+
 template <typename R>
 int launch_fit(ps2_session *s) {
     const FitParams fp = fit_params(s);
@@ -497,18 +507,26 @@ int launch_fit(ps2_session *s) {
     }
     {
     SlotTimer tm(s, SLOT_SCAN);
-    if (s->tn < 8) {
-        hipLaunchKernelGGL((k_scan<R, 8>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
+                           s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                            s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
-                           s->rescore_list.as<int32_t>(),
-                           s->rescore_count.as<uint32_t>(), fp, s->gate);
-    } else {
-        hipLaunchKernelGGL((k_scan<R, 16>), dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
-                           s->rescore_list.as<int32_t>(),
-                           s->rescore_count.as<uint32_t>(), fp, s->gate);
+                           s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(), fp, s->gate);
+    };
+    // trials per chunk: the FP32 form scans all trials in one pass up to 48
+    // positive trials (larger grids take 48-trial chunks)
+    const int tc = scan_chunk(s);
+    if (tc == 8) {
+        go(k_scan<R, 8>);
+    } else if (tc == 16) {
+        go(k_scan<R, 16>);
+    } else if constexpr (sizeof(R) == 4) {
+        if (tc == 24)
+            go(k_scan<R, 24>);
+        else if (tc == 32)
+            go(k_scan<R, 32>);
+        else
+            go(k_scan<R, 48>);
     }
     PS2_LAUNCHED();
     }
@@ -783,12 +801,13 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
                          " has CLAP windows but is narrower than the 24-cell window (the reference's "
                          "window placement underflows here)");
     // derived FP32 re-score bound: the scan's worst-case rounding error on one
-    // coherence, doubled (recurrence drift over a 16-trial chunk + an I-term
-    // f32 sum + the sincos anchors), in units of 2^-24
+    // coherence, doubled (recurrence drift over one trial chunk of 8-48 trials
+    // + an I-term f32 sum + the sincos anchors), in units of 2^-24
     if (opts && opts->rescore_bound > 0.0)
         s->rescore_bound = (float)opts->rescore_bound;
     else
-        s->rescore_bound = (float)(2.0 * (16.0 + 8.0 + 2.0 * (double)s->I) * 5.9604644775390625e-08);
+        s->rescore_bound = (float)(2.0 * ((double)scan_chunk(s.get()) + 8.0 + 2.0 * (double)s->I) *
+                                   5.9604644775390625e-08);
     if (s->form == PS2_FORM_FP64) {
         if ((rc = upload_inputs<double>(s.get(), in))) return rc;
         if ((rc = alloc_state<double>(s.get()))) return rc;
