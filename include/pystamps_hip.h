@@ -180,6 +180,19 @@ int ps2_hip_grid_accumulate(const float *ph_weight, const int64_t *grid_lin, int
 int ps2_hip_histogram_with_centers(const double *values, int64_t n, const double *centers,
                                    int64_t n_centers, double *out);
 
+/* ---- Prep helpers (host code, host threads; no device use) ----
+ * prepare_stage2_inputs' phase normalisation (native_stage2.rs:449-467): for
+ * every row of ph [n_rows][n_cols] complex64 and each kept column cols[i]
+ * (the master column dropped), amp = |v| (glibc hypotf), 1 where it is 0, and
+ * ph_nm = v / amp; outputs [n_rows][n_sel].  n_threads <= 0: all cores. */
+int ps2_hip_normalize_phase(const float *ph, int64_t n_rows, int64_t n_cols, const int64_t *cols,
+                            int64_t n_sel, int32_t n_threads, float *ph_nm, float *amp);
+
+/* Column selection of a row-major f64 matrix (bp1.bperp_mat without the
+ * master column, load_bperp_mat native_stage2.rs:594-606). */
+int ps2_hip_select_columns_f64(const double *src, int64_t n_rows, int64_t n_cols,
+                               const int64_t *cols, int64_t n_sel, int32_t n_threads, double *dst);
+
 /* ---- Sessions: device-resident inputs, for repeated runs and per-step use ----
  * A session uploads a patch's constant inputs once.  ps2_hip_session_run then
  * runs the whole loop from the initial state (K = 0, weighting = 1/D_A) with

@@ -104,6 +104,10 @@ def lib() -> C.CDLL:
                                           C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_float)]
     L.ps2_hip_histogram_with_centers.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p]
     L.ps2_hip_session_export_device.argtypes = [C.c_void_p, C.POINTER(DeviceOutputs)]
+    L.ps2_hip_normalize_phase.argtypes = [C.POINTER(C.c_float), C.c_int64, C.c_int64, _i64p, C.c_int64,
+                                          C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.ps2_hip_select_columns_f64.argtypes = [_f64p, C.c_int64, C.c_int64, _i64p, C.c_int64, C.c_int32,
+                                             _f64p]
     L.ps2_hip_session_create.argtypes = [C.POINTER(PatchIn), C.POINTER(RunOpts), C.POINTER(C.c_void_p)]
     L.ps2_hip_session_run.argtypes = [C.c_void_p, _i64p, _f64p]
     L.ps2_hip_session_download.argtypes = [C.c_void_p, C.POINTER(PatchOut)]

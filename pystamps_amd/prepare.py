@@ -285,6 +285,43 @@ def small_baseline_random_indices(ifgday_ix, n_ifg: int):
 NrFn = Callable[..., tuple]
 
 
+def _threads() -> int:
+    import os
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def normalize_phase(ph_full: np.ndarray, cols) -> tuple[np.ndarray, np.ndarray]:
+    """ph_nm and amp of the kept columns (native_stage2.rs:449-467) with the
+    library's threaded host routine; f32 results equal numpy's hypot and
+    divisions."""
+    ph = np.ascontiguousarray(ph_full, dtype=np.complex64)
+    P, n_full = ph.shape
+    c = np.ascontiguousarray(cols, dtype=np.int64)
+    ph_nm = np.empty((P, c.size), np.complex64)
+    amp = np.empty((P, c.size), np.float32)
+    f32p = C.POINTER(C.c_float)
+    rc = _native.lib().ps2_hip_normalize_phase(
+        ph.ctypes.data_as(f32p), P, n_full, c.ctypes.data_as(C.POINTER(C.c_int64)), c.size, _threads(),
+        ph_nm.ctypes.data_as(f32p), amp.ctypes.data_as(f32p))
+    if rc != 0:
+        raise Stage2Error(_native.last_error())
+    return ph_nm, amp
+
+
+def select_columns(src: np.ndarray, cols) -> np.ndarray:
+    """Row-major f64 column selection with the library's threaded routine."""
+    a = np.ascontiguousarray(src, dtype=np.float64)
+    c = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.empty((a.shape[0], c.size), np.float64)
+    f64p = C.POINTER(C.c_double)
+    rc = _native.lib().ps2_hip_select_columns_f64(
+        a.ctypes.data_as(f64p), a.shape[0], a.shape[1], c.ctypes.data_as(C.POINTER(C.c_int64)), c.size,
+        _threads(), out.ctypes.data_as(f64p))
+    if rc != 0:
+        raise Stage2Error(_native.last_error())
+    return out
+
+
 def prepare_stage2_inputs(raw: RawPatch, nr_fn: Optional[NrFn] = None,
                           nr_rows: int = STAGE2_RANDOM_COUNT, device: int = 0,
                           nr_sb_fn: Optional[NrFn] = None) -> Stage2Prepared:
@@ -312,16 +349,33 @@ def prepare_stage2_inputs(raw: RawPatch, nr_fn: Optional[NrFn] = None,
     small_baseline = parms.small_baseline_flag.lower() == "y"
     no_master = [ix for ix in range(n_ifg_full) if small_baseline or ix != master_ix - 1]
     n_ifg = len(no_master)
-    ph = np.ascontiguousarray(ph_full[:, no_master]).astype(np.complex64)
-    re = ph.real.astype(np.float32)
-    im = ph.imag.astype(np.float32)
-    mag = np.hypot(re, im)
-    safe = np.where(mag == np.float32(0.0), np.float32(1.0), mag).astype(np.float32)
-    ph_nm = np.empty_like(ph)
-    ph_nm.real = re / safe
-    ph_nm.imag = im / safe
-    amp = safe
     bperp_nm = bperp_full[no_master]
+
+    # the random-coherence histogram needs only bperp and the trial grid: it
+    # runs on a worker thread (its C call releases the GIL) while the phases
+    # are normalised
+    bins = coh_bins()
+    mean_incidence = trial_wrap_mean_incidence(raw)
+    max_k = options.max_topo_err / (options.lambda_m * TRIAL_RHO_M * math.sin(mean_incidence)
+                                    / (4.0 * math.pi))
+    min_bp = min(bperp_nm) if bperp_nm.size else math.inf
+    max_bp = max(bperp_nm) if bperp_nm.size else -math.inf
+    n_trial_wraps = float((max_bp - min_bp) * max_k / (2.0 * math.pi))
+    trials = trial_values(n_trial_wraps)
+    low_coh_thresh = 15 if small_baseline else 31
+    if small_baseline:
+        ia, ib, n_image = small_baseline_random_indices(raw.ifgday_ix, n_ifg)
+        sb_fn = nr_sb_fn if nr_sb_fn is not None else (
+            lambda bp, a, b, ni, tv, cb, n: nr_hip_sb(bp, a, b, ni, tv, cb, n, device=device))
+        nr_call = lambda: sb_fn(bperp_nm, ia, ib, n_image, trials, bins, nr_rows)  # noqa: E731
+    else:
+        sm_fn = nr_fn if nr_fn is not None else (lambda bp, tv, cb, n: nr_hip(bp, tv, cb, n, device=device))
+        nr_call = lambda: sm_fn(bperp_nm, trials, bins, nr_rows)  # noqa: E731
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        nr_future = pool.submit(nr_call)
+        ph_nm, amp = normalize_phase(ph_full, no_master)
+        nr_base, nr_max_nz_ix = nr_future.result()
 
     # load_bperp_mat, native_stage2.rs:563-612
     bperp_mat = None
@@ -340,7 +394,7 @@ def prepare_stage2_inputs(raw: RawPatch, nr_fn: Optional[NrFn] = None,
         if src.shape[1] == n_ifg:
             bperp_mat = np.ascontiguousarray(src)
         elif not small_baseline and src.shape[1] == n_ifg_full:
-            bperp_mat = np.ascontiguousarray(src[:, no_master])
+            bperp_mat = select_columns(src, no_master)
         else:
             raise Stage2Error(
                 f"bp1.bperp_mat has incompatible shape {src.shape[0]}x{src.shape[1]} for stage-2 "
@@ -369,23 +423,6 @@ def prepare_stage2_inputs(raw: RawPatch, nr_fn: Optional[NrFn] = None,
     grid_lin = (grid_ij[:, 0].astype(np.int64) - 1) * n_j + (grid_ij[:, 1].astype(np.int64) - 1)
 
     low_pass = build_low_pass(options)
-    bins = coh_bins()
-    mean_incidence = trial_wrap_mean_incidence(raw)
-    max_k = options.max_topo_err / (options.lambda_m * TRIAL_RHO_M * math.sin(mean_incidence)
-                                    / (4.0 * math.pi))
-    min_bp = min(bperp_nm) if bperp_nm.size else math.inf
-    max_bp = max(bperp_nm) if bperp_nm.size else -math.inf
-    n_trial_wraps = float((max_bp - min_bp) * max_k / (2.0 * math.pi))
-    trials = trial_values(n_trial_wraps)
-    low_coh_thresh = 15 if small_baseline else 31
-    if small_baseline:
-        ia, ib, n_image = small_baseline_random_indices(raw.ifgday_ix, n_ifg)
-        fn = nr_sb_fn if nr_sb_fn is not None else (
-            lambda bp, a, b, ni, tv, cb, n: nr_hip_sb(bp, a, b, ni, tv, cb, n, device=device))
-        nr_base, nr_max_nz_ix = fn(bperp_nm, ia, ib, n_image, trials, bins, nr_rows)
-    else:
-        fn = nr_fn if nr_fn is not None else (lambda bp, tv, cb, n: nr_hip(bp, tv, cb, n, device=device))
-        nr_base, nr_max_nz_ix = fn(bperp_nm, trials, bins, nr_rows)
 
     return Stage2Prepared(
         n_ps=n_ps, n_ifg=n_ifg, ph_nm=ph_nm, amp=amp, bperp_mat=bperp_mat,

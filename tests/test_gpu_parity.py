@@ -73,6 +73,27 @@ def test_gpu_nr_matches_oracle(ora, prep_c1):
     assert mx_g == mx_o
 
 
+def test_gpu_nr_draw_reuse_across_shapes(ora):
+    """The library keeps the last uniform-draw stream: calls that alternate row
+    counts, interferogram counts and the small-baseline image count must each
+    equal the oracle's fresh draw."""
+    from pystamps_amd.prepare import coh_bins, nr_hip, nr_hip_sb, trial_values
+    tv, cb = trial_values(0.6), coh_bins()
+    bp_a, bp_b = np.linspace(-150.0, 170.0, 9), np.linspace(-90.0, 260.0, 14)
+    ia = np.arange(9, dtype=np.int32) % 8
+    ib = (ia + 1).astype(np.int32)
+    for kind, bp, rows in (("sm", bp_a, 7000), ("sm", bp_a, 7000), ("sm", bp_b, 7000),
+                           ("sb", bp_a, 7000), ("sm", bp_a, 5000), ("sm", bp_a, 7000)):
+        if kind == "sm":
+            got = nr_hip(bp, tv, cb, n_rows=rows)
+            ref = ora.random_coherence_histogram(bp, tv, cb, n_rows=rows)
+        else:
+            got = nr_hip_sb(bp, ia, ib, 9, tv, cb, n_rows=rows)
+            ref = ora.random_coherence_histogram_sb(bp, ia, ib, 9, tv, cb, n_rows=rows)
+        assert np.array_equal(got[0], ref[0]), (kind, bp.size, rows)
+        assert got[1] == ref[1]
+
+
 # --- state-injected steps ----------------------------------------------------------
 
 def _state(prep, seed=11):

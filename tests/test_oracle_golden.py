@@ -339,3 +339,33 @@ def test_small_baseline_indices_and_errors():
         small_baseline_random_indices(np.ones((3, 3)), 2)
     with pytest.raises(Stage2Error, match="missing"):
         small_baseline_random_indices(None, 2)
+
+
+def test_library_phase_normalisation_equals_numpy_f32():
+    """The library's threaded prep routine reproduces prepare_stage2_inputs'
+    f32 arithmetic (native_stage2.rs:449-467; numpy float32 hypot is libm
+    hypotf, divisions round once) bit for bit, including zeros, infinities,
+    NaN and subnormals; and the column selection is a plain copy."""
+    from pystamps_amd.prepare import normalize_phase, select_columns
+    rng = np.random.default_rng(7)
+    P, n = 3000, 9
+    re = rng.standard_normal((P, n)).astype(np.float32) * np.float32(10.0) ** rng.integers(-40, 30, (P, n)).astype(np.float32)
+    im = rng.standard_normal((P, n)).astype(np.float32)
+    re[0, :3] = [0.0, np.inf, np.nan]
+    im[0, :3] = [0.0, 1.0, 2.0]
+    re[1, :2] = [1e-45, -0.0]
+    im[1, :2] = [0.0, 0.0]
+    ph = (re + 1j * im).astype(np.complex64)
+    cols = [0, 2, 3, 5, 8, 1]
+    got_nm, got_amp = normalize_phase(ph, cols)
+    sel = ph[:, cols]
+    r, i = sel.real.astype(np.float32), sel.imag.astype(np.float32)
+    with np.errstate(all="ignore"):
+        mag = np.hypot(r, i)
+        safe = np.where(mag == np.float32(0.0), np.float32(1.0), mag).astype(np.float32)
+        ref_re, ref_im = r / safe, i / safe
+    assert np.array_equal(got_amp, safe, equal_nan=True)
+    assert np.array_equal(got_nm.real, ref_re, equal_nan=True)
+    assert np.array_equal(got_nm.imag, ref_im, equal_nan=True)
+    m = rng.standard_normal((P, n))
+    assert np.array_equal(select_columns(m, cols), m[:, cols])
