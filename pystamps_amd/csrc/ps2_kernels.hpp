@@ -724,11 +724,36 @@ __global__ __launch_bounds__(256) void k_planes_f32(const float *__restrict__ sr
 }
 
 // sorted pixel k -> the caller's row order[k]
-__global__ __launch_bounds__(256) void k_unpermute_f64(const double *__restrict__ src,
-                                                       const int64_t *__restrict__ order,
-                                                       double *__restrict__ dst, int64_t P) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_unpermute(const T *__restrict__ src, const int64_t *__restrict__ order,
+                                                   T *__restrict__ dst, int64_t P) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < P) dst[order[k]] = src[k];
+}
+
+// grid planes [I][C] -> complex64 cells [C][I] (the caller's ph_grid layout),
+// through a 32 x 32 LDS tile so that both sides are coalesced
+template <typename R>
+__global__ __launch_bounds__(256) void k_grid_to_cells(const cplx<R> *__restrict__ grid, float2 *__restrict__ out,
+                                                       int64_t C, int I) {
+    __shared__ float2 tile[32][33];
+    const int64_t c0 = (int64_t)blockIdx.x * 32;
+    const int f0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int f = f0 + r;
+        const int64_t c = c0 + tx;
+        if (f < I && c < C) {
+            const cplx<R> v = grid[(int64_t)f * C + c];
+            tile[r][tx] = make_float2((float)v.re, (float)v.im);
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t c = c0 + r;
+        const int f = f0 + tx;
+        if (f < I && c < C) out[c * I + f] = tile[tx][r];
+    }
 }
 
 // Loop-invariant inputs of the trial scan (native_stage2.rs:1215-1225),

@@ -100,6 +100,123 @@ struct HostPinned {
     }
 };
 
+// Bulk host <-> device copies of caller-owned (pageable) arrays.  A direct
+// hipMemcpy from pageable memory runs at ~10-17 GB/s and faults the caller's
+// fresh pages in on one thread; here the data moves through two pinned chunks
+// of one process-wide pool: the DMA of one chunk overlaps a multi-threaded
+// host copy of the other.  Transfers share the pool under a mutex.
+class PinnedStaging {
+  public:
+    static constexpr size_t kChunk = (size_t)64 << 20;
+    static constexpr size_t kDirect = (size_t)2 << 20;   // below this: plain hipMemcpy
+
+    int d2h(void *dst, const void *src, size_t n, hipStream_t st) {
+        if (n == 0) return PS2_OK;
+        if (n < kDirect) {
+            PS2_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, st));
+            PS2_HIP(hipStreamSynchronize(st));
+            return PS2_OK;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        int rc;
+        if ((rc = ensure())) return rc;
+        Events ev;
+        if ((rc = ev.create())) return rc;
+        const size_t nk = (n + kChunk - 1) / kChunk;
+        auto issue = [&](size_t k) -> int {
+            const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+            PS2_HIP(hipMemcpyAsync(buf_[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, st));
+            PS2_HIP(hipEventRecord(ev.e[k & 1], st));
+            return PS2_OK;
+        };
+        if ((rc = issue(0))) return rc;
+        for (size_t k = 0; k < nk; ++k) {
+            // the other buffer held chunk k-1, already copied out below
+            if (k + 1 < nk && (rc = issue(k + 1))) return rc;
+            PS2_HIP(hipEventSynchronize(ev.e[k & 1]));
+            const size_t off = k * kChunk;
+            host_copy((char *)dst + off, buf_[k & 1], std::min(kChunk, n - off));
+        }
+        return PS2_OK;
+    }
+
+    int h2d(void *dst, const void *src, size_t n, hipStream_t st) {
+        if (n == 0) return PS2_OK;
+        if (n < kDirect) {
+            PS2_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+            PS2_HIP(hipStreamSynchronize(st));
+            return PS2_OK;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        int rc;
+        if ((rc = ensure())) return rc;
+        Events ev;
+        if ((rc = ev.create())) return rc;
+        const size_t nk = (n + kChunk - 1) / kChunk;
+        for (size_t k = 0; k < nk; ++k) {
+            // this buffer's previous DMA (chunk k-2) must have left it
+            if (k >= 2) PS2_HIP(hipEventSynchronize(ev.e[k & 1]));
+            const size_t off = k * kChunk, len = std::min(kChunk, n - off);
+            host_copy(buf_[k & 1], (const char *)src + off, len);
+            PS2_HIP(hipMemcpyAsync((char *)dst + off, buf_[k & 1], len, hipMemcpyHostToDevice, st));
+            PS2_HIP(hipEventRecord(ev.e[k & 1], st));
+        }
+        PS2_HIP(hipStreamSynchronize(st));
+        return PS2_OK;
+    }
+
+  private:
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        int create() {
+            for (auto &x : e) PS2_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            return PS2_OK;
+        }
+        ~Events() {
+            for (auto x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    };
+
+    int ensure() {
+        for (auto &b : buf_) {
+            if (b) continue;
+            hipError_t e = hipHostMalloc(&b, kChunk, hipHostMallocPortable);
+            if (e != hipSuccess) {
+                b = nullptr;
+                set_error(std::string("pinned staging allocation failed: ") + hipGetErrorString(e));
+                return PS2_E_NOMEM;
+            }
+        }
+        return PS2_OK;
+    }
+
+    static void host_copy(void *dst, const void *src, size_t n) {
+        constexpr size_t kSlice = (size_t)4 << 20;
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>({8, hw, (n + kSlice - 1) / kSlice});
+        if (nt <= 1) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        std::vector<std::thread> th;
+        th.reserve(nt);
+        for (size_t t = 0; t < nt; ++t) {
+            const size_t a = n * t / nt, b = n * (t + 1) / nt;
+            th.emplace_back([=]() { std::memcpy((char *)dst + a, (const char *)src + a, b - a); });
+        }
+        for (auto &x : th) x.join();
+    }
+
+    std::mutex mu_;
+    void *buf_[2] = {nullptr, nullptr};
+};
+
+PinnedStaging &staging() {
+    static PinnedStaging *pool = new PinnedStaging();   // process lifetime
+    return *pool;
+}
+
 }  // namespace ps2
 
 using namespace ps2;
@@ -126,6 +243,7 @@ struct ps2_session {
     // device state
     DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, wt, offset;
     DBuf ex_k, ex_c, ex_coh, ex_nopt, ex_patch, ex_res;  // device export (row order)
+    DBuf dl_bulk, dl_vec;  // download scratch: finalized planes and row-order vectors
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
@@ -232,7 +350,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     if ((rc = stage.alloc(sizeof(double) * P * I))) return rc;
     {   // ph_nm planes [I][P]
         if ((rc = s->phnm.alloc(sizeof(cplx<R>) * P * I))) return rc;
-        PS2_HIP(hipMemcpy(stage.p, in->ph_nm, sizeof(float) * 2 * P * I, hipMemcpyHostToDevice));
+        if ((rc = staging().h2d(stage.p, in->ph_nm, sizeof(float) * 2 * P * I, s->stream))) return rc;
         hipLaunchKernelGGL(k_planes_cplx<R>, dim3(nb), dim3(256), 0, s->stream, stage.as<float2>(),
                            s->d_order.as<int64_t>(), s->phnm.as<cplx<R>>(), P, (int)I);
         PS2_LAUNCHED();
@@ -250,7 +368,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
                 PS2_HIP(hipMemcpy(s->bp64.p, in->bperp_row, sizeof(double) * n, hipMemcpyHostToDevice));
         } else {
             PS2_HIP(hipStreamSynchronize(s->stream));  // the staging buffer is free again
-            PS2_HIP(hipMemcpy(stage.p, in->bperp_mat, sizeof(double) * P * I, hipMemcpyHostToDevice));
+            if ((rc = staging().h2d(stage.p, in->bperp_mat, sizeof(double) * P * I, s->stream))) return rc;
             hipLaunchKernelGGL(k_planes_real<R>, dim3(nb), dim3(256), 0, s->stream, stage.as<double>(),
                                s->d_order.as<int64_t>(), s->bp.as<R>(),
                                sizeof(R) == 4 ? s->bp64.as<double>() : nullptr, P, (int)I);
@@ -260,7 +378,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     if (in->amp && s->weighting == PS2_WEIGHT_SNR) {
         if ((rc = s->amp.alloc(sizeof(float) * P * I))) return rc;
         PS2_HIP(hipStreamSynchronize(s->stream));
-        PS2_HIP(hipMemcpy(stage.p, in->amp, sizeof(float) * P * I, hipMemcpyHostToDevice));
+        if ((rc = staging().h2d(stage.p, in->amp, sizeof(float) * P * I, s->stream))) return rc;
         hipLaunchKernelGGL(k_planes_f32, dim3(nb), dim3(256), 0, s->stream, stage.as<float>(),
                            s->d_order.as<int64_t>(), s->amp.as<float>(), P, (int)I);
         PS2_LAUNCHED();
@@ -688,63 +806,66 @@ int run_loop(ps2_session *s) {
 }
 
 // download helpers ------------------------------------------------------------
+// Every output is put in the caller's row order and layout on the device (into
+// session-owned scratch) and then moved by one bulk copy.
 
-int download_sorted_vec(ps2_session *s, const DBuf &src, double *dst) {
-    if (!dst) return PS2_OK;
-    std::vector<double> h(s->P);
-    PS2_HIP(hipMemcpy(h.data(), src.p, sizeof(double) * s->P, hipMemcpyDeviceToHost));
-    for (int64_t k = 0; k < s->P; ++k) dst[s->order[k]] = h[k];
+int dl_scratch(ps2_session *s, size_t bulk_bytes) {
+    int rc;
+    if (s->dl_vec.bytes < sizeof(double) * s->P && (rc = s->dl_vec.alloc(sizeof(double) * s->P))) return rc;
+    if (s->dl_bulk.bytes < bulk_bytes && (rc = s->dl_bulk.alloc(bulk_bytes))) return rc;
     return PS2_OK;
+}
+
+template <typename T>
+int download_row_vec(ps2_session *s, const DBuf &src, T *dst) {
+    if (!dst) return PS2_OK;
+    hipLaunchKernelGGL(k_unpermute<T>, dim3(blocks_for(s->P, 256)), dim3(256), 0, s->stream, src.as<T>(),
+                       s->d_order.as<int64_t>(), s->dl_vec.as<T>(), s->P);
+    PS2_LAUNCHED();
+    return staging().d2h(dst, s->dl_vec.p, sizeof(T) * s->P, s->stream);
 }
 
 template <typename R>
 int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
     const int64_t P = s->P, I = s->I, C = s->C;
     int rc;
-    if ((rc = download_sorted_vec(s, s->K, out->k_ps))) return rc;
-    if ((rc = download_sorted_vec(s, s->Cc, out->c_ps))) return rc;
-    if ((rc = download_sorted_vec(s, s->coh, out->coh_ps))) return rc;
-    if ((rc = download_sorted_vec(s, s->nopt, out->n_opt))) return rc;
-    if (out->trial_ix) {
-        std::vector<int32_t> h(P);
-        PS2_HIP(hipMemcpy(h.data(), s->tix.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
-        for (int64_t k = 0; k < P; ++k) out->trial_ix[s->order[k]] = h[k];
-    }
+    const size_t plane = (size_t)P * I;
+    size_t bulk = (out->ph_patch || out->ph_res) ? plane * (sizeof(float2) + sizeof(float)) : 0;
+    if (loop_outputs && out->ph_weight) bulk = std::max(bulk, plane * sizeof(float2));
+    if (loop_outputs && out->ph_grid) bulk = std::max(bulk, (size_t)C * I * sizeof(float2));
+    if ((rc = dl_scratch(s, bulk))) return rc;
+    if ((rc = download_row_vec(s, s->K, out->k_ps))) return rc;
+    if ((rc = download_row_vec(s, s->Cc, out->c_ps))) return rc;
+    if ((rc = download_row_vec(s, s->coh, out->coh_ps))) return rc;
+    if ((rc = download_row_vec(s, s->nopt, out->n_opt))) return rc;
+    if ((rc = download_row_vec(s, s->tix, out->trial_ix))) return rc;
     if (out->ph_patch || out->ph_res) {
-        DBuf pbuf, rbuf;
-        if (out->ph_patch && (rc = pbuf.alloc(sizeof(float2) * P * I))) return rc;
-        if (out->ph_res && (rc = rbuf.alloc(sizeof(float) * P * I))) return rc;
+        float2 *pbuf = s->dl_bulk.as<float2>();
+        float *rbuf = reinterpret_cast<float *>(pbuf + plane);
         hipLaunchKernelGGL(k_final_patch_res<R>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
                            s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
-                           s->d_order.as<int64_t>(), pbuf.as<float2>(), rbuf.as<float>(), P, (int)I,
-                           s->rowinv, s->C);
+                           s->d_order.as<int64_t>(), out->ph_patch ? pbuf : nullptr,
+                           out->ph_res ? rbuf : nullptr, P, (int)I, s->rowinv, s->C);
         PS2_LAUNCHED();
-        PS2_HIP(hipStreamSynchronize(s->stream));
-        if (out->ph_patch)
-            PS2_HIP(hipMemcpy(out->ph_patch, pbuf.p, sizeof(float2) * P * I, hipMemcpyDeviceToHost));
-        if (out->ph_res) PS2_HIP(hipMemcpy(out->ph_res, rbuf.p, sizeof(float) * P * I, hipMemcpyDeviceToHost));
+        if (out->ph_patch && (rc = staging().d2h(out->ph_patch, pbuf, sizeof(float2) * plane, s->stream)))
+            return rc;
+        if (out->ph_res && (rc = staging().d2h(out->ph_res, rbuf, sizeof(float) * plane, s->stream))) return rc;
     }
     if (!loop_outputs) return PS2_OK;
-    if ((rc = download_sorted_vec(s, s->cohprev, out->coh_ps_save))) return rc;
+    if ((rc = download_row_vec(s, s->cohprev, out->coh_ps_save))) return rc;
     if (out->ph_weight) {
-        DBuf wbuf;
-        if ((rc = wbuf.alloc(sizeof(float2) * P * I))) return rc;
         hipLaunchKernelGGL(k_final_weight<R>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->Kprev.as<double>(),
-                           s->Wprev.as<double>(), s->d_order.as<int64_t>(), wbuf.as<float2>(), P, (int)I);
+                           s->Wprev.as<double>(), s->d_order.as<int64_t>(), s->dl_bulk.as<float2>(), P, (int)I);
         PS2_LAUNCHED();
-        PS2_HIP(hipStreamSynchronize(s->stream));
-        PS2_HIP(hipMemcpy(out->ph_weight, wbuf.p, sizeof(float2) * P * I, hipMemcpyDeviceToHost));
+        if ((rc = staging().d2h(out->ph_weight, s->dl_bulk.p, sizeof(float2) * plane, s->stream))) return rc;
     }
-    if (out->ph_grid) {   // device [I][C] -> caller [C][I]
-        std::vector<cplx<R>> h((size_t)(C * I));
-        PS2_HIP(hipMemcpy(h.data(), s->grid.p, sizeof(cplx<R>) * C * I, hipMemcpyDeviceToHost));
-        for (int64_t f = 0; f < I; ++f)
-            for (int64_t c = 0; c < C; ++c) {
-                out->ph_grid[2 * (c * I + f)] = (float)h[f * C + c].re;
-                out->ph_grid[2 * (c * I + f) + 1] = (float)h[f * C + c].im;
-            }
+    if (out->ph_grid && C * I > 0) {   // device [I][C] -> caller [C][I]
+        hipLaunchKernelGGL(k_grid_to_cells<R>, dim3((unsigned)((C + 31) / 32), (unsigned)((I + 31) / 32)),
+                           dim3(256), 0, s->stream, s->grid.as<cplx<R>>(), s->dl_bulk.as<float2>(), C, (int)I);
+        PS2_LAUNCHED();
+        if ((rc = staging().d2h(out->ph_grid, s->dl_bulk.p, sizeof(float2) * C * I, s->stream))) return rc;
     }
     if (out->nr)
         for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
@@ -888,7 +1009,7 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out) {
     const std::pair<DBuf *, DBuf *> vecs[] = {
         {&s->K, &s->ex_k}, {&s->Cc, &s->ex_c}, {&s->coh, &s->ex_coh}, {&s->nopt, &s->ex_nopt}};
     for (const auto &pr : vecs) {
-        hipLaunchKernelGGL(k_unpermute_f64, dim3(nb), dim3(256), 0, s->stream, pr.first->as<double>(),
+        hipLaunchKernelGGL(k_unpermute<double>, dim3(nb), dim3(256), 0, s->stream, pr.first->as<double>(),
                            s->d_order.as<int64_t>(), pr.second->as<double>(), P);
         PS2_LAUNCHED();
     }

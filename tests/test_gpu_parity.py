@@ -419,6 +419,31 @@ def test_export_device_matches_download(S, prep_small):
         assert np.array_equal(dev["Nr"], host["Nr"])
 
 
+def test_large_patch_transfers_one_iteration_fp64_vs_ref64(S, ora):
+    """300,000 x 60: the inputs and every plane output cross the bus in several
+    pinned chunks.  One FP64 iteration against ref64, and the host download
+    against the device export."""
+    prep = dataclasses.replace(small_prep(n_ps=300_000, n_ifg=60, extent=25_000.0, seed=0x5EED1700),
+                               gamma_max_iterations=1)
+    P, I = prep.n_ps, prep.n_ifg
+    assert P * I * 8 > 2 * (64 << 20)
+    with S.Stage2Session(prep, form="fp64") as s:
+        s.run()
+        g = s.download()
+        dev = s.export_device()
+        assert np.array_equal(_hip_to_host(dev["ph_patch"], (P, I), np.complex64), g["ph_patch"])
+        assert np.array_equal(_hip_to_host(dev["ph_res"], (P, I), np.float32), g["ph_res"])
+        assert np.array_equal(_hip_to_host(dev["coh_ps"], (P,), np.float64), g["coh_ps"], equal_nan=True)
+    r = ora.run(prep, mode=ora.REF64)
+    assert g["i_loop"] == r["i_loop"] == 1
+    assert np.array_equal(g["trial_ix"], r["trial_ix"])
+    for key in ("K_ps", "coh_ps", "coh_ps_save"):
+        np.testing.assert_allclose(g[key], r[key], rtol=0, atol=1e-9)
+    for key in ("ph_grid", "ph_weight", "ph_patch"):
+        np.testing.assert_allclose(g[key], r[key].astype(np.complex64), rtol=1e-5, atol=1e-5)
+    assert np.all(_phase_close(g["ph_res"], r["ph_res"], 1e-4))
+
+
 # --- inputs the reference treats specially --------------------------------------------
 
 def _edge_prep(ora, n_ps=5000, n_ifg=12, seed=0x5EED1500, nan_frac=0.0, da_zero_frac=0.0):
