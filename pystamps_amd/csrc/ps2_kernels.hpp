@@ -145,7 +145,15 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
     if (mine) grid[(int64_t)f * C + c] = {sre, sim};
 }
 
-// ph_weight output of the last fill, written in the caller's row order.
+// Outputs in the caller's row order [P][I] from sorted ifg-major planes [I][P]:
+// a workgroup takes FINAL_TP sorted pixels x FINAL_TF interferograms, reads
+// the planes along the pixel axis, and writes each pixel's run of
+// interferograms as one contiguous row segment (order[k] scatters whole rows,
+// not single elements).  Grid (ceil(P / FINAL_TP), ceil(I / FINAL_TF)).
+constexpr int FINAL_TP = 32;
+constexpr int FINAL_TF = 64;
+
+// ph_weight output of the last fill.
 template <typename R>
 __global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict__ phnm,
                                                       const R *__restrict__ bp, int bp_rowinv,
@@ -153,13 +161,24 @@ __global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict_
                                                       const double *__restrict__ W,
                                                       const int64_t *__restrict__ order,
                                                       float2 *__restrict__ out, int64_t P, int I) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= P * I) return;
-    const int f = (int)(gid / P);
-    const int64_t k = gid - (int64_t)f * P;
-    const R b = bp_rowinv ? bp[f] : bp[gid];
-    const cplx<R> v = weight_value<R>(phnm[gid], b, K[k], W[k]);
-    out[order[k] * I + f] = make_float2((float)v.re, (float)v.im);
+    __shared__ float2 tile[FINAL_TP][FINAL_TF + 1];
+    const int64_t k0 = (int64_t)blockIdx.x * FINAL_TP;
+    const int f0 = blockIdx.y * FINAL_TF;
+    const int np = (int)min<int64_t>(FINAL_TP, P - k0), nf = min(FINAL_TF, I - f0);
+    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
+        const int kl = e % FINAL_TP, fl = e / FINAL_TP;
+        if (kl < np && fl < nf) {
+            const int64_t k = k0 + kl, g = (int64_t)(f0 + fl) * P + k;
+            const R b = bp_rowinv ? bp[f0 + fl] : bp[g];
+            const cplx<R> v = weight_value<R>(phnm[g], b, K[k], W[k]);
+            tile[kl][fl] = make_float2((float)v.re, (float)v.im);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
+        const int kl = e / FINAL_TF, fl = e % FINAL_TF;
+        if (kl < np && fl < nf) out[order[k0 + kl] * I + f0 + fl] = tile[kl][fl];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1194,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
 }
 
 // ph_patch (unit filt) for every row and ph_res = arg(residual) of valid rows,
-// in the caller's row order.
+// in the caller's row order (tiled like k_final_weight).
 template <typename R>
 __global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restrict__ phnm,
                                                          const R *__restrict__ bp,
@@ -1206,23 +1225,37 @@ __global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restri
                                                          float2 *__restrict__ patch_out,
                                                          float *__restrict__ res_out, int64_t P,
                                                          int I, int rowinv, int64_t C) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= P * I) return;
-    const int f = (int)(gid / P);
-    const int64_t k = gid - (int64_t)f * P;
-    const cplx<R> fv = filt[(int64_t)f * C + cell_of[k]];
-    const cplx<R> p = fv;
-    const int64_t o = order[k] * I + f;
-    if (patch_out) patch_out[o] = make_float2((float)p.re, (float)p.im);
-    if (res_out) {
-        float pr = 0.0f;
-        if (nopt[k] == 1.0) {
-            const cplx<R> v = psdph_value<R>(fv, phnm[gid]);
-            const R b = rowinv ? bp[f] : bp[gid];
-            const cplx<R> r = rotate_neg<R>(v, (R)K[k] * b);
-            pr = (float)RT<R>::atan2_(r.im, r.re);
+    __shared__ float2 tp[FINAL_TP][FINAL_TF + 1];
+    __shared__ float tr[FINAL_TP][FINAL_TF + 1];
+    const int64_t k0 = (int64_t)blockIdx.x * FINAL_TP;
+    const int f0 = blockIdx.y * FINAL_TF;
+    const int np = (int)min<int64_t>(FINAL_TP, P - k0), nf = min(FINAL_TF, I - f0);
+    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
+        const int kl = e % FINAL_TP, fl = e / FINAL_TP;
+        if (kl < np && fl < nf) {
+            const int64_t k = k0 + kl;
+            const int f = f0 + fl;
+            const int64_t g = (int64_t)f * P + k;
+            const cplx<R> fv = filt[(int64_t)f * C + cell_of[k]];
+            tp[kl][fl] = make_float2((float)fv.re, (float)fv.im);
+            float pr = 0.0f;
+            if (res_out && nopt[k] == 1.0) {
+                const cplx<R> v = psdph_value<R>(fv, phnm[g]);
+                const R b = rowinv ? bp[f] : bp[g];
+                const cplx<R> r = rotate_neg<R>(v, (R)K[k] * b);
+                pr = (float)RT<R>::atan2_(r.im, r.re);
+            }
+            tr[kl][fl] = pr;
         }
-        res_out[o] = pr;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
+        const int kl = e / FINAL_TF, fl = e % FINAL_TF;
+        if (kl < np && fl < nf) {
+            const int64_t o = order[k0 + kl] * I + f0 + fl;
+            if (patch_out) patch_out[o] = tp[kl][fl];
+            if (res_out) res_out[o] = tr[kl][fl];
+        }
     }
 }
 

@@ -48,6 +48,11 @@ static int arg_error(const std::string &msg) {
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// launch grid of the row-order output kernels (k_final_*)
+static inline dim3 final_grid(int64_t P, int64_t I) {
+    return dim3(blocks_for(P, FINAL_TP), blocks_for(I, FINAL_TF));
+}
+
 // Device buffer that frees itself.
 struct DBuf {
     void *p = nullptr;
@@ -842,7 +847,7 @@ int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
     if (out->ph_patch || out->ph_res) {
         float2 *pbuf = s->dl_bulk.as<float2>();
         float *rbuf = reinterpret_cast<float *>(pbuf + plane);
-        hipLaunchKernelGGL(k_final_patch_res<R>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+        hipLaunchKernelGGL(k_final_patch_res<R>, final_grid(P, I), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
                            s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
                            s->d_order.as<int64_t>(), out->ph_patch ? pbuf : nullptr,
@@ -855,7 +860,7 @@ int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
     if (!loop_outputs) return PS2_OK;
     if ((rc = download_row_vec(s, s->cohprev, out->coh_ps_save))) return rc;
     if (out->ph_weight) {
-        hipLaunchKernelGGL(k_final_weight<R>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+        hipLaunchKernelGGL(k_final_weight<R>, final_grid(P, I), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->Kprev.as<double>(),
                            s->Wprev.as<double>(), s->d_order.as<int64_t>(), s->dl_bulk.as<float2>(), P, (int)I);
         PS2_LAUNCHED();
@@ -1014,13 +1019,13 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out) {
         PS2_LAUNCHED();
     }
     if (s->form == PS2_FORM_FP64)
-        hipLaunchKernelGGL(k_final_patch_res<double>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+        hipLaunchKernelGGL(k_final_patch_res<double>, final_grid(P, I), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<double>>(), s->bp.as<double>(), s->filt.as<cplx<double>>(),
                            s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
                            s->d_order.as<int64_t>(), s->ex_patch.as<float2>(), s->ex_res.as<float>(), P,
                            (int)I, s->rowinv, s->C);
     else
-        hipLaunchKernelGGL(k_final_patch_res<float>, dim3(blocks_for(P * I, 256)), dim3(256), 0, s->stream,
+        hipLaunchKernelGGL(k_final_patch_res<float>, final_grid(P, I), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<float>>(), s->bp.as<float>(), s->filt.as<cplx<float>>(),
                            s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
                            s->d_order.as<int64_t>(), s->ex_patch.as<float2>(), s->ex_res.as<float>(), P,
