@@ -349,14 +349,16 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         const uint32_t below511 = team_pick_u32(run, sl, sh, hi_team) - n511;
         const uint32_t b511 = (uint32_t)(owner * 32 + qs);
         if (n511 > 64u) return team_median_bits<R>(keys, hi_team);
+        // bucket b511 holds the keys lo <= k < lo + width
+        const uint32_t lo = kmin + (b511 << shift), width = 1u << shift;
         // compact the keys of bucket b511 into cand, in team order
         uint32_t mine = 0;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) mine += ((keys[r] - kmin) >> shift) == b511 ? 1u : 0u;
+        for (int r = 0; r < 32; ++r) mine += keys[r] - lo < width ? 1u : 0u;
         uint32_t off = team_incl_sum_u32(mine) - mine;
 #pragma unroll
         for (int r = 0; r < 32; ++r)
-            if (((keys[r] - kmin) >> shift) == b511) cand[off++] = keys[r];
+            if (keys[r] - lo < width) cand[off++] = keys[r];
         wave_sync();
         // rank of the candidates q = lane and q = lane + 32 within the bucket
         // (ties by position), eight reads in flight per step
@@ -386,9 +388,10 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         uint32_t k512 = team_min_key<uint32_t>(v1);
         if (j0 + 1u >= n511) {
             uint32_t mn = 0xffffffffu;
+            const uint32_t hi = lo + width;
 #pragma unroll
             for (int r = 0; r < 32; ++r)
-                if (((keys[r] - kmin) >> shift) > b511 && keys[r] < mn) mn = keys[r];
+                if (keys[r] >= hi && keys[r] < mn) mn = keys[r];
             k512 = team_min_key<uint32_t>(mn);
         }
         return (__uint_as_float(k511) + __uint_as_float(k512)) / 2.0f;
@@ -542,14 +545,23 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         const R med = team_median_bits<R>(hk, hi_team);
 #endif
         const R scl = RT<R>::gain_scale(med);
+        // low-pass column `lane` (the table is stored transposed): 4 values per load
+        typedef R lp4 __attribute__((ext_vector_type(4)));
+        const __attribute__((address_space(1))) lp4 *lpc =
+            (const __attribute__((address_space(1))) lp4 *)(lpw + lane * 32);
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            R v = ALPHA_ONE ? RT<R>::gain_rel_minus_one(RT<R>::from_key(hk[r]), med, scl)
-                            : RT<R>::pow_(RT<R>::gain_div(RT<R>::from_key(hk[r]), med, scl), prm.alpha) - (R)1;
-            if (v < (R)0) v = (R)0;
-            const R gsc = fma(v, prm.beta, lpw[r * 32 + lane]);
-            xr[r] *= gsc;
-            xi[r] *= gsc;
+        for (int q = 0; q < 8; ++q) {
+            const lp4 l4 = lpc[q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = 4 * q + u;
+                R v = ALPHA_ONE ? RT<R>::gain_rel_minus_one(RT<R>::from_key(hk[r]), med, scl)
+                                : RT<R>::pow_(RT<R>::gain_div(RT<R>::from_key(hk[r]), med, scl), prm.alpha) - (R)1;
+                if (v < (R)0) v = (R)0;
+                const R gsc = fma(v, prm.beta, l4[u]);
+                xr[r] *= gsc;
+                xi[r] *= gsc;
+            }
         }
         // inverse along columns; only the window's 24 output rows go on
         fft32<R, true, 32, CLAP_WIN>(xr, xi);
@@ -660,15 +672,32 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
         const int prev_last = s1 * wseg;   // first window of s1 = one past s1-1's last
         if (c < CLAP_INC * prev_last + (CLAP_WIN - CLAP_INC)) s0 = s1 - 1;
     }
-    R sre = (R)0, sim = (R)0;
-    for (int ix = ix_lo; ix <= ix_hi; ++ix) {
+    // at most 4 window rows (ix_hi - ix_lo <= 3) x 2 segments: every load is
+    // issued before the first add, then the sum runs in the fixed order
+    // (ix ascending, segment ascending)
+    cplx<R> v[4][2];
+    bool ok[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int ix = ix_lo + q;
         const int lr = r - ix * CLAP_INC;
-        for (int sg = s0; sg <= s1; ++sg) {
-            const cplx<R> v = part[((((int64_t)f * nwi + ix) * 2 + (sg & 1)) * CLAP_WIN + lr) * nj + c];
-            sre += v.re;
-            sim += v.im;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int sg = s0 + t;
+            ok[q][t] = ix <= ix_hi && sg <= s1;
+            v[q][t] = ok[q][t] ? part[((((int64_t)f * nwi + ix) * 2 + (sg & 1)) * CLAP_WIN + lr) * nj + c]
+                               : cplx<R>{(R)0, (R)0};
         }
     }
+    R sre = (R)0, sim = (R)0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+            if (ok[q][t]) {
+                sre += v[q][t].re;
+                sim += v[q][t].im;
+            }
     filt[gid] = UNIT ? unit_phase<R>({sre, sim}) : cplx<R>{sre, sim};
 }
 

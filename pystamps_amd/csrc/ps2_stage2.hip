@@ -399,8 +399,11 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
         PS2_HIP(hipMemcpy(s->W0.p, w0.data(), sizeof(double) * P, hipMemcpyHostToDevice));
     }
     {   // CLAP constants: low_pass, window weight x 1/n^2
+        // stored transposed: column c's 32 values are contiguous (the CLAP
+        // kernel's lane c reads them with vector loads)
         std::vector<R> lp(CLAP_FFT * CLAP_FFT);
-        for (int q = 0; q < CLAP_FFT * CLAP_FFT; ++q) lp[q] = (R)in->low_pass[q];
+        for (int r = 0; r < CLAP_FFT; ++r)
+            for (int c = 0; c < CLAP_FFT; ++c) lp[c * CLAP_FFT + r] = (R)in->low_pass[r * CLAP_FFT + c];
         if ((rc = s->lp.alloc(sizeof(R) * lp.size()))) return rc;
         PS2_HIP(hipMemcpy(s->lp.p, lp.data(), sizeof(R) * lp.size(), hipMemcpyHostToDevice));
     }
