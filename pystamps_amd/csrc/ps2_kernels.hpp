@@ -117,28 +117,56 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
     const bool mine = c < C && f < I;
     const int32_t my0 = mine ? cell_start[c] : 0, my1 = mine ? cell_start[c + 1] : 0;
     R sre = (R)0, sim = (R)0;
+    constexpr int NJ = CHUNK / 256;   // pixels per thread and chunk
     for (int32_t base = ks; base < ke; base += CHUNK) {
         const int n = ke - base < CHUNK ? ke - base : CHUNK;
-        for (int j = t; j < n; j += 256) {
-            const int64_t k = base + j;
-            const double kk = K[k], ww = W[k];
+        // every load of the thread's NJ x FG elements is issued before the
+        // first weight is computed
+        double kk[NJ], ww[NJ];
+        cplx<R> sv[NJ][FG];
+        R bv[NJ][FG];
 #pragma unroll
-            for (int q = 0; q < FG; ++q) {
-                const int ff = f0 + q;
-                if (ff < I) {
-                    const int64_t e = (int64_t)ff * P + k;
-                    const R b = bp_rowinv ? bp[ff] : bp[e];
-                    vals[q][j] = weight_value<R>(phnm[e], b, kk, ww);
+        for (int u = 0; u < NJ; ++u) {
+            const int j = t + u * 256;
+            if (j < n) {
+                const int64_t k = base + j;
+                kk[u] = K[k];
+                ww[u] = W[k];
+#pragma unroll
+                for (int q = 0; q < FG; ++q) {
+                    const int ff = f0 + q;
+                    if (ff < I) {
+                        const int64_t e = (int64_t)ff * P + k;
+                        bv[u][q] = bp_rowinv ? bp[ff] : bp[e];
+                        sv[u][q] = phnm[e];
+                    }
                 }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NJ; ++u) {
+            const int j = t + u * 256;
+            if (j < n) {
+#pragma unroll
+                for (int q = 0; q < FG; ++q)
+                    if (f0 + q < I) vals[q][j] = weight_value<R>(sv[u][q], bv[u][q], kk[u], ww[u]);
             }
         }
         __syncthreads();
         const int32_t lo = my0 > base ? my0 : base;
         const int32_t hi = my1 < base + n ? my1 : base + n;
-        for (int32_t k = lo; k < hi; ++k) {
-            const cplx<R> v = vals[fs][k - base];
-            sre += v.re;
-            sim += v.im;
+        // ascending order; eight LDS reads in flight per step
+        for (int32_t k = lo; k < hi; k += 8) {
+            cplx<R> v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < hi) v[u] = vals[fs][k + u - base];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < hi) {
+                    sre += v[u].re;
+                    sim += v[u].im;
+                }
         }
         __syncthreads();
     }
