@@ -322,11 +322,13 @@ __device__ inline R smooth7(const R (&h)[32], int c, const R (&k)[7]) {
 // LDS atomics in the team's tile, then exact selection among the few keys of
 // the bucket holding rank 511 (rank 512 is in the same bucket or is the
 // smallest key above it).  A crowded bucket falls back to the bitwise search.
-// Bucket b lives at word (b & 31) * 33 + (b >> 5): lane L owns buckets
-// [32L, 32L + 32), and both the per-lane totals (lane L reads q*33 + L) and
-// the owner's bucket row (lane q reads q*33 + owner) hit 32 distinct banks.
-// hist (>= 1056 words) and cand (>= 64 words, distinct from hist) are the
-// team's LDS scratch.
+// Bucket b = 32 L + q lives at word 36 L + q: lane L owns buckets
+// [32L, 32L + 32), a 16-byte aligned run that it reads with 8 four-word loads
+// for its total (the 36-word rows put 16 lanes' runs on 64 distinct banks),
+// and the owner's bucket row (lane q reads 36 owner + q) is contiguous.  The
+// region is cleared with four-word stores.  hist (>= 1152 words, 16-byte
+// aligned) and cand (>= 64 words, distinct from hist) are the team's LDS
+// scratch.
 template <typename R>
 __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi_team, R *hist_mem,
                                 R *cand_mem) {
@@ -348,19 +350,24 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         const uint32_t span = kmax - kmin;
         const int bits = 32 - __clz((int)span);
         const int shift = bits > 10 ? bits - 10 : 0;
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        u4 *hist4 = reinterpret_cast<u4 *>(hist);
 #pragma unroll
-        for (int q = 0; q < 32; ++q) hist[q * 33 + lane] = 0u;
+        for (int q = 0; q < 9; ++q) hist4[q * 32 + lane] = u4{0u, 0u, 0u, 0u};
         wave_sync();
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             const uint32_t bk = (keys[r] - kmin) >> shift;
-            atomicAdd(&hist[(bk & 31u) * 33u + (bk >> 5)], 1u);
+            atomicAdd(&hist[bk + (bk >> 5) * 4u], 1u);
         }
         wave_sync();
         // per-lane totals, team inclusive scan, owner of rank 511
         uint32_t tot = 0;
 #pragma unroll
-        for (int q = 0; q < 32; ++q) tot += hist[q * 33 + lane];
+        for (int q = 0; q < 8; ++q) {
+            const u4 h = hist4[lane * 9 + q];
+            tot += (h.x + h.y) + (h.z + h.w);
+        }
         const uint32_t incl = team_incl_sum_u32(tot);
         const uint32_t excl = incl - tot;
         int ol, oh;
@@ -368,7 +375,7 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
         const int owner = (hi_team ? oh : ol) & 31;
         const uint32_t base = team_pick_u32(excl, ol, oh, hi_team);
         // the owner's 32 buckets, one per lane, scanned the same way
-        const uint32_t c = hist[lane * 33 + owner];
+        const uint32_t c = hist[owner * 36 + lane];
         const uint32_t run = team_incl_sum_u32(c) + base;
         int sl, sh;
         team_first_lanes(__ballot(run > 511u), sl, sh);
@@ -457,11 +464,13 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         i1 = ni - CLAP_WIN;
     }
     // separate re / im planes; a row of 33 keeps row and column walks on
-    // distinct banks, and each team's plane starts half the banks apart
-    __shared__ R tbr_s[TEAMS][32][33];
-    __shared__ R tbi_s[TEAMS][32][33];
-    R (&tbr)[32][33] = tbr_s[team];
-    R (&tbi)[32][33] = tbi_s[team];
+    // distinct banks.  One array per team: re plane, im plane, then 32 words
+    // of padding so that the other team's planes start half the banks apart;
+    // 16-byte aligned for the median's four-word histogram access.
+    constexpr int TILE = 2 * 32 * 33 + 128 / (int)sizeof(R);
+    __shared__ __attribute__((aligned(16))) R tile_s[TEAMS][TILE];
+    R (&tbr)[32][33] = *reinterpret_cast<R (*)[32][33]>(&tile_s[team][0]);
+    R (&tbi)[32][33] = *reinterpret_cast<R (*)[32][33]>(&tile_s[team][32 * 33]);
     const int cS = (lane + 16) & 31;  // fftshift of this lane's column
     // rolling accumulator of output row lr = lane - rs: slot e holds column
     // 6*jx + e of the current window jx (native_stage2.rs:904-915)
@@ -568,7 +577,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         }
         wave_sync();  // the tile is scratch for the median from here to the gain
 #if PS2_CLAP_MEDIAN_HIST
-        const R med = team_median<R>(hk, hi_team, &tbr[0][0], &tbi[0][0]);
+        const R med = team_median<R>(hk, hi_team, &tile_s[team][0], &tile_s[team][1152]);
 #else
         const R med = team_median_bits<R>(hk, hi_team);
 #endif
