@@ -236,27 +236,13 @@ done:
 
 constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
 
-// The uniform draws depend only on (seed, rows, columns), not on the patch:
-// the last stream is kept for the next call (patches of one run share their
-// interferogram count).  Returns a copy-free view valid under the lock.
-std::mutex g_draw_mu;
-struct DrawCache {
-    uint32_t seed = 0;
-    int64_t rows = -1, cols = -1;
-    std::vector<double> v;  // column-major [rows][cols], x 2 pi
-} g_draws;
-
-const std::vector<double> &uniform_draws(uint32_t seed, int64_t rows, int64_t cols) {
-    if (g_draws.seed != seed || g_draws.rows != rows || g_draws.cols != cols) {
-        const int64_t total = rows * cols;
-        g_draws.v.assign((size_t)(total > 0 ? total : 1), 0.0);
-        MatlabV5Uniform rng(seed);
-        for (int64_t q = 0; q < total; ++q) g_draws.v[q] = rng.next() * kTwoPi;
-        g_draws.seed = seed;
-        g_draws.rows = rows;
-        g_draws.cols = cols;
-    }
-    return g_draws.v;
+// rows x cols uniform draws of the MATLAB v5 stream, x 2 pi, in draw order
+std::vector<double> uniform_draws(uint32_t seed, int64_t rows, int64_t cols) {
+    const int64_t total = rows * cols;
+    std::vector<double> v((size_t)(total > 0 ? total : 1), 0.0);
+    MatlabV5Uniform rng(seed);
+    for (int64_t q = 0; q < total; ++q) v[q] = rng.next() * kTwoPi;
+    return v;
 }
 
 }  // namespace
@@ -268,8 +254,7 @@ extern "C" int ps2_hip_random_coherence_histogram(const double *bperp_nm, int64_
                                                   double *out_nr_max_nz_ix) {
     int rc = nr_check(bperp_nm, n_ifg, trial_values, n_trial, coh_bins, n_rows, out_nr, out_nr_max_nz_ix);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(g_draw_mu);
-    const std::vector<double> &ph = uniform_draws(seed, n_rows, n_ifg);
+    const std::vector<double> ph = uniform_draws(seed, n_rows, n_ifg);
     return nr_histogram(ph, bperp_nm, n_ifg, n_trial, coh_bins, n_rows, device, out_nr, out_nr_max_nz_ix);
 }
 
@@ -291,8 +276,7 @@ extern "C" int ps2_hip_random_coherence_histogram_sb(const double *bperp_nm, int
                       std::to_string(i));
             return PS2_E_ARG;
         }
-    std::lock_guard<std::mutex> lk(g_draw_mu);
-    const std::vector<double> &img = uniform_draws(seed, n_rows, n_image);
+    const std::vector<double> img = uniform_draws(seed, n_rows, n_image);
     const int64_t total = n_rows * n_ifg;
     std::vector<double> ph((size_t)(total > 0 ? total : 1));
     for (int64_t i = 0; i < n_ifg; ++i)

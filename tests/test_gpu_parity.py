@@ -73,10 +73,9 @@ def test_gpu_nr_matches_oracle(ora, prep_c1):
     assert mx_g == mx_o
 
 
-def test_gpu_nr_draw_reuse_across_shapes(ora):
-    """The library keeps the last uniform-draw stream: calls that alternate row
-    counts, interferogram counts and the small-baseline image count must each
-    equal the oracle's fresh draw."""
+def test_gpu_nr_repeated_calls_across_shapes(ora):
+    """Repeated calls that alternate row counts, interferogram counts and the
+    small-baseline image count each equal the oracle's fresh draw."""
     from pystamps_amd.prepare import coh_bins, nr_hip, nr_hip_sb, trial_values
     tv, cb = trial_values(0.6), coh_bins()
     bp_a, bp_b = np.linspace(-150.0, 170.0, 9), np.linspace(-90.0, 260.0, 14)
