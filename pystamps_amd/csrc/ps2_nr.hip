@@ -10,7 +10,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
-#include <mutex>
 #include <string>
 #include <vector>
 
