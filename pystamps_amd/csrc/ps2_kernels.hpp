@@ -53,6 +53,8 @@ __device__ inline cplx<R> weight_value(cplx<R> s, R b, double K, double W) {
 // fill_phase_weight + accumulate_grid (native_stage2.rs:629-657, 722-740):
 // one thread per (ifg, cell); the cell's pixels are summed in ascending row
 // order, which is the reference's sequential order, so no atomics are needed.
+// With Kprev/Wprev non-null, the K and weighting this fill used are kept
+// there (for the ph_weight output of the last fill).
 template <typename R>
 __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ phnm,
                                                    const R *__restrict__ bp, int bp_rowinv,
@@ -60,7 +62,8 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
                                                    const double *__restrict__ W,
                                                    const int32_t *__restrict__ cell_start,
                                                    cplx<R> *__restrict__ grid, int64_t C, int I,
-                                                   int64_t P,
+                                                   int64_t P, double *__restrict__ Kprev,
+                                                   double *__restrict__ Wprev,
     const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,11 +71,17 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
     const int f = (int)(gid / C);
     const int64_t c = gid - (int64_t)f * C;
     const int32_t k0 = cell_start[c], k1 = cell_start[c + 1];
+    const bool keep = Kprev && f == 0;
     R sre = (R)0, sim = (R)0;
     for (int32_t k = k0; k < k1; ++k) {
         const int64_t e = (int64_t)f * P + k;
         const R b = bp_rowinv ? bp[f] : bp[e];
-        const cplx<R> v = weight_value<R>(phnm[e], b, K[k], W[k]);
+        const double kk = K[k], ww = W[k];
+        if (keep) {
+            Kprev[k] = kk;
+            Wprev[k] = ww;
+        }
+        const cplx<R> v = weight_value<R>(phnm[e], b, kk, ww);
         sre += v.re;
         sim += v.im;
     }
@@ -85,7 +94,8 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
 // are computed with coalesced loads into LDS, CHUNK pixels per plane at a
 // time, then each thread adds its cell's weights in ascending row order, so
 // the sums are the same as k_fill_grid's.  CHUNK is sized to the mean pixel
-// count of CB cells: LDS per workgroup bounds occupancy.
+// count of CB cells: LDS per workgroup bounds occupancy.  Kprev/Wprev as in
+// k_fill_grid.
 template <typename R, int CB, int CHUNK>
 __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restrict__ phnm,
                                                           const R *__restrict__ bp, int bp_rowinv,
@@ -94,6 +104,8 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
                                                           const int32_t *__restrict__ cell_start,
                                                           cplx<R> *__restrict__ grid, int64_t C,
                                                           int I, int64_t P, int npg,
+                                                          double *__restrict__ Kprev,
+                                                          double *__restrict__ Wprev,
     const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
     constexpr int FG = 256 / CB;
@@ -132,6 +144,10 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
                 const int64_t k = base + j;
                 kk[u] = K[k];
                 ww[u] = W[k];
+                if (Kprev && pg == 0) {   // each pixel once: its cell group's first plane group
+                    Kprev[k] = kk[u];
+                    Wprev[k] = ww[u];
+                }
 #pragma unroll
                 for (int q = 0; q < FG; ++q) {
                     const int ff = f0 + q;
@@ -1461,6 +1477,7 @@ __global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__
 // later kernels of the iteration and of further iterations return at entry
 // once ctl->stopped is set.
 constexpr int CTL_BINS = 100;
+constexpr int CTL_HIST_WORDS = 128;   // coherence histogram buffer (CTL_BINS used)
 constexpr int CTL_HI = 1001;      // interp([1, prand], 10) keeps 1010 - 9 samples
 constexpr int CTL_UP = 1049;      // 101 * 10 + 39-sample delay
 
@@ -1519,8 +1536,8 @@ __constant__ double kCtlInterpTaps[79] = {
 constexpr int CTL_THREADS = 1024;
 __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ ctl,
                                                         const double *__restrict__ rms_part,
-                                                        const uint32_t *__restrict__ hist,
-                                                        const uint32_t *__restrict__ rescore_count,
+                                                        uint32_t *__restrict__ hist,
+                                                        uint32_t *__restrict__ rescore_count,
                                                         double *__restrict__ prand_hi,
                                                         CtlParams cp) {
     __shared__ double red[CTL_THREADS];
@@ -1552,7 +1569,14 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
         if (t == 0) ctl->stopped = 1;
         return;
     }
+    // the next iteration's histogram and re-score counter start at zero
+    // (read above and below before they are cleared)
+    auto clear_counters = [&]() {
+        if (t < CTL_HIST_WORDS) hist[t] = 0u;
+        if (t < 4 && rescore_count) rescore_count[t] = 0u;
+    };
     if (!cp.psquare) {
+        clear_counters();
         if (t == 0) ctl->i_loop += 1;
         return;
     }
@@ -1563,6 +1587,7 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
         nr[t] = ctl->nr[t];
     }
     __syncthreads();
+    clear_counters();
     if (t == 0) {
         double denom = 0.0, num = 0.0;
         for (int b = 0; b < lct; ++b) denom += nr[b];
@@ -1610,20 +1635,6 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
         prand_hi[q] = sacc * 10.0;
     }
     if (t == 0) ctl->i_loop += 1;
-}
-
-// K and weighting of this iteration's fill, kept for the ph_weight output
-__global__ __launch_bounds__(256) void k_copy_state(const double *__restrict__ K,
-                                                    const double *__restrict__ W,
-                                                    double *__restrict__ Kprev,
-                                                    double *__restrict__ Wprev, int64_t P,
-                                                    const int32_t *__restrict__ gate) {
-    if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < P) {
-        Kprev[k] = K[k];
-        Wprev[k] = W[k];
-    }
 }
 
 }  // namespace ps2

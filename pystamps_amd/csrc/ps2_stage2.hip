@@ -499,7 +499,8 @@ void collect_timers(ps2_session *s) {
 #endif
 
 template <typename R>
-int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
+int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *Kprev = nullptr,
+                     double *Wprev = nullptr) {
     const int64_t n = s->C * s->I;
     SlotTimer tm(s, SLOT_FILL);
     if (sizeof(R) == 4) {
@@ -516,7 +517,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
                                s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
-                               (int)s->I, s->P, npg, s->gate);
+                               (int)s->I, s->P, npg, Kprev, Wprev, s->gate);
         };
         if (mean <= 256.0)
             go(k_fill_grid_staged<R, CB, 256>);
@@ -528,7 +529,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W) {
         hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
                            s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P,
-                           s->gate);
+                           Kprev, Wprev, s->gate);
     }
     PS2_LAUNCHED();
     return PS2_OK;
@@ -623,10 +624,10 @@ int scan_chunk(const ps2_session *s) {
 }
 
 template <typename R>
-int launch_fit(ps2_session *s) {
+int launch_fit(ps2_session *s, bool clear_count = true) {
     const FitParams fp = fit_params(s);
     const unsigned nb = blocks_for(s->P, 256);
-    if (sizeof(R) == 4) {
+    if (sizeof(R) == 4 && clear_count) {   // (inside the loop k_control clears it)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4,
                            s->gate);
         PS2_LAUNCHED();
@@ -674,15 +675,18 @@ int launch_fit(ps2_session *s) {
     return PS2_OK;
 }
 
-int launch_rms_hist(ps2_session *s) {
+int launch_rms_hist(ps2_session *s, bool clear_hist = true) {
     HistParams hp;
     hp.P = s->P;
     hp.equal = s->hist_rule.equal ? 1 : 0;
     hp.cutoff0 = s->hist_rule.cutoff0;
     hp.d = s->hist_rule.d;
     hp.nbins = PS2_COH_BINS;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s->stream, s->hist.as<uint32_t>(), 128, s->gate);
-    PS2_LAUNCHED();
+    if (clear_hist) {   // (inside the loop k_control clears it)
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s->stream, s->hist.as<uint32_t>(),
+                           CTL_HIST_WORDS, s->gate);
+        PS2_LAUNCHED();
+    }
     SlotTimer tm(s, SLOT_RMS);
     hipLaunchKernelGGL(k_rms_hist, dim3(s->n_rms_blocks), dim3(RMS_BLOCK), 0, s->stream,
                        s->coh.as<double>(), s->cohprev.as<double>(), s->rms_part.as<double>(),
@@ -698,18 +702,17 @@ template <typename R>
 int launch_iteration(ps2_session *s) {
     const int64_t P = s->P;
     int rc;
-    // K and weighting used by this fill (for the ph_weight output)
-    hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream, s->K.as<double>(),
-                       s->W.as<double>(), s->Kprev.as<double>(), s->Wprev.as<double>(), P, s->gate);
-    PS2_LAUNCHED();
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
-    if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>()))) return rc;
+    // the fill also keeps the K and weighting it used (for the ph_weight output)
+    if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>(), s->Kprev.as<double>(),
+                                  s->Wprev.as<double>())))
+        return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
     if ((rc = launch_clap<R>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
-    if ((rc = launch_fit<R>(s))) return rc;
+    if ((rc = launch_fit<R>(s, false))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
-    if ((rc = launch_rms_hist(s))) return rc;
+    if ((rc = launch_rms_hist(s, false))) return rc;
     // gamma-change RMS, stop test, Nr scaling and the P-square table
     hipLaunchKernelGGL(k_control, dim3(1), dim3(CTL_THREADS), 0, s->stream, s->ctl.as<LoopCtl>(),
                        s->rms_part.as<double>(), s->hist.as<uint32_t>(),
@@ -744,6 +747,10 @@ int run_loop(ps2_session *s) {
     PS2_HIP(hipEventRecord(s->ev[0], s->stream));
     PS2_HIP(hipMemsetAsync(s->K.p, 0, sizeof(double) * P, s->stream));
     PS2_HIP(hipMemsetAsync(s->cohprev.p, 0, sizeof(double) * P, s->stream));
+    // the first iteration's histogram and re-score counter (k_control clears
+    // them for every later one)
+    PS2_HIP(hipMemsetAsync(s->hist.p, 0, sizeof(uint32_t) * CTL_HIST_WORDS, s->stream));
+    PS2_HIP(hipMemsetAsync(s->rescore_count.p, 0, sizeof(uint32_t) * 4, s->stream));
     PS2_HIP(hipMemcpyAsync(s->W.p, s->W0.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
     std::memset(hc, 0, sizeof(LoopCtl));
     hc->i_loop = 1;
