@@ -65,14 +65,12 @@ def kernel_work(prep, name):
         return 143_360.0 * W * I, 8.0 * C * I + part
     if name == "scan":
         return 8.0 * P * I * T, 16.0 * P * I + 8.0 * C * I
-    if name == "refine":
-        return 2 * 20.0 * P * I, 2 * 12.0 * P * I + 8.0 * C * I
+    if name == "refine":   # (+ the fused RMS/histogram step: coh_prev read and written)
+        return 2 * 20.0 * P * I, 2 * 12.0 * P * I + 8.0 * C * I + 16.0 * P
     if name == "fill_grid":
         return 16.0 * P * I, 12.0 * P * I + 8.0 * C * I
     if name == "clap_combine":
         return 0.0, part + 8.0 * C * I
-    if name == "rms_hist":
-        return 0.0, 16.0 * P
     if name == "weights":
         return 0.0, 24.0 * P
     return 0.0, 0.0
@@ -93,7 +91,7 @@ def aggregate_roofline(prep, kt, peak_tflops):
 
 KERNEL_SYMBOLS = {"clap_strip": "k_clap_strip", "scan": "k_scan", "refine": "k_refine",
                   "fill_grid": "k_fill_grid", "clap_combine": "k_clap_combine",
-                  "rescore": "k_rescore", "rms_hist": "k_rms_hist", "weights": "k_weights"}
+                  "rescore": "k_rescore", "weights": "k_weights"}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage2_fp32_500k_60ifg_pmc_traffic.md")

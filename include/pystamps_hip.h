@@ -239,7 +239,9 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
  * rescore, refine, rms_hist, weights.  ms[k] is the summed HIP-event time of
- * slot k over launches[k] launches.  Timing events are recorded only when
+ * slot k over launches[k] launches.  The gamma-change RMS and coherence
+ * histogram run inside the refine kernel, so slot rms_hist records no
+ * launches (it is kept for the ABI).  Timing events are recorded only when
  * ps2_hip_session_set_timing(s, 1) was called before the runs; such runs
  * launch the iterations one by one (no graph), with the same results. */
 #define PS2_NUM_KERNEL_SLOTS 8

@@ -1211,31 +1211,46 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
     }
 }
 
-// refine_candidate (native_stage2.rs:1259-1303) from the selected trial:
-// offset sum, f64 least-squares step mopt, final residual sum.
+// Coherence histogram rule (hist_with_centers, native_stage2.rs:1538-1622):
+// the bin of one value, equal-spacing fast path or the bin mid-points.
+struct HistParams {
+    int64_t P;
+    int equal;       // equal-spacing rule
+    double cutoff0, d;
+    int nbins;
+};
+
+__device__ inline int hist_bin(double v, const double *__restrict__ mids, HistParams hp) {
+    if (!__builtin_isfinite(v)) return -1;
+    if (hp.equal) {
+        if (hp.d == 0.0) return 0;
+        double a = ceil((v - hp.cutoff0) / hp.d);
+        if (a < 0.0) a = 0.0;
+        if (a > (double)(hp.nbins - 1)) a = (double)(hp.nbins - 1);
+        return (int)a;
+    }
+    int b = 0;
+    while (b < hp.nbins - 1 && mids[b] < v) ++b;
+    return b;
+}
+
+// refine_candidate (native_stage2.rs:1259-1303) of row k from the selected
+// trial: offset sum, f64 least-squares step mopt, final residual sum.  Writes
+// K, C, coh and N_opt of the row and returns its coh.
 template <typename R>
-__global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm,
-                                                const R *__restrict__ bp,
-                                                const cplx<R> *__restrict__ filt,
-                                                const int32_t *__restrict__ cell_of,
-                                                const int32_t *__restrict__ tix,
-                                                const R *__restrict__ rng,
-                                                const cplx<R> *__restrict__ offset,
-                                                double *__restrict__ Kout,
-                                                double *__restrict__ Cout,
-                                                double *__restrict__ cohout,
-                                                double *__restrict__ nopt, FitParams prm,
-    const int32_t *__restrict__ gate) {
-    if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= prm.P) return;
+__device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm, const R *__restrict__ bp,
+                                    const cplx<R> *__restrict__ filt, const int32_t *__restrict__ cell_of,
+                                    const int32_t *__restrict__ tix, const R *__restrict__ rng,
+                                    const cplx<R> *__restrict__ offset, double *__restrict__ Kout,
+                                    double *__restrict__ Cout, double *__restrict__ cohout,
+                                    double *__restrict__ nopt, const FitParams &prm) {
     const int t = tix[k];
     if (t < 0) {
         Kout[k] = __builtin_nan("");
         Cout[k] = 0.0;
         cohout[k] = 0.0;
         nopt[k] = 0.0;
-        return;
+        return 0.0;
     }
     const int I = prm.I;
     const int64_t P = prm.P, C = prm.C;
@@ -1290,8 +1305,66 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     if (dn == (R)0) dn = (R)1;
     Kout[k] = (double)k0;
     Cout[k] = (double)RT<R>::atan2_(mim, mre);
-    cohout[k] = (double)(RT<R>::abs_(mre, mim) / dn);
+    const double coh = (double)(RT<R>::abs_(mre, mim) / dn);
+    cohout[k] = coh;
     nopt[k] = 1.0;
+    return coh;
+}
+
+
+// The loop's gamma-change and histogram step (native_stage2.rs:2045-2057 and
+// hist_with_centers :1538-1622) over the coherences just computed, fused into
+// the refine kernel: per workgroup of 256 rows the sum of (coh - coh_prev)^2
+// in a fixed tree (one partial per workgroup, summed in a fixed order by
+// k_control), coh_prev <- coh, and the bin counts (integer, order-free).
+struct RmsHistOut {
+    double *coh_prev;   // NULL: no RMS/histogram step (single-step use)
+    double *part;       // [gridDim.x]
+    uint32_t *hist;
+    const double *mids;
+    HistParams hp;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm,
+                                                const R *__restrict__ bp,
+                                                const cplx<R> *__restrict__ filt,
+                                                const int32_t *__restrict__ cell_of,
+                                                const int32_t *__restrict__ tix,
+                                                const R *__restrict__ rng,
+                                                const cplx<R> *__restrict__ offset,
+                                                double *__restrict__ Kout,
+                                                double *__restrict__ Cout,
+                                                double *__restrict__ cohout,
+                                                double *__restrict__ nopt, FitParams prm, RmsHistOut ro,
+    const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool row = k < prm.P;
+    double coh = 0.0;
+    if (row) coh = refine_row<R>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm);
+    if (!ro.coh_prev) return;
+    __shared__ double red[256];
+    __shared__ uint32_t lh[128];
+    if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
+    __syncthreads();
+    double d2 = 0.0;
+    if (row) {
+        const double d = coh - ro.coh_prev[k];
+        d2 = d * d;
+        ro.coh_prev[k] = coh;
+        const int bin = hist_bin(coh, ro.mids, ro.hp);
+        if (bin >= 0) atomicAdd(&lh[bin], 1u);
+    }
+    red[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ro.part[blockIdx.x] = red[0];
+    for (int q = threadIdx.x; q < ro.hp.nbins; q += 256)
+        if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
 }
 
 // ph_patch (unit filt) for every row and ph_res = arg(residual) of valid rows,
@@ -1339,70 +1412,6 @@ __global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restri
             if (res_out) res_out[o] = tr[kl][fl];
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// Gamma-change RMS partial sums (native_stage2.rs:2045-2057) and the coh
-// histogram (hist_with_centers, :1538-1622); coh_prev <- coh (:310).
-struct HistParams {
-    int64_t P;
-    int equal;       // equal-spacing rule
-    double cutoff0, d;
-    int nbins;
-};
-
-constexpr int RMS_BLOCK = 256;
-constexpr int RMS_PER_THREAD = 8;
-
-__device__ inline int hist_bin(double v, const double *__restrict__ mids, HistParams hp) {
-    if (!__builtin_isfinite(v)) return -1;
-    if (hp.equal) {
-        if (hp.d == 0.0) return 0;
-        double a = ceil((v - hp.cutoff0) / hp.d);
-        if (a < 0.0) a = 0.0;
-        if (a > (double)(hp.nbins - 1)) a = (double)(hp.nbins - 1);
-        return (int)a;
-    }
-    int b = 0;
-    while (b < hp.nbins - 1 && mids[b] < v) ++b;
-    return b;
-}
-
-__global__ __launch_bounds__(RMS_BLOCK) void k_rms_hist(const double *__restrict__ coh,
-                                                        double *__restrict__ coh_prev,
-                                                        double *__restrict__ part,
-                                                        uint32_t *__restrict__ hist,
-                                                        const double *__restrict__ mids,
-                                                        HistParams hp,
-    const int32_t *__restrict__ gate) {
-    if (loop_stopped(gate)) return;
-    __shared__ double red[RMS_BLOCK];
-    __shared__ uint32_t lh[128];
-    for (int q = threadIdx.x; q < 128; q += RMS_BLOCK) lh[q] = 0;
-    __syncthreads();
-    double s = 0.0;
-    const int64_t base = (int64_t)blockIdx.x * RMS_BLOCK * RMS_PER_THREAD;
-#pragma unroll
-    for (int e = 0; e < RMS_PER_THREAD; ++e) {
-        const int64_t k = base + (int64_t)e * RMS_BLOCK + threadIdx.x;
-        if (k < hp.P) {
-            const double c = coh[k];
-            const double d = c - coh_prev[k];
-            s += d * d;
-            coh_prev[k] = c;
-            const int bin = hist_bin(c, mids, hp);
-            if (bin >= 0) atomicAdd(&lh[bin], 1u);
-        }
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = RMS_BLOCK / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-    for (int q = threadIdx.x; q < hp.nbins; q += RMS_BLOCK)
-        if (lh[q]) atomicAdd(&hist[q], lh[q]);
 }
 
 __global__ void k_zero_u32(uint32_t *p, int64_t n, const int32_t *__restrict__ gate) {

@@ -435,7 +435,7 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->prand_hi.alloc(sizeof(double) * 2048))) return rc;
     if ((rc = s->ctl.alloc(sizeof(LoopCtl)))) return rc;
     if ((rc = s->h_ctl.alloc(sizeof(LoopCtl)))) return rc;
-    s->n_rms_blocks = (int)blocks_for(P, RMS_BLOCK * RMS_PER_THREAD);
+    s->n_rms_blocks = (int)blocks_for(P, 256);   // one partial per refine workgroup
     if ((rc = s->rms_part.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
     {   // control-kernel constants: gausswin(7, 2.5) taps and their sum (native_stage2.rs:1685-1699)
         CtlParams &cp = s->ctl_params;
@@ -624,10 +624,10 @@ int scan_chunk(const ps2_session *s) {
 }
 
 template <typename R>
-int launch_fit(ps2_session *s, bool clear_count = true) {
+int launch_fit(ps2_session *s, bool in_loop = false) {
     const FitParams fp = fit_params(s);
     const unsigned nb = blocks_for(s->P, 256);
-    if (sizeof(R) == 4 && clear_count) {   // (inside the loop k_control clears it)
+    if (sizeof(R) == 4 && !in_loop) {   // (inside the loop k_control clears it)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4,
                            s->gate);
         PS2_LAUNCHED();
@@ -665,32 +665,26 @@ int launch_fit(ps2_session *s, bool clear_count = true) {
                            s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp, s->gate);
         PS2_LAUNCHED();
     }
+    // inside the loop the refine kernel also runs the gamma-change RMS partials,
+    // coh_prev <- coh and the coherence histogram (k_control reads them)
+    RmsHistOut ro{};
+    if (in_loop) {
+        ro.coh_prev = s->cohprev.as<double>();
+        ro.part = s->rms_part.as<double>();
+        ro.hist = s->hist.as<uint32_t>();
+        ro.mids = s->mids.as<double>();
+        ro.hp.P = s->P;
+        ro.hp.equal = s->hist_rule.equal ? 1 : 0;
+        ro.hp.cutoff0 = s->hist_rule.cutoff0;
+        ro.hp.d = s->hist_rule.d;
+        ro.hp.nbins = PS2_COH_BINS;
+    }
     SlotTimer tm(s, SLOT_REFINE);
     hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                        s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                        s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
                        s->Cc.as<double>(),
-                       s->coh.as<double>(), s->nopt.as<double>(), fp, s->gate);
-    PS2_LAUNCHED();
-    return PS2_OK;
-}
-
-int launch_rms_hist(ps2_session *s, bool clear_hist = true) {
-    HistParams hp;
-    hp.P = s->P;
-    hp.equal = s->hist_rule.equal ? 1 : 0;
-    hp.cutoff0 = s->hist_rule.cutoff0;
-    hp.d = s->hist_rule.d;
-    hp.nbins = PS2_COH_BINS;
-    if (clear_hist) {   // (inside the loop k_control clears it)
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, s->stream, s->hist.as<uint32_t>(),
-                           CTL_HIST_WORDS, s->gate);
-        PS2_LAUNCHED();
-    }
-    SlotTimer tm(s, SLOT_RMS);
-    hipLaunchKernelGGL(k_rms_hist, dim3(s->n_rms_blocks), dim3(RMS_BLOCK), 0, s->stream,
-                       s->coh.as<double>(), s->cohprev.as<double>(), s->rms_part.as<double>(),
-                       s->hist.as<uint32_t>(), s->mids.as<double>(), hp, s->gate);
+                       s->coh.as<double>(), s->nopt.as<double>(), fp, ro, s->gate);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -710,9 +704,8 @@ int launch_iteration(ps2_session *s) {
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
     if ((rc = launch_clap<R>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
-    if ((rc = launch_fit<R>(s, false))) return rc;
+    if ((rc = launch_fit<R>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
-    if ((rc = launch_rms_hist(s, false))) return rc;
     // gamma-change RMS, stop test, Nr scaling and the P-square table
     hipLaunchKernelGGL(k_control, dim3(1), dim3(CTL_THREADS), 0, s->stream, s->ctl.as<LoopCtl>(),
                        s->rms_part.as<double>(), s->hist.as<uint32_t>(),
