@@ -895,6 +895,24 @@ __global__ __launch_bounds__(256) void k_trial_setup(const R *__restrict__ bp, i
     }
 }
 
+// FP32 scan accumulators: (pre, pim) hold A_q = sum v Re(z^q) and (nre, nim)
+// B_q = sum v Im(z^q) for q in [q0, qmax]; turn them into the trial sums
+// sum v z^q = A + iB (pre, pim) and sum v conj(z)^q = A - iB (nre, nim).
+template <typename R, int TC>
+__device__ inline void scan_sums_from_ab(R (&pre)[TC], R (&pim)[TC], R (&nre)[TC], R (&nim)[TC], int q0,
+                                         int qmax) {
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+        if (q >= q0 && q <= qmax) {
+            const R ar = pre[q], ai = pim[q], br = nre[q], bi = nim[q];
+            pre[q] = ar - bi;
+            pim[q] = ai + br;
+            nre[q] = ar + bi;
+            nim[q] = ai - br;
+        }
+    }
+}
+
 // Trial scan of topofit_row (native_stage2.rs:1207-1256) for one pixel per
 // thread.  Trials are processed in chunks of TC positive trials t and their
 // negatives -t.
@@ -978,10 +996,13 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
 #pragma unroll
                 for (int q = 1; q < TC; ++q) {
                     if (q <= tn) {
-                        pre[q] = fmaf(v.re, zr, fmaf(-v.im, zi, pre[q]));
-                        pim[q] = fmaf(v.re, zi, fmaf(v.im, zr, pim[q]));
-                        nre[q] = fmaf(v.re, zr, fmaf(v.im, zi, nre[q]));
-                        nim[q] = fmaf(v.im, zr, fmaf(-v.re, zi, nim[q]));
+                        // A_q = sum v Re(z^q) in (pre, pim), B_q = sum v Im(z^q)
+                        // in (nre, nim): both trial sums v z^q = A + iB and
+                        // v conj(z)^q = A - iB from four FMAs
+                        pre[q] = fmaf(v.re, zr, pre[q]);
+                        pim[q] = fmaf(v.im, zr, pim[q]);
+                        nre[q] = fmaf(v.re, zi, nre[q]);
+                        nim[q] = fmaf(v.im, zi, nim[q]);
                         const R nzr = fmaf(zr, wr, -zi * wi);
                         const R nzi = fmaf(zr, wi, zi * wr);
                         zr = nzr;
@@ -997,6 +1018,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                 return;
             }
             if (denom == (R)0) denom = (R)1;
+            scan_sums_from_ab<R, TC>(pre, pim, nre, nim, 1, tn);
 #pragma unroll
             for (int q = 0; q < TC; ++q) {
                 if (q <= tn) {
@@ -1041,11 +1063,11 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
 #pragma unroll
                 for (int q = 0; q < TC; ++q) {
                     if (t0 + q <= tn) {
-                        // v*z and v*conj(z)
-                        pre[q] = fmaf(v.re, zr, fmaf(-v.im, zi, pre[q]));
-                        pim[q] = fmaf(v.re, zi, fmaf(v.im, zr, pim[q]));
-                        nre[q] = fmaf(v.re, zr, fmaf(v.im, zi, nre[q]));
-                        nim[q] = fmaf(v.im, zr, fmaf(-v.re, zi, nim[q]));
+                        // A and B as in the single-chunk path
+                        pre[q] = fmaf(v.re, zr, pre[q]);
+                        pim[q] = fmaf(v.im, zr, pim[q]);
+                        nre[q] = fmaf(v.re, zi, nre[q]);
+                        nim[q] = fmaf(v.im, zi, nim[q]);
                         const R nzr = fmaf(zr, wr, -zi * wi);
                         const R nzi = fmaf(zr, wi, zi * wr);
                         zr = nzr;
@@ -1070,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                 }
             }
         }
+        if constexpr (sizeof(R) == 4) scan_sums_from_ab<R, TC>(pre, pim, nre, nim, 0, tn - t0);
         if (t0 == 0) {
             if (!valid) {
                 tix[k] = -1;
