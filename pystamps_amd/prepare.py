@@ -216,7 +216,11 @@ def trial_wrap_mean_incidence(raw: RawPatch) -> float:
 def _bperp_rows_invariant(mat: Optional[np.ndarray]) -> bool:
     if mat is None or mat.shape[0] <= 1:
         return True
-    return bool(np.all(mat[1:] == mat[0:1]))
+    # per-pixel baselines usually differ within the first rows: decide there
+    # before comparing the whole matrix
+    if not np.all(mat[1:17] == mat[0:1]):
+        return False
+    return bool(np.all(mat[17:] == mat[0:1]))
 
 
 def nr_hip(bperp_nm, trials, bins, n_rows=STAGE2_RANDOM_COUNT, seed=STAGE2_RANDOM_SEED, device=0):
