@@ -125,6 +125,28 @@ class PinnedStaging {
         std::lock_guard<std::mutex> lk(mu_);
         int rc;
         if ((rc = ensure())) return rc;
+        // on an error, wait for the DMA already queued before the pool is released
+        if ((rc = d2h_chunks(dst, src, n, st))) (void)hipStreamSynchronize(st);
+        return rc;
+    }
+
+    int h2d(void *dst, const void *src, size_t n, hipStream_t st) {
+        if (n == 0) return PS2_OK;
+        if (n < kDirect) {
+            PS2_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+            PS2_HIP(hipStreamSynchronize(st));
+            return PS2_OK;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        int rc;
+        if ((rc = ensure())) return rc;
+        if ((rc = h2d_chunks(dst, src, n, st))) (void)hipStreamSynchronize(st);
+        return rc;
+    }
+
+  private:
+    int d2h_chunks(void *dst, const void *src, size_t n, hipStream_t st) {
+        int rc;
         Events ev;
         if ((rc = ev.create())) return rc;
         const size_t nk = (n + kChunk - 1) / kChunk;
@@ -145,16 +167,8 @@ class PinnedStaging {
         return PS2_OK;
     }
 
-    int h2d(void *dst, const void *src, size_t n, hipStream_t st) {
-        if (n == 0) return PS2_OK;
-        if (n < kDirect) {
-            PS2_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
-            PS2_HIP(hipStreamSynchronize(st));
-            return PS2_OK;
-        }
-        std::lock_guard<std::mutex> lk(mu_);
+    int h2d_chunks(void *dst, const void *src, size_t n, hipStream_t st) {
         int rc;
-        if ((rc = ensure())) return rc;
         Events ev;
         if ((rc = ev.create())) return rc;
         const size_t nk = (n + kChunk - 1) / kChunk;
@@ -170,7 +184,6 @@ class PinnedStaging {
         return PS2_OK;
     }
 
-  private:
     struct Events {
         hipEvent_t e[2] = {nullptr, nullptr};
         int create() {
