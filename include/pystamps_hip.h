@@ -123,7 +123,12 @@ int ps2_hip_device_count(void);
 int ps2_hip_run(const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts);
 
 /* Several patches, each with its own convergence; the reference runs them one
- * after another (pystamps-native.rs:158-176). */
+ * after another (pystamps-native.rs:158-176).  Patches go in waves of up to
+ * 16: the wave's patches are uploaded, their loops run concurrently (one
+ * stream and host worker each), their outputs are downloaded and their device
+ * memory is freed before the next wave, so device memory holds one wave.
+ * After a failure no new patch starts; the error names the lowest failing
+ * patch ("patch k: ..."); outputs of earlier waves are written. */
 int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out,
                       const ps2_run_opts *opts);
 
@@ -210,9 +215,9 @@ int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_se
 int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out);
 void ps2_hip_session_destroy(ps2_session *s);
 
-/* Runs the loops of n sessions concurrently (each on its own stream and host
- * thread; sessions may be on the same device) and returns after every loop's
- * stop decision.  i_loops[k] / seconds[k] receive each session's iteration
+/* Runs the loops of n sessions concurrently (each on its own stream; up to 16
+ * host workers, so at most 16 loops are in flight; sessions may be on the
+ * same device) and returns after every loop's stop decision.  i_loops[k] / seconds[k] receive each session's iteration
  * count and device loop time (either may be NULL).  This is the batched form
  * of the reference's sequential per-patch calls (pystamps-native.rs:158-176). */
 int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loops,

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -966,6 +967,46 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
 // ============================================================================
 // C ABI
 
+namespace {
+
+// Runs job(q) for q = 0..n-1 on min(n, kBatchWorkers) host threads (each
+// patch's loop is on its own stream, so at most that many loops are in
+// flight).  After the first failure no new job starts; the failure with the
+// lowest index is reported as "patch <base + q>: <message>".
+constexpr size_t kBatchWorkers = 16;
+
+template <typename Job>
+int run_jobs(size_t n, Job job, size_t base = 0) {
+    std::vector<int> rcs(n, PS2_OK);
+    std::vector<std::string> errs(n);
+    std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    auto worker = [&]() {
+        for (;;) {
+            const size_t q = next.fetch_add(1);
+            if (q >= n || failed.load()) return;
+            rcs[q] = job(q);
+            if (rcs[q]) {
+                errs[q] = g_last_error;
+                failed.store(true);
+            }
+        }
+    };
+    const size_t nw = std::min(n, kBatchWorkers);
+    std::vector<std::thread> th;
+    th.reserve(nw);
+    for (size_t w = 0; w < nw; ++w) th.emplace_back(worker);
+    for (auto &t : th) t.join();
+    for (size_t q = 0; q < n; ++q)
+        if (rcs[q]) {
+            set_error("patch " + std::to_string(base + q) + ": " + errs[q]);
+            return rcs[q];
+        }
+    return PS2_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ps2_hip_abi_version(void) { return PS2_HIP_ABI_VERSION; }
@@ -1104,49 +1145,32 @@ int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loop
     if (n && !sessions) return arg_error("sessions is NULL");
     for (size_t q = 0; q < n; ++q)
         if (!sessions[q]) return arg_error("session " + std::to_string(q) + " is NULL");
-    std::vector<int> rcs(n, PS2_OK);
-    std::vector<std::string> errs(n);
-    std::vector<std::thread> th;
-    th.reserve(n);
-    for (size_t q = 0; q < n; ++q)
-        th.emplace_back([&, q]() {
-            int64_t il = 0;
-            double sec = 0.0;
-            rcs[q] = ps2_hip_session_run(sessions[q], &il, &sec);
-            if (rcs[q]) errs[q] = g_last_error;
-            if (i_loops) i_loops[q] = il;
-            if (seconds) seconds[q] = sec;
-        });
-    for (auto &t : th) t.join();
-    for (size_t q = 0; q < n; ++q)
-        if (rcs[q]) {
-            set_error("patch " + std::to_string(q) + ": " + errs[q]);
-            return rcs[q];
-        }
-    return PS2_OK;
+    return run_jobs(n, [&](size_t q) {
+        int64_t il = 0;
+        double sec = 0.0;
+        const int rc = ps2_hip_session_run(sessions[q], &il, &sec);
+        if (i_loops) i_loops[q] = il;
+        if (seconds) seconds[q] = sec;
+        return rc;
+    });
 }
 
 int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts) {
     if (n && (!in || !out)) return arg_error("batch arrays are NULL");
-    std::vector<ps2_session *> ss(n, nullptr);
-    auto cleanup = [&]() {
+    // waves of kBatchWorkers patches: upload all, run their loops together,
+    // download all, free.  Device memory holds one wave, and no allocation,
+    // free or blocking copy runs while a loop's graph is being captured.
+    for (size_t w0 = 0; w0 < n; w0 += kBatchWorkers) {
+        const size_t m = std::min(kBatchWorkers, n - w0);
+        std::vector<ps2_session *> ss(m, nullptr);
+        int rc = run_jobs(m, [&](size_t q) { return session_create(&in[w0 + q], opts, &ss[q]); }, w0);
+        if (!rc) rc = run_jobs(m, [&](size_t q) { return ps2_hip_session_run(ss[q], nullptr, nullptr); }, w0);
+        if (!rc)
+            rc = run_jobs(m, [&](size_t q) { return ps2_hip_session_download(ss[q], &out[w0 + q]); }, w0);
         for (auto *x : ss) ps2_hip_session_destroy(x);
-    };
-    for (size_t q = 0; q < n; ++q) {
-        int rc = session_create(&in[q], opts, &ss[q]);
-        if (rc) {
-            set_error("patch " + std::to_string(q) + ": " + g_last_error);
-            cleanup();
-            return rc;
-        }
+        if (rc) return rc;
     }
-    int rc = ps2_hip_sessions_run(n, ss.data(), nullptr, nullptr);
-    for (size_t q = 0; q < n && !rc; ++q) {
-        rc = ps2_hip_session_download(ss[q], &out[q]);
-        if (rc) set_error("patch " + std::to_string(q) + ": " + g_last_error);
-    }
-    cleanup();
-    return rc;
+    return PS2_OK;
 }
 
 // --- single steps -------------------------------------------------------------

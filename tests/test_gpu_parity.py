@@ -335,6 +335,24 @@ def test_batch_of_uneven_patches(S, ora):
                 np.testing.assert_allclose(b["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
 
 
+def test_batch_more_patches_than_workers_and_failure(S):
+    """20 patches (more than the 16 batch workers): every result equals its
+    single run.  A batch with one invalid patch fails naming that patch."""
+    import dataclasses as dc
+    preps = [small_prep(n_ps=1200 + 150 * k, n_ifg=8, seed=0x5EED1400 + k) for k in range(20)]
+    batch = S.run_stage2_hip_batch(preps, form="fp32")
+    assert len(batch) == 20
+    for p, b in zip(preps, batch):
+        single = S.run_stage2_hip(p, form="fp32")
+        assert b["i_loop"] == single["i_loop"]
+        for key in ("K_ps", "coh_ps", "trial_ix", "ph_patch"):
+            assert np.array_equal(b[key], single[key], equal_nan=True), key
+    bad = dc.replace(preps[5], grid_lin=preps[5].grid_lin.copy())
+    bad.grid_lin[3] = bad.n_i * bad.n_j   # outside the grid: rejected on the host
+    with pytest.raises(S.Stage2Error, match="patch 5: grid_lin"):
+        S.run_stage2_hip_batch(preps[:5] + [bad] + preps[6:8], form="fp32")
+
+
 def test_sessions_run_concurrently_matches_sequential(S, prep_small):
     p2 = small_prep(n_ps=3000, n_ifg=24, seed=0x5EED1200)
     ss = [S.Stage2Session(prep_small, form="fp32"), S.Stage2Session(p2, form="fp32")]
