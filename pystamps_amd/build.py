@@ -21,7 +21,7 @@ LIB = PKG / "libpystamps_hip.so"
 ARCH = os.environ.get("PS2_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["ps2_stage2.hip", "ps2_nr.hip", "ps2_registry.hip", "ps2_host_util.cpp", "ps2_prep.cpp"]
-HEADERS = ["ps2_device.hpp", "ps2_kernels.hpp", "ps2_host_util.hpp"]
+HEADERS = ["ps2_device.hpp", "ps2_kernels.hpp", "ps2_host_util.hpp", "ps2_staging.hpp"]
 
 
 def hipcc() -> str:

@@ -6,6 +6,7 @@
 // integer atomics, so the counts are exact and order-independent.
 #include "pystamps_hip.h"
 #include "ps2_host_util.hpp"
+#include "ps2_staging.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -172,6 +173,7 @@ int nr_histogram(const std::vector<double> &ph, const double *bperp_nm, int64_t 
     int rc = PS2_OK;
     double *d_ph = nullptr, *d_bp = nullptr, *d_mids = nullptr;
     uint32_t *d_hist = nullptr;
+    hipStream_t st = nullptr;   // private stream: no device-wide synchronisation
     std::vector<uint32_t> counts(128, 0);
     NrParams p{};
     {
@@ -201,21 +203,22 @@ int nr_histogram(const std::vector<double> &ph, const double *bperp_nm, int64_t 
         p.nbins = PS2_COH_BINS;
         std::vector<double> mids(128, 0.0);
         for (size_t k = 0; k < hr.mids.size() && k < 128; ++k) mids[k] = hr.mids[k];
+        NR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         NR_HIP(hipMalloc(&d_ph, sizeof(double) * ph.size()));
         NR_HIP(hipMalloc(&d_bp, sizeof(double) * n_ifg));
         NR_HIP(hipMalloc(&d_mids, sizeof(double) * 128));
         NR_HIP(hipMalloc(&d_hist, sizeof(uint32_t) * 128));
-        NR_HIP(hipMemcpy(d_ph, ph.data(), sizeof(double) * ph.size(), hipMemcpyHostToDevice));
-        NR_HIP(hipMemcpy(d_bp, bperp_nm, sizeof(double) * n_ifg, hipMemcpyHostToDevice));
-        NR_HIP(hipMemcpy(d_mids, mids.data(), sizeof(double) * 128, hipMemcpyHostToDevice));
-        NR_HIP(hipMemset(d_hist, 0, sizeof(uint32_t) * 128));
+        if ((rc = staging_h2d(d_ph, ph.data(), sizeof(double) * ph.size(), st))) goto done;
+        NR_HIP(hipMemcpyAsync(d_bp, bperp_nm, sizeof(double) * n_ifg, hipMemcpyHostToDevice, st));
+        NR_HIP(hipMemcpyAsync(d_mids, mids.data(), sizeof(double) * 128, hipMemcpyHostToDevice, st));
+        NR_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint32_t) * 128, st));
         if (n_rows > 0) {
-            hipLaunchKernelGGL(k_nr_rows, dim3((unsigned)((n_rows + 255) / 256)), dim3(256), 0, 0, d_ph,
+            hipLaunchKernelGGL(k_nr_rows, dim3((unsigned)((n_rows + 255) / 256)), dim3(256), 0, st, d_ph,
                                d_bp, d_mids, d_hist, p);
             NR_HIP(hipGetLastError());
         }
-        NR_HIP(hipDeviceSynchronize());
-        NR_HIP(hipMemcpy(counts.data(), d_hist, sizeof(uint32_t) * 128, hipMemcpyDeviceToHost));
+        NR_HIP(hipMemcpyAsync(counts.data(), d_hist, sizeof(uint32_t) * 128, hipMemcpyDeviceToHost, st));
+        NR_HIP(hipStreamSynchronize(st));
     }
     {
         int last = -1;
@@ -226,10 +229,12 @@ int nr_histogram(const std::vector<double> &ph, const double *bperp_nm, int64_t 
         *out_nr_max_nz_ix = last >= 0 ? (double)(last + 1) : 1.0;
     }
 done:
+    if (st) (void)hipStreamSynchronize(st);
     if (d_ph) (void)hipFree(d_ph);
     if (d_bp) (void)hipFree(d_bp);
     if (d_mids) (void)hipFree(d_mids);
     if (d_hist) (void)hipFree(d_hist);
+    if (st) (void)hipStreamDestroy(st);
     return rc;
 }
 

@@ -222,6 +222,19 @@ struct DevBuf {
         }                                                                                 \
     } while (0)
 
+// A private non-blocking stream for one call: no device-wide synchronisation,
+// so calls run beside other work on the device (sessions, graph captures).
+struct CallStream {
+    hipStream_t s = nullptr;
+    ~CallStream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+    hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+};
+
 int set_device(int32_t device) {
     int n = 0;
     REG_HIP(hipGetDeviceCount(&n));
@@ -281,19 +294,22 @@ extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int3
     REG_HIP(dC.alloc(P));
     REG_HIP(dcoh.alloc(P));
     if (res_out) REG_HIP(dres.alloc(P * I));
-    REG_HIP(hipMemcpy(dc.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice));
-    REG_HIP(hipMemcpy(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
+    CallStream cs;
+    REG_HIP(cs.create());
+    REG_HIP(hipMemcpyAsync(dc.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, cs.s));
+    REG_HIP(hipMemcpyAsync(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, cs.s));
     PyFitParams prm{P, (int)I, T, tn, row_invariant ? 1 : 0};
-    hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, dc.p, db.p,
+    hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, cs.s, dc.p, db.p,
                        dcoh_t.p, dK.p, dC.p, dcoh.p, res_out ? dres.p : nullptr, prm);
     REG_HIP(hipGetLastError());
-    REG_HIP(hipDeviceSynchronize());
-    REG_HIP(hipMemcpy(k_out, dK.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    REG_HIP(hipMemcpy(c_out, dC.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    REG_HIP(hipMemcpy(coh_out, dcoh.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    REG_HIP(hipMemcpyAsync(k_out, dK.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
+    REG_HIP(hipMemcpyAsync(c_out, dC.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
+    REG_HIP(hipMemcpyAsync(coh_out, dcoh.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
+    REG_HIP(hipStreamSynchronize(cs.s));
     if (res_out) {
         std::vector<float2> hr(P * I);
-        REG_HIP(hipMemcpy(hr.data(), dres.p, sizeof(float2) * hr.size(), hipMemcpyDeviceToHost));
+        REG_HIP(hipMemcpyAsync(hr.data(), dres.p, sizeof(float2) * hr.size(), hipMemcpyDeviceToHost, cs.s));
+        REG_HIP(hipStreamSynchronize(cs.s));
         for (int64_t k = 0; k < P; ++k)
             for (int64_t i = 0; i < I; ++i) {
                 res_out[2 * (k * I + i)] = hr[i * P + k].x;
@@ -335,16 +351,18 @@ extern "C" int ps2_hip_grid_accumulate(const float *ph_weight, const int64_t *gr
     REG_HIP(dout.alloc(C * I));
     REG_HIP(dorder.alloc(P));
     REG_HIP(dstart.alloc(C + 1));
+    CallStream cs;
+    REG_HIP(cs.create());
     if (P > 0) {
-        REG_HIP(hipMemcpy(dph.p, ph_weight, sizeof(float2) * P * I, hipMemcpyHostToDevice));
-        REG_HIP(hipMemcpy(dorder.p, order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice));
+        REG_HIP(hipMemcpyAsync(dph.p, ph_weight, sizeof(float2) * P * I, hipMemcpyHostToDevice, cs.s));
+        REG_HIP(hipMemcpyAsync(dorder.p, order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, cs.s));
     }
-    REG_HIP(hipMemcpy(dstart.p, start.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_py_grid_accumulate, dim3((unsigned)((C * I + 255) / 256)), dim3(256), 0, 0,
+    REG_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, cs.s));
+    hipLaunchKernelGGL(k_py_grid_accumulate, dim3((unsigned)((C * I + 255) / 256)), dim3(256), 0, cs.s,
                        dph.p, dorder.p, dstart.p, dout.p, C, (int)I);
     REG_HIP(hipGetLastError());
-    REG_HIP(hipDeviceSynchronize());
-    REG_HIP(hipMemcpy(out, dout.p, sizeof(float2) * C * I, hipMemcpyDeviceToHost));
+    REG_HIP(hipMemcpyAsync(out, dout.p, sizeof(float2) * C * I, hipMemcpyDeviceToHost, cs.s));
+    REG_HIP(hipStreamSynchronize(cs.s));
     return PS2_OK;
 }
 

@@ -3,6 +3,7 @@
 #include "pystamps_hip.h"
 #include "ps2_host_util.hpp"
 #include "ps2_kernels.hpp"
+#include "ps2_staging.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -236,6 +237,14 @@ PinnedStaging &staging() {
     return *pool;
 }
 
+int staging_h2d(void *dst_device, const void *src_host, size_t bytes, hipStream_t stream) {
+    return staging().h2d(dst_device, src_host, bytes, stream);
+}
+
+int staging_d2h(void *dst_host, const void *src_device, size_t bytes, hipStream_t stream) {
+    return staging().d2h(dst_host, src_device, bytes, stream);
+}
+
 }  // namespace ps2
 
 using namespace ps2;
@@ -274,6 +283,7 @@ struct ps2_session {
     CtlParams ctl_params{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool no_graph = false;   // a capture failed: launch step-wise
     // per-kernel timing (ps2_hip_session_set_timing)
     bool timing = false;
     std::vector<hipEvent_t> tev;
@@ -765,26 +775,29 @@ int run_loop(ps2_session *s) {
     PS2_HIP(hipMemcpyAsync(s->ctl.p, hc, sizeof(LoopCtl), hipMemcpyHostToDevice, s->stream));
     s->gate = &s->ctl.as<LoopCtl>()->stopped;
     const int64_t max_iter = s->ctl_params.max_iter;
-    const bool stepwise = s->timing || s->verbose;
-    if (!stepwise) {
-        if (!s->graph_exec) {
-            PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            rc = launch_iteration<R>(s);
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(s->stream, &g);
-            if (rc) {
-                if (g) (void)hipGraphDestroy(g);
-                s->gate = nullptr;
-                return rc;
-            }
-            if (e != hipSuccess) {
-                set_error(std::string("graph capture of the gamma iteration failed: ") + hipGetErrorString(e));
-                s->gate = nullptr;
-                return PS2_E_DEVICE;
-            }
+    bool stepwise = s->timing || s->verbose || s->no_graph;
+    if (!stepwise && !s->graph_exec) {
+        PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        rc = launch_iteration<R>(s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        hipError_t ei = hipErrorUnknown;
+        if (!rc && e == hipSuccess && g) ei = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
+        if (ei == hipSuccess) {
             s->graph = g;
-            PS2_HIP(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+        } else {
+            // The capture failed, e.g. invalidated by another host thread's
+            // allocation, free or blocking copy on the device meanwhile.
+            // Nothing of the captured iteration ran: this session launches its
+            // iterations one by one from now on (same kernels, same results).
+            if (g) (void)hipGraphDestroy(g);
+            s->graph_exec = nullptr;
+            (void)hipGetLastError();
+            s->no_graph = true;
+            stepwise = true;
         }
+    }
+    if (!stepwise) {
         for (int64_t it = 0; it < max_iter; ++it) PS2_HIP(hipGraphLaunch(s->graph_exec, s->stream));
     } else {
         for (int64_t it = 0; it < max_iter; ++it) {
