@@ -353,6 +353,58 @@ def test_batch_more_patches_than_workers_and_failure(S):
         S.run_stage2_hip_batch(preps[:5] + [bad] + preps[6:8], form="fp32")
 
 
+def test_calls_from_several_host_threads(S, ora):
+    """Sessions' first runs (graph capture), Nr histograms and registry
+    topofits issued from different host threads at once: every result equals
+    the same call made alone."""
+    import threading
+    from pystamps_amd import registry_hip
+    from pystamps_amd.prepare import coh_bins, nr_hip, trial_values
+    preps = [small_prep(n_ps=3000 + 500 * k, n_ifg=12, seed=0x5EED1600 + k) for k in range(4)]
+    alone = [S.run_stage2_hip(p, form="fp32") for p in preps]
+    rng = np.random.default_rng(9)
+    cpx = np.exp(1j * rng.uniform(-np.pi, np.pi, (2000, 15)))
+    bp = rng.uniform(-200, 200, (2000, 15))
+    fit_alone = registry_hip.stage2_topofit(cpx, bp, 0.6)
+    bpn, tv, cb = np.linspace(-120.0, 150.0, 11), trial_values(0.6), coh_bins()
+    nr_alone = nr_hip(bpn, tv, cb, n_rows=8000)
+    out, errors = {}, []
+
+    def guard(fn):
+        def run():
+            try:
+                fn()
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(e)
+        return run
+
+    def loops(k):
+        for _ in range(2):
+            with S.Stage2Session(preps[k], form="fp32") as s:
+                s.run()
+                out[("loop", k)] = s.download()
+
+    def others():
+        for q in range(3):
+            out[("fit", q)] = registry_hip.stage2_topofit(cpx, bp, 0.6)
+            out[("nr", q)] = nr_hip(bpn, tv, cb, n_rows=8000)
+
+    threads = [threading.Thread(target=guard(lambda k=k: loops(k))) for k in range(4)]
+    threads.append(threading.Thread(target=guard(others)))
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for k in range(4):
+        for key in ("K_ps", "coh_ps", "trial_ix", "ph_patch"):
+            assert np.array_equal(out[("loop", k)][key], alone[k][key], equal_nan=True), (k, key)
+    for q in range(3):
+        for a, b in zip(out[("fit", q)], fit_alone):
+            assert np.array_equal(a, b, equal_nan=True)
+        assert np.array_equal(out[("nr", q)][0], nr_alone[0]) and out[("nr", q)][1] == nr_alone[1]
+
+
 def test_sessions_run_concurrently_matches_sequential(S, prep_small):
     p2 = small_prep(n_ps=3000, n_ifg=24, seed=0x5EED1200)
     ss = [S.Stage2Session(prep_small, form="fp32"), S.Stage2Session(p2, form="fp32")]
