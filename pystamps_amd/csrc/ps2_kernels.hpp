@@ -924,7 +924,9 @@ __device__ inline void scan_sums_from_ab(R (&pre)[TC], R (&pim)[TC], R (&nre)[TC
 //   FP64 form (R=double): every rotation is an exact sincos, as the reference.
 // Output: tix (selected trial index, -1 for invalid rows) and offset (the
 // trial sum at the selected trial).  rng and wt come from k_trial_setup.
-template <typename R, int TC>
+// SINGLE (FP32 form, tn < TC): only the single-chunk path is compiled, so the
+// kernel needs fewer registers and more wavefronts fit per SIMD.
+template <typename R, int TC, bool SINGLE = false>
 __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                                               const R *__restrict__ bp,
                                               const cplx<R> *__restrict__ wt,
@@ -971,8 +973,9 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
         }
     };
 
+    static_assert(!SINGLE || sizeof(R) == 4, "the single-chunk scan is the FP32 form's");
     if constexpr (sizeof(R) == 4) {
-        if (tn < TC) {
+        if (SINGLE || tn < TC) {
             // one chunk: trials 0..tn and their negatives.  Trial 0 adds v
             // itself and the recurrence starts at z = e^{-i phi}, which is what
             // the general loop computes from z = 1 (exact multiplies by 1, 0).
@@ -1035,6 +1038,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
             return;
         }
     }
+    if constexpr (SINGLE) return;
     for (int t0 = 0; t0 <= tn; t0 += TC) {
         // chunk: positive trials t0 .. t0+TC-1 (t0 = 0 holds trial 0) and their negatives
         R pre[TC], pim[TC], nre[TC], nim[TC];

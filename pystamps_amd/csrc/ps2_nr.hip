@@ -203,11 +203,14 @@ int nr_histogram(const std::vector<double> &ph, const double *bperp_nm, int64_t 
         p.nbins = PS2_COH_BINS;
         std::vector<double> mids(128, 0.0);
         for (size_t k = 0; k < hr.mids.size() && k < 128; ++k) mids[k] = hr.mids[k];
-        NR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        NR_HIP(hipMalloc(&d_ph, sizeof(double) * ph.size()));
-        NR_HIP(hipMalloc(&d_bp, sizeof(double) * n_ifg));
-        NR_HIP(hipMalloc(&d_mids, sizeof(double) * 128));
-        NR_HIP(hipMalloc(&d_hist, sizeof(uint32_t) * 128));
+        {
+            AllocGuard g(capture_mutex());
+            NR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            NR_HIP(hipMalloc(&d_ph, sizeof(double) * ph.size()));
+            NR_HIP(hipMalloc(&d_bp, sizeof(double) * n_ifg));
+            NR_HIP(hipMalloc(&d_mids, sizeof(double) * 128));
+            NR_HIP(hipMalloc(&d_hist, sizeof(uint32_t) * 128));
+        }
         if ((rc = staging_h2d(d_ph, ph.data(), sizeof(double) * ph.size(), st))) goto done;
         NR_HIP(hipMemcpyAsync(d_bp, bperp_nm, sizeof(double) * n_ifg, hipMemcpyHostToDevice, st));
         NR_HIP(hipMemcpyAsync(d_mids, mids.data(), sizeof(double) * 128, hipMemcpyHostToDevice, st));
@@ -230,6 +233,7 @@ int nr_histogram(const std::vector<double> &ph, const double *bperp_nm, int64_t 
     }
 done:
     if (st) (void)hipStreamSynchronize(st);
+    AllocGuard g(capture_mutex());
     if (d_ph) (void)hipFree(d_ph);
     if (d_bp) (void)hipFree(d_bp);
     if (d_mids) (void)hipFree(d_mids);

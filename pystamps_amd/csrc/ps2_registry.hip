@@ -10,6 +10,7 @@
 // picks among several candidates (endpoint pairs keep the coarse argmax).
 #include "pystamps_hip.h"
 #include "ps2_host_util.hpp"
+#include "ps2_staging.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -208,9 +209,15 @@ template <typename T>
 struct DevBuf {
     T *p = nullptr;
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            AllocGuard g(capture_mutex());
+            (void)hipFree(p);
+        }
     }
-    hipError_t alloc(size_t n) { return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)); }
+    hipError_t alloc(size_t n) {
+        AllocGuard g(capture_mutex());
+        return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1));
+    }
 };
 
 #define REG_HIP(call)                                                                     \
@@ -229,10 +236,14 @@ struct CallStream {
     ~CallStream() {
         if (s) {
             (void)hipStreamSynchronize(s);
+            AllocGuard g(capture_mutex());
             (void)hipStreamDestroy(s);
         }
     }
-    hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+    hipError_t create() {
+        AllocGuard g(capture_mutex());
+        return hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    }
 };
 
 int set_device(int32_t device) {

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <utility>
 #include <string>
 #include <thread>
@@ -24,6 +25,11 @@ namespace ps2 {
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+std::shared_mutex &capture_mutex() {
+    static std::shared_mutex *m = new std::shared_mutex();   // process lifetime
+    return *m;
+}
 
 #define PS2_HIP(call)                                                                   \
     do {                                                                                \
@@ -63,9 +69,13 @@ struct DBuf {
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            AllocGuard g(capture_mutex());
+            (void)hipFree(p);
+        }
     }
     int alloc(size_t n) {
+        AllocGuard g(capture_mutex());
         if (p) {
             (void)hipFree(p);
             p = nullptr;
@@ -90,9 +100,13 @@ struct DBuf {
 struct HostPinned {
     void *p = nullptr;
     ~HostPinned() {
-        if (p) (void)hipHostFree(p);
+        if (p) {
+            AllocGuard g(capture_mutex());
+            (void)hipHostFree(p);
+        }
     }
     int alloc(size_t n) {
+        AllocGuard g(capture_mutex());
         hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
         if (e != hipSuccess) {
             p = nullptr;
@@ -199,6 +213,7 @@ class PinnedStaging {
     };
 
     int ensure() {
+        AllocGuard g(capture_mutex());
         for (auto &b : buf_) {
             if (b) continue;
             hipError_t e = hipHostMalloc(&b, kChunk, hipHostMallocPortable);
@@ -302,6 +317,7 @@ struct ps2_session {
         if (stream) {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
+            AllocGuard g(capture_mutex());
             if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
             if (graph) (void)hipGraphDestroy(graph);
             for (auto &e : ev)
@@ -313,6 +329,15 @@ struct ps2_session {
     }
     size_t rsz() const { return form == PS2_FORM_FP64 ? sizeof(double) : sizeof(float); }
 };
+
+// Host -> device copy on the session's stream, complete on return (the
+// library issues nothing on the legacy default stream, which would wait on
+// other threads' streams, including one being captured).
+static hipError_t h2d_now(ps2_session *s, void *dst, const void *src, size_t n) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return e;
+}
 
 namespace {
 
@@ -368,9 +393,9 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     if ((rc = s->cell_start.alloc(sizeof(int32_t) * (s->C + 1)))) return rc;
     if ((rc = s->cell_of.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->d_order.alloc(sizeof(int64_t) * P))) return rc;
-    PS2_HIP(hipMemcpy(s->cell_start.p, cstart.data(), sizeof(int32_t) * (s->C + 1), hipMemcpyHostToDevice));
-    PS2_HIP(hipMemcpy(s->cell_of.p, cell_of.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice));
-    PS2_HIP(hipMemcpy(s->d_order.p, s->order.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice));
+    PS2_HIP(h2d_now(s, s->cell_start.p, cstart.data(), sizeof(int32_t) * (s->C + 1)));
+    PS2_HIP(h2d_now(s, s->cell_of.p, cell_of.data(), sizeof(int32_t) * P));
+    PS2_HIP(h2d_now(s, s->d_order.p, s->order.data(), sizeof(int64_t) * P));
 
     // the row-major inputs go up as they are; the device permutes them into
     // sorted ifg-major planes (one staging buffer, reused)
@@ -392,9 +417,9 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
         if (s->rowinv) {
             std::vector<R> h((size_t)n);
             for (int64_t f = 0; f < I; ++f) h[f] = (R)in->bperp_row[f];
-            PS2_HIP(hipMemcpy(s->bp.p, h.data(), sizeof(R) * n, hipMemcpyHostToDevice));
+            PS2_HIP(h2d_now(s, s->bp.p, h.data(), sizeof(R) * n));
             if (sizeof(R) == 4)
-                PS2_HIP(hipMemcpy(s->bp64.p, in->bperp_row, sizeof(double) * n, hipMemcpyHostToDevice));
+                PS2_HIP(h2d_now(s, s->bp64.p, in->bperp_row, sizeof(double) * n));
         } else {
             PS2_HIP(hipStreamSynchronize(s->stream));  // the staging buffer is free again
             if ((rc = staging().h2d(stage.p, in->bperp_mat, sizeof(double) * P * I, s->stream))) return rc;
@@ -420,7 +445,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
             w0[k] = da != 0.0 ? 1.0 / da : 0.0;
         }
         if ((rc = s->W0.alloc(sizeof(double) * P))) return rc;
-        PS2_HIP(hipMemcpy(s->W0.p, w0.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+        PS2_HIP(h2d_now(s, s->W0.p, w0.data(), sizeof(double) * P));
     }
     {   // CLAP constants: low_pass, window weight x 1/n^2
         // stored transposed: column c's 32 values are contiguous (the CLAP
@@ -429,7 +454,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
         for (int r = 0; r < CLAP_FFT; ++r)
             for (int c = 0; c < CLAP_FFT; ++c) lp[c * CLAP_FFT + r] = (R)in->low_pass[r * CLAP_FFT + c];
         if ((rc = s->lp.alloc(sizeof(R) * lp.size()))) return rc;
-        PS2_HIP(hipMemcpy(s->lp.p, lp.data(), sizeof(R) * lp.size(), hipMemcpyHostToDevice));
+        PS2_HIP(h2d_now(s, s->lp.p, lp.data(), sizeof(R) * lp.size()));
     }
     return PS2_OK;
 }
@@ -481,7 +506,7 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->mids.alloc(sizeof(double) * 128))) return rc;
     std::vector<double> mids(128, 0.0);
     for (size_t k = 0; k < s->hist_rule.mids.size() && k < 128; ++k) mids[k] = s->hist_rule.mids[k];
-    PS2_HIP(hipMemcpy(s->mids.p, mids.data(), sizeof(double) * 128, hipMemcpyHostToDevice));
+    PS2_HIP(h2d_now(s, s->mids.p, mids.data(), sizeof(double) * 128));
     return PS2_OK;
 }
 
@@ -667,17 +692,26 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     // trials per chunk: the FP32 form scans all trials in one pass up to 48
     // positive trials (larger grids take 48-trial chunks)
     const int tc = scan_chunk(s);
-    if (tc == 8) {
-        go(k_scan<R, 8>);
-    } else if (tc == 16) {
-        go(k_scan<R, 16>);
-    } else if constexpr (sizeof(R) == 4) {
-        if (tc == 24)
-            go(k_scan<R, 24>);
+    if constexpr (sizeof(R) == 4) {
+        // FP32: every trial fits one chunk (tn < tc) except beyond 47
+        // positive trials, which take 48-trial chunks
+        if (tc == 8)
+            go(k_scan<R, 8, true>);
+        else if (tc == 16)
+            go(k_scan<R, 16, true>);
+        else if (tc == 24)
+            go(k_scan<R, 24, true>);
         else if (tc == 32)
-            go(k_scan<R, 32>);
+            go(k_scan<R, 32, true>);
+        else if (s->tn < 48)
+            go(k_scan<R, 48, true>);
         else
             go(k_scan<R, 48>);
+    } else {
+        if (tc == 8)
+            go(k_scan<R, 8>);
+        else
+            go(k_scan<R, 16>);
     }
     PS2_LAUNCHED();
     }
@@ -777,10 +811,13 @@ int run_loop(ps2_session *s) {
     const int64_t max_iter = s->ctl_params.max_iter;
     bool stepwise = s->timing || s->verbose || s->no_graph;
     if (!stepwise && !s->graph_exec) {
+        // no allocation anywhere in the library while the capture is open
+        std::unique_lock<std::shared_mutex> cap(capture_mutex());
         PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         rc = launch_iteration<R>(s);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        cap.unlock();
         hipError_t ei = hipErrorUnknown;
         if (!rc && e == hipSuccess && g) ei = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
         if (ei == hipSuccess) {
@@ -928,6 +965,7 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
         if (s->device < 0 || s->device >= n)
             return arg_error("device " + std::to_string(s->device) + " not present (" + std::to_string(n) + " visible)");
         PS2_HIP(hipSetDevice(s->device));
+        AllocGuard g(capture_mutex());
         PS2_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         for (auto &e : s->ev) PS2_HIP(hipEventCreate(&e));
     }
@@ -1196,7 +1234,7 @@ static int plane_upload(ps2_session *s, void *dev, const void *host) {
     for (int64_t c = 0; c < C; ++c)
         for (int64_t f = 0; f < I; ++f)
             std::memcpy(&t[2 * rs * (f * C + c)], (const char *)host + 2 * rs * (c * I + f), 2 * rs);
-    PS2_HIP(hipMemcpy(dev, t.data(), t.size(), hipMemcpyHostToDevice));
+    PS2_HIP(h2d_now(s, dev, t.data(), t.size()));
     return PS2_OK;
 }
 
@@ -1204,7 +1242,8 @@ static int plane_download(ps2_session *s, void *host, const void *dev) {
     const int64_t C = s->C, I = s->I;
     const size_t rs = s->rsz();
     std::vector<char> t((size_t)(2 * rs * C * I));
-    PS2_HIP(hipMemcpy(t.data(), dev, t.size(), hipMemcpyDeviceToHost));
+    PS2_HIP(hipMemcpyAsync(t.data(), dev, t.size(), hipMemcpyDeviceToHost, s->stream));
+    PS2_HIP(hipStreamSynchronize(s->stream));
     for (int64_t c = 0; c < C; ++c)
         for (int64_t f = 0; f < I; ++f)
             std::memcpy((char *)host + 2 * rs * (c * I + f), &t[2 * rs * (f * C + c)], 2 * rs);
@@ -1220,8 +1259,8 @@ int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *wei
         hk[k] = k_ps[s->order[k]];
         hw[k] = weighting[s->order[k]];
     }
-    PS2_HIP(hipMemcpy(s->K.p, hk.data(), sizeof(double) * s->P, hipMemcpyHostToDevice));
-    PS2_HIP(hipMemcpy(s->W.p, hw.data(), sizeof(double) * s->P, hipMemcpyHostToDevice));
+    PS2_HIP(h2d_now(s, s->K.p, hk.data(), sizeof(double) * s->P));
+    PS2_HIP(h2d_now(s, s->W.p, hw.data(), sizeof(double) * s->P));
     int rc = s->form == PS2_FORM_FP64 ? launch_fill_grid<double>(s, s->K.as<double>(), s->W.as<double>())
                                       : launch_fill_grid<float>(s, s->K.as<double>(), s->W.as<double>());
     if (rc) return rc;
