@@ -207,7 +207,14 @@ int ps2_hip_select_columns_f64(const double *src, int64_t n_rows, int64_t n_cols
  * table, one iteration is captured as a HIP graph on the session's first run
  * and replayed gamma_max_iterations times (iterations after the stop return at
  * once), and the host waits once per loop.  With timing or verbose output the
- * same kernels are launched iteration by iteration instead. */
+ * same kernels are launched iteration by iteration instead.
+ * Threads: every entry point may be called from several host threads at once.
+ * A session is used by one call at a time (calls on the same session
+ * serialise).  Sessions, Nr and registry calls each work on streams of their
+ * own and never on the legacy default stream; the library's device and pinned
+ * allocations wait while any of its sessions captures its graph.  If a
+ * capture is still disturbed by other code in the process, that session
+ * launches its iterations one by one from then on (same results). */
 typedef struct ps2_session ps2_session;
 
 int ps2_hip_session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session **out);
