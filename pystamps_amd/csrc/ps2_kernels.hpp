@@ -1134,6 +1134,19 @@ __device__ inline double wave_sum_f64(double v) {
     return v;
 }
 
+// four wave sums at once (the exchanges of the four values overlap)
+__device__ inline void wave_sum4_f64(double &a, double &b, double &c, double &d) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double ta = __shfl_xor(a, o, 64), tb = __shfl_xor(b, o, 64);
+        const double tc = __shfl_xor(c, o, 64), td = __shfl_xor(d, o, 64);
+        a += ta;
+        b += tb;
+        c += tc;
+        d += td;
+    }
+}
+
 template <typename RF>
 __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phnm,
                                                 const double *__restrict__ bp64,
@@ -1198,6 +1211,50 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
         if (range == 0.0) range = 1.0;
         double bestv = -INFINITY, bsr = 0.0, bsi = 0.0;
         int besti = -1;
+        if (I <= 64 * RESCORE_PER_LANE) {
+            // e^{-i phi t} by an f64 recurrence from t = 0 (drift ~t ulp, far
+            // below the f64 sums' own rounding): per positive trial t the
+            // wave sums A = sum v Re(z^t), B = sum v Im(z^t) give both trial
+            // sums, v z^t = A + iB (trial t) and v conj(z)^t = A - iB (trial
+            // -t); the first argmax over ascending trial index is kept with
+            // the smaller index winning exact ties
+            double zr[RESCORE_PER_LANE], zi[RESCORE_PER_LANE], wr[RESCORE_PER_LANE], wi[RESCORE_PER_LANE];
+#pragma unroll
+            for (int u = 0; u < RESCORE_PER_LANE; ++u) {
+                zr[u] = 1.0;
+                zi[u] = 0.0;
+                double sn, cs;
+                sincos((vb[u] / range) * 0.78539816339744830962, &sn, &cs);
+                wr[u] = cs;
+                wi[u] = -sn;
+            }
+            auto offer = [&](double sr, double si, int idx) {
+                const double c = hypot(sr, si) / denom;
+                if (besti < 0 || c > bestv || (c == bestv && idx < besti)) {
+                    bestv = c;
+                    besti = idx;
+                    bsr = sr;
+                    bsi = si;
+                }
+            };
+            for (int t = 0; t <= prm.tn; ++t) {
+                double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+#pragma unroll
+                for (int u = 0; u < RESCORE_PER_LANE; ++u) {
+                    ar = fma(vr[u], zr[u], ar);
+                    ai = fma(vi[u], zr[u], ai);
+                    br = fma(vr[u], zi[u], br);
+                    bi = fma(vi[u], zi[u], bi);
+                    const double nzr = zr[u] * wr[u] - zi[u] * wi[u];
+                    const double nzi = zr[u] * wi[u] + zi[u] * wr[u];
+                    zr[u] = nzr;
+                    zi[u] = nzi;
+                }
+                wave_sum4_f64(ar, ai, br, bi);
+                offer(ar - bi, ai + br, prm.tn + t);
+                if (t > 0) offer(ar + bi, ai - br, prm.tn - t);
+            }
+        } else
         for (int t = 0; t < prm.T; ++t) {
             const double tv = (double)(t - prm.tn);
             double sr = 0.0, si = 0.0;
