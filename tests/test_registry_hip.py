@@ -8,10 +8,22 @@ fit of rows with zero elements; the grid accumulation against numpy's
 (host code) against the oracle's golden-pinned rule; and the registry glue.
 GPU tests compare the provider's kernels with the oracle.
 """
+import json
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _real_row():
+    """The reference's real-data 75-interferogram row (input of its
+    near-max refinement test, tests/test_stage2_ported.py:376-537)."""
+    d = json.loads((GOLDEN / "real_row_75ifg.json").read_text())
+    cpx = np.asarray(d["cpx_re"]) + 1j * np.asarray(d["cpx_im"])
+    return cpx, np.asarray(d["bperp"], dtype=np.float64), float(d["n_trial_wraps"])
 
 
 # --- rules of the Python-path topofit ------------------------------------------
@@ -79,6 +91,24 @@ def test_py_topofit_zero_elements_fit_the_subset(ora):
     np.testing.assert_array_equal(res[:, keep], res2)
     Kz, Cz, cz, rz = ora.py_topofit_rows(np.zeros((2, 5)), np.ones((2, 5)), 0.5)
     assert np.all(np.isnan(Kz)) and np.all(np.isnan(cz)) and not np.any(rz)
+
+
+def test_real_row_takes_the_near_max_refinement_path(ora):
+    """The real row has several near-max trials (the reference's test exists
+    for that path); the oracle's fit refines from one of them."""
+    from pystamps_amd.prepare import trial_values
+    cpx, bp, wraps = _real_row()
+    t = trial_values(wraps)
+    assert list(t) == list(range(-6, 7))
+    step = np.pi / 4 / (bp.max() - bp.min())
+    coh_t = np.abs(np.exp(-1j * np.outer(t, bp * step)) @ cpx) / np.abs(cpx).sum()
+    cand = ora.py_near_max_candidates(coh_t)
+    assert len(cand) >= 2
+    K, Cc, coh, res = ora.py_topofit_rows(cpx[None, :], bp[None, :], wraps)
+    assert 0.0 < coh[0] <= 1.0 and -np.pi <= Cc[0] <= np.pi
+    # the refined K stays within one trial step of a candidate's coarse K
+    assert min(abs(K[0] - step * t[c]) for c in cand) <= step
+    np.testing.assert_allclose(np.abs(res[0]), np.abs(cpx), atol=1e-6)   # a rotation (complex64 out)
 
 
 def test_py_grid_accumulate_matches_numpy_add_at(ora):
@@ -161,6 +191,22 @@ def test_gpu_topofit_matches_oracle(ora):
     np.testing.assert_allclose(Kr, Ko2, rtol=0, atol=1e-9, equal_nan=True)
     c_only = registry_hip.stage2_topofit_coh_row_invariant(cpx, sym, 0.7)
     np.testing.assert_array_equal(c_only, cr)
+
+
+@pytest.mark.gpu
+def test_gpu_topofit_real_row_matches_oracle(ora):
+    from pystamps_amd import registry_hip
+    cpx, bp, wraps = _real_row()
+    rows = np.tile(cpx, (3, 1))
+    bps = np.tile(bp, (3, 1))
+    K, Cc, coh, res = registry_hip.stage2_topofit(rows, bps, wraps)
+    Ko, Co, co, ro = ora.py_topofit_rows(rows, bps, wraps)
+    np.testing.assert_allclose(K, Ko, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(coh, co, rtol=0, atol=1e-12)
+    assert np.max(np.abs(np.angle(np.exp(1j * (Cc - Co))))) < 1e-10
+    Kr, Cr, cr, rr = registry_hip.stage2_topofit_row_invariant(rows, bp, wraps)
+    np.testing.assert_array_equal(Kr, K)
+    np.testing.assert_array_equal(cr, coh)
 
 
 @pytest.mark.gpu
