@@ -209,6 +209,44 @@ def test_fit_fp32_vs_ref32_and_ref64(S, ora, prep_c1):
     np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
 
 
+def test_fit_real_row_fp64_and_fp32(S, ora):
+    """The reference's real-data 75-interferogram row (tests/golden), fitted by
+    the native-path kernels: 48 pixels carry the row with different constant
+    phase offsets and noise levels, a unit filtered phase makes psdph the row
+    itself.  FP64 form vs ref64 exact; FP32 form vs ref32 / ref64."""
+    import json
+    from pathlib import Path
+    from pystamps_amd.prepare import trial_values
+    d = json.loads((Path(__file__).resolve().parent / "golden" / "real_row_75ifg.json").read_text())
+    row = np.asarray(d["cpx_re"]) + 1j * np.asarray(d["cpx_im"])
+    bp = np.asarray(d["bperp"])
+    wraps = float(d["n_trial_wraps"])
+    base = small_prep(n_ps=48, n_ifg=75, seed=0x5EED1800)
+    rng = np.random.default_rng(18)
+    P = base.n_ps
+    ph = row[None, :] * np.exp(1j * rng.uniform(-np.pi, np.pi, (P, 1)))
+    ph = ph * np.exp(1j * rng.normal(0.0, 1.0, (P, 1)) * rng.uniform(0.0, 0.6, (P, 75)))
+    ph_nm = (ph / np.abs(ph)).astype(np.complex64)
+    prep = dataclasses.replace(base, ph_nm=ph_nm, amp=np.ones((P, 75), np.float32),
+                               bperp_mat=np.tile(bp, (P, 1)), row_invariant_bperp=False, row_bperp_nm=bp,
+                               bperp_nm=bp,
+                               n_trial_wraps=wraps, trial_values=trial_values(wraps))
+    unit = np.ones((prep.n_i * prep.n_j, 75), np.complex128)
+    with S.Stage2Session(prep, form="fp64") as s:
+        o64 = s.fit(unit)
+    r64 = ora.fit_rows(prep, ora.extract_patch_phase(prep, unit, mode=ora.REF64), mode=ora.REF64)
+    assert np.array_equal(o64["trial_ix"], r64["trial_ix"])
+    np.testing.assert_allclose(o64["K_ps"], r64["K_ps"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(o64["coh_ps"], r64["coh_ps"], rtol=0, atol=1e-12)
+    with S.Stage2Session(prep, form="fp32") as s:
+        o32 = s.fit(unit.astype(np.complex64))
+    r32 = ora.fit_rows(prep, ora.extract_patch_phase(prep, unit, mode=ora.REF32), mode=ora.REF32)
+    assert np.array_equal(o32["trial_ix"], r64["trial_ix"])
+    for key in ("K_ps", "coh_ps"):
+        assert np.all(_close(o32[key], r32[key], F64_ATOL, F64_RTOL)), key
+    np.testing.assert_allclose(o32["coh_ps"], r64["coh_ps"], rtol=0, atol=1e-5)
+
+
 # --- whole loop ---------------------------------------------------------------------
 
 @pytest.fixture(scope="module")
