@@ -235,10 +235,12 @@ __global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict_
 // transposed through LDS.  The weighted window outputs of the segment are
 // summed per team in a rolling 24-column register accumulator; every column
 // block that no later window of the segment touches is flushed to `part`
-// (segment s in plane s % 2, so neighbouring segments, which overlap by 18
-// columns, never write the same element).  A second kernel adds, in a fixed
-// order, the (at most four) window rows and (at most two) segments covering
-// each cell.  No atomics, so the result is deterministic.
+// (segment s in plane s % nplanes: a segment's columns reach 18 past its
+// windows' start columns, so it overlaps the next ceil(3 / wseg) segments,
+// and nplanes = 1 + ceil(3 / wseg) keeps overlapping segments on different
+// planes).  A second kernel adds, in a fixed order, the (at most four) window
+// rows and (at most nplanes) segments covering each cell.  No atomics, so the
+// result is deterministic.
 
 template <typename R>
 struct ClapParams {
@@ -246,7 +248,8 @@ struct ClapParams {
     R alpha, beta;  // clap_alpha, clap_beta
     int alpha_is_one;
     int ni, nj, I, nwi, nwj, ngroups;
-    int nseg, wseg;   // window segments per row, windows per segment (>= 3)
+    int nseg, wseg;   // window segments per row, windows per segment
+    int nplanes;      // partial-sum planes per window row: 1 + ceil(3 / wseg)
     int units;        // strips: ngroups * nseg (interferogram pair x segment)
     int upx;          // strips per XCD: ceil(units / PS2_NUM_XCD)
 };
@@ -511,7 +514,7 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         for (int q = 0; q < CLAP_WIN / 2; ++q)
             wv[q] = ((R)(arow + q) + bias) * (R)(1.0 / (CLAP_FFT * CLAP_FFT));
     }
-    cplx<R> *dst = part + ((((int64_t)f * prm.nwi + ix) * 2 + (seg & 1)) * CLAP_WIN + lrc) * nj;
+    cplx<R> *dst = part + ((((int64_t)f * prm.nwi + ix) * prm.nplanes + seg % prm.nplanes) * CLAP_WIN + lrc) * nj;
 
     for (int jx = jx0; jx < jx1; ++jx) {
         // re-read the low-pass table (L1-resident) every window instead of
@@ -700,10 +703,10 @@ __device__ inline cplx<R> unit_phase(cplx<R> v) {
 // UNIT the cell's unit phase is stored instead: that is ph_patch of every
 // pixel in the cell (extract_patch_phase gathers then normalises each copy of
 // the same value, native_stage2.rs:1186-1196).
-template <typename R, bool UNIT>
+template <typename R, bool UNIT, int MAXSEG>
 __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
                                                       cplx<R> *__restrict__ filt, int ni, int nj,
-                                                      int I, int nwi, int nseg, int wseg,
+                                                      int I, int nwi, int nseg, int wseg, int nplanes,
     const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -715,30 +718,33 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
     int ix_hi = r / CLAP_INC;
     if (ix_hi > nwi - 1) ix_hi = nwi - 1;
     const int ix_lo = r >= CLAP_WIN - 1 ? (r - (CLAP_WIN - 1) + CLAP_INC - 1) / CLAP_INC : 0;
-    // segments whose flushed columns contain c: s1 and, inside its first 18
-    // columns, the previous one (the spans overlap by n_win - n_inc)
+    // segments whose columns [span s, span s + span + 18) contain c: s0..s1,
+    // at most nplanes <= 4 of them
     const int span = CLAP_INC * wseg;
     int s1 = c / span;
     if (s1 > nseg - 1) s1 = nseg - 1;
-    int s0 = s1;
-    if (s1 > 0) {
-        const int prev_last = s1 * wseg;   // first window of s1 = one past s1-1's last
-        if (c < CLAP_INC * prev_last + (CLAP_WIN - CLAP_INC)) s0 = s1 - 1;
+    int s0;
+    if constexpr (MAXSEG == 2) {   // wseg >= 3: s1, and s1 - 1 inside s1's first 18 columns
+        s0 = s1 > 0 && c < s1 * span + (CLAP_WIN - CLAP_INC) ? s1 - 1 : s1;
+    } else {
+        const int d = c - span - (CLAP_WIN - CLAP_INC);
+        s0 = d < 0 ? 0 : d / span + 1;
     }
-    // at most 4 window rows (ix_hi - ix_lo <= 3) x 2 segments: every load is
-    // issued before the first add, then the sum runs in the fixed order
-    // (ix ascending, segment ascending)
-    cplx<R> v[4][2];
-    bool ok[4][2];
+    // at most 4 window rows (ix_hi - ix_lo <= 3) x MAXSEG (>= nplanes)
+    // segments: every load is issued before the first add, then the sum runs
+    // in the fixed order (ix ascending, segment ascending)
+    cplx<R> v[4][MAXSEG];
+    bool ok[4][MAXSEG];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int ix = ix_lo + q;
         const int lr = r - ix * CLAP_INC;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < MAXSEG; ++t) {
             const int sg = s0 + t;
             ok[q][t] = ix <= ix_hi && sg <= s1;
-            v[q][t] = ok[q][t] ? part[((((int64_t)f * nwi + ix) * 2 + (sg & 1)) * CLAP_WIN + lr) * nj + c]
+            const int plane = MAXSEG == 2 ? (sg & 1) : sg % nplanes;
+            v[q][t] = ok[q][t] ? part[((((int64_t)f * nwi + ix) * nplanes + plane) * CLAP_WIN + lr) * nj + c]
                                : cplx<R>{(R)0, (R)0};
         }
     }
@@ -746,7 +752,7 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < MAXSEG; ++t)
             if (ok[q][t]) {
                 sre += v[q][t].re;
                 sim += v[q][t].im;

@@ -459,14 +459,40 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
     return PS2_OK;
 }
 
+// Window segments of the CLAP strips: each window row is split into nseg
+// segments of wseg windows until there are ~4 wavefronts per SIMD (1024
+// SIMDs).  Segment s covers columns [6 wseg s, 6 wseg (s + 1) + 18), so it
+// overlaps the next ceil(3 / wseg) segments; segments write their partial
+// sums into nplanes = 1 + ceil(3 / wseg) planes in turn (2 from 3 windows per
+// segment on), so no two overlapping segments share a plane.
+struct ClapSegs {
+    int nseg, wseg, nplanes;
+};
+
+constexpr int CLAP_TEAMS = 2;   // teams (interferograms) per CLAP wavefront
+
+static ClapSegs clap_segments(int nwi, int nwj, int ngroups) {
+    const int rows = std::max(1, nwi * ngroups);
+    int nseg = std::max(1, (4096 + rows - 1) / rows);
+    nseg = std::min(nseg, std::max(1, nwj));
+    ClapSegs c;
+    c.wseg = (nwj + nseg - 1) / nseg;
+    c.nseg = (nwj + c.wseg - 1) / c.wseg;
+    c.nplanes = 1 + (3 + c.wseg - 1) / c.wseg;
+    return c;
+}
+
 template <typename R>
 int alloc_state(ps2_session *s) {
     const int64_t P = s->P, I = s->I, C = s->C;
     int rc;
     if ((rc = s->grid.alloc(sizeof(cplx<R>) * C * I))) return rc;
     if ((rc = s->filt.alloc(sizeof(cplx<R>) * C * I))) return rc;
-    if (s->windows)
-        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * 2 * CLAP_WIN * s->nj * I))) return rc;
+    if (s->windows) {
+        const ClapSegs cs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS));
+        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * cs.nplanes * CLAP_WIN * s->nj * I)))
+            return rc;
+    }
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
     if (sizeof(R) == 4)
         if ((rc = s->wt.alloc(sizeof(cplx<R>) * (s->rowinv ? I : P * I)))) return rc;
@@ -601,14 +627,10 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
     p.nwi = s->nwi;
     p.nwj = s->nwj;
     p.ngroups = (int)((s->I + teams - 1) / teams);
-    // split window rows into segments until there are ~4 wavefronts per SIMD
-    // (1024 SIMDs); a segment keeps at least 3 windows so only neighbouring
-    // segments overlap
-    const int rows = std::max(1, s->nwi * p.ngroups);
-    int nseg = std::max(1, (4096 + rows - 1) / rows);
-    nseg = std::min(nseg, std::max(1, s->nwj / 3));
-    p.wseg = (s->nwj + nseg - 1) / nseg;
-    p.nseg = (s->nwj + p.wseg - 1) / p.wseg;
+    const ClapSegs cs = clap_segments(s->nwi, s->nwj, p.ngroups);
+    p.wseg = cs.wseg;
+    p.nseg = cs.nseg;
+    p.nplanes = cs.nplanes;
     p.units = p.ngroups * p.nseg;
     p.upx = (p.units + PS2_NUM_XCD - 1) / PS2_NUM_XCD;
     return p;
@@ -624,7 +646,7 @@ int launch_clap(ps2_session *s, bool unit) {
         PS2_LAUNCHED();
         return PS2_OK;
     }
-    constexpr int TEAMS = 2;  // one wavefront per workgroup
+    constexpr int TEAMS = CLAP_TEAMS;
     const ClapParams<R> prm = clap_params<R>(s, TEAMS);
     {
         SlotTimer tm(s, SLOT_CLAP_STRIP);
@@ -638,14 +660,15 @@ int launch_clap(ps2_session *s, bool unit) {
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
-    if (unit)
-        hipLaunchKernelGGL((k_clap_combine<R, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
-                           s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                           (int)s->I, s->nwi, prm.nseg, prm.wseg, s->gate);
+    auto combine = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, s->part.as<cplx<R>>(),
+                           s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj, (int)s->I, s->nwi, prm.nseg,
+                           prm.wseg, prm.nplanes, s->gate);
+    };
+    if (prm.nplanes <= 2)
+        unit ? combine(k_clap_combine<R, true, 2>) : combine(k_clap_combine<R, false, 2>);
     else
-        hipLaunchKernelGGL((k_clap_combine<R, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
-                           s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj,
-                           (int)s->I, s->nwi, prm.nseg, prm.wseg, s->gate);
+        unit ? combine(k_clap_combine<R, true, 4>) : combine(k_clap_combine<R, false, 4>);
     PS2_LAUNCHED();
     return PS2_OK;
 }
