@@ -66,3 +66,30 @@ def test_header_structs_match_ctypes_layout():
     pos = [body.find(n) for n in names]
     assert all(p >= 0 for p in pos), [n for n, p in zip(names, pos) if p < 0]
     assert pos == sorted(pos)
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda h: setattr(h.s, "n_ifg", 0), "n_ifg must be positive"),
+    (lambda h: h.grid_lin.__setitem__(7, h.s.n_i * h.s.n_j), "grid_lin[7] outside"),
+    (lambda h: h.grid_lin.__setitem__(0, -1), "grid_lin[0] outside"),
+    (lambda h: h.trial_values.__setitem__(1, 0.5), "consecutive integers"),
+    (lambda h: setattr(h.s, "n_trial", 4), "2*tn+1"),
+    (lambda h: setattr(h.s, "clap_window", 20), "clap_win = 32"),
+    (lambda h: (setattr(h.s, "row_invariant", 0), setattr(h.s, "bperp_mat", None)), "bperp_mat is required"),
+    (lambda h: (setattr(h.s, "weighting", _native.WEIGHT_SNR), setattr(h.s, "amp", None)), "amp is required"),
+    (lambda h: setattr(h.s, "d_a", None), "required input array is NULL"),
+])
+def test_patch_validation_messages_on_host(mutate, message):
+    """Inputs the library rejects before any device call, with the message."""
+    import dataclasses
+    from conftest import small_prep
+    from pystamps_amd.stage2 import _InHolder
+    prep = small_prep(n_ps=500, n_ifg=6, seed=0x5EED1900)
+    # private copies of the arrays a case edits (the prepared patch is cached)
+    h = _InHolder(dataclasses.replace(prep, grid_lin=prep.grid_lin.copy(),
+                                      trial_values=prep.trial_values.copy()))
+    mutate(h)
+    out = _native.PatchOut()
+    rc = _native.lib().ps2_hip_run(C.byref(h.s), C.byref(out), C.byref(_native.RunOpts()))
+    assert rc == -1
+    assert message in _native.last_error(), _native.last_error()
