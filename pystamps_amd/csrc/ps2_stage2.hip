@@ -569,9 +569,6 @@ void collect_timers(ps2_session *s) {
 
 // --- kernel launch helpers ---------------------------------------------------
 
-#ifndef PS2_FILL_CB
-#define PS2_FILL_CB 128
-#endif
 
 template <typename R>
 int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *Kprev = nullptr,
@@ -579,27 +576,41 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *K
     const int64_t n = s->C * s->I;
     SlotTimer tm(s, SLOT_FILL);
     if (sizeof(R) == 4) {
-        // 128 cells x 2 planes per workgroup (measured fastest of 32 x 8,
-        // 64 x 4, 128 x 2, 256 x 1 on a 500k x 60 patch); the LDS chunk is the
-        // smallest of 256 / 512 / 1024 pixels covering the mean pixel count of
-        // 128 cells (larger ranges take several chunks: LDS beyond 16 KB per
-        // workgroup costs more in occupancy than the extra passes)
-        constexpr int CB = PS2_FILL_CB;
-        const double mean = (double)s->P / (double)std::max<int64_t>(s->C, 1) * CB;
-        const int64_t ncg = blocks_for(s->C, CB);
-        const int npg = (int)blocks_for(s->I, 256 / CB);
-        const dim3 grid((unsigned)(blocks_for(ncg, PS2_NUM_XCD) * PS2_NUM_XCD * npg));
-        auto go = [&](auto kern) {
-            hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
-                               s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
-                               (int)s->I, s->P, npg, Kprev, Wprev, s->gate);
+        // 128 cells x 2 planes per workgroup below 12 pixels per cell, else
+        // 64 cells x 4 planes (measured: 128 x 2 fastest of 32 x 8, 64 x 4,
+        // 128 x 2, 256 x 1 at 500k x 60 = 5.6 pixels per cell; 64 x 4 6%
+        // faster at 4M x 120 = 25 per cell, where reading K and W once per 4
+        // planes pays).  The LDS chunk is the smallest of 256 / 512 / 1024
+        // pixels covering the mean pixel count of the workgroup's cells (larger
+        // ranges take several chunks: LDS beyond 16 KB per workgroup costs more
+        // in occupancy than the extra passes).
+        const double density = (double)s->P / (double)std::max<int64_t>(s->C, 1);
+        auto launch = [&](auto cb_tag) {
+            constexpr int CB = decltype(cb_tag)::value;
+            const double mean = density * CB;
+            const int64_t ncg = blocks_for(s->C, CB);
+            const int npg = (int)blocks_for(s->I, 256 / CB);
+            const dim3 grid((unsigned)(blocks_for(ncg, PS2_NUM_XCD) * PS2_NUM_XCD * npg));
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
+                                   s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
+                                   (int)s->I, s->P, npg, Kprev, Wprev, s->gate);
+            };
+            if (mean <= 256.0)
+                go(k_fill_grid_staged<R, CB, 256>);
+            else if (mean <= 512.0)
+                go(k_fill_grid_staged<R, CB, 512>);
+            else
+                go(k_fill_grid_staged<R, CB, 1024>);
         };
-        if (mean <= 256.0)
-            go(k_fill_grid_staged<R, CB, 256>);
-        else if (mean <= 512.0)
-            go(k_fill_grid_staged<R, CB, 512>);
+#ifdef PS2_FILL_CB
+        launch(std::integral_constant<int, PS2_FILL_CB>{});
+#else
+        if (density < 12.0)
+            launch(std::integral_constant<int, 128>{});
         else
-            go(k_fill_grid_staged<R, CB, 1024>);
+            launch(std::integral_constant<int, 64>{});
+#endif
     } else {
         hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
