@@ -116,6 +116,11 @@ __device__ constexpr int bitrev5(int v) {
     return ((v & 1) << 4) | ((v & 2) << 2) | (v & 4) | ((v & 8) >> 2) | ((v & 16) >> 4);
 }
 
+// FP32 butterflies with a general twiddle use the fused form below; the FP64
+// form keeps the separately rounded product and sums.
+template <typename R>
+constexpr bool FMA_BUTTERFLY = sizeof(R) == 4;
+
 // 32-point radix-2 decimation-in-time FFT held in registers, unnormalised:
 // forward X[k] = sum x[n] e^{-2 pi i nk/32}, inverse with e^{+...} (the
 // rustfft conventions of native_stage2.rs:1055-1095).
@@ -140,6 +145,23 @@ __device__ inline void fft32(R (&xr)[32], R (&xi)[32]) {
                 const int tw = j * step;
                 R br = ar[b + j + half], bi = ai[b + j + half];
                 R vr, vi;
+                if constexpr (FMA_BUTTERFLY<R>) {
+                    if (tw != 0 && tw != 8 && !(s == 1 && bitrev5(b + j + half) >= NZ)) {
+                        // x = u + w b in four fused steps, u - w b = 2u - x in
+                        // one each: 6 instructions per butterfly instead of 8
+                        const R wr = twr<R>(tw);
+                        const R wi = INV ? -twi_fwd<R>(tw) : twi_fwd<R>(tw);
+                        const R ur = ar[b + j], ui = ai[b + j];
+                        const R xr = fma(-wi, bi, fma(wr, br, ur));
+                        const R xi = fma(wi, br, fma(wr, bi, ui));
+                        ar[b + j] = xr;
+                        ai[b + j] = xi;
+                        if (s == 5 && b + j + half >= NOUT) continue;
+                        ar[b + j + half] = fma((R)2, ur, -xr);
+                        ai[b + j + half] = fma((R)2, ui, -xi);
+                        continue;
+                    }
+                }
                 if (tw == 0) {
                     vr = br;
                     vi = bi;
