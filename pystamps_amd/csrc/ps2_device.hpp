@@ -19,6 +19,67 @@ struct cplx {
     R re, im;
 };
 
+// FP32 sine and cosine of the per-element rotation angles.  x = q pi/2 + r with
+// q = rint(x 2/pi) and pi/2 split into three floats (their sum is pi/2 to
+// ~2^-74), subtracted by fused multiply-adds, so r keeps its own relative
+// precision for |x| <= 8192.  sin r = r + r s (S1 + s S2 + s^2 S3) and
+// cos r = 1 + s (C1 + ... + s^3 C4), s = r^2, with coefficients fitted to sin
+// and cos on |r| <= 1.002 pi/4 by reweighted least squares (Lawson's
+// minimax iteration); q mod 4 swaps and negates.  Evaluated in float
+// arithmetic over 4M arguments in |x| <= 8192, both stay within 1.5 ulp of
+// the float64 values.  Wider arguments and non-finite ones go to sincosf.
+__device__ inline void sincos_fit(float x, float *sp, float *cp) {
+    if (__builtin_expect(!(__builtin_fabsf(x) <= 8192.0f), 0)) {
+        sincosf(x, sp, cp);
+        return;
+    }
+    const float q = __builtin_rintf(x * 0x1.45f306p-1f);
+    float r = __builtin_fmaf(-q, 0x1.921fb6p+0f, x);
+    r = __builtin_fmaf(-q, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-q, -0x1.ee59dap-50f, r);
+    const float s = r * r;
+    const float ps = __builtin_fmaf(__builtin_fmaf(-0x1.993ce4p-13f, s, 0x1.110726p-7f), s, -0x1.555544p-3f);
+    const float sn = __builtin_fmaf(r * s, ps, r);
+    const float pc = __builtin_fmaf(
+        __builtin_fmaf(__builtin_fmaf(0x1.990f5ep-16f, s, -0x1.6c07d2p-10f), s, 0x1.55553cp-5f), s, -0.5f);
+    const float cn = __builtin_fmaf(s, pc, 1.0f);
+    const int qi = (int)q;
+    const bool odd = (qi & 1) != 0;
+    // sin(q pi/2 + r): sin r, cos r, -sin r, -cos r for q mod 4 = 0..3;
+    // cos(q pi/2 + r): cos r, -sin r, -cos r, sin r
+    const float S = odd ? cn : sn, Cv = odd ? sn : cn;
+    *sp = __int_as_float(__float_as_int(S) ^ ((qi & 2) << 30));
+    *cp = __int_as_float(__float_as_int(Cv) ^ (((qi + 1) & 2) << 30));
+}
+
+// FP32 atan2 for finite arguments ((0, 0) gives 0 with the sign of y).
+// t = min(|x|,|y|) / max(|x|,|y|) in [0, 1] by the hardware reciprocal,
+// atan t = t + t s P(s), s = t^2, P of degree 7 fitted like sincos_fit
+// (within 8e-8 of atan on [0, 1] in float arithmetic), then pi/2 - a when
+// |y| > |x|, pi - a when x < 0, and the sign of y.  Within 3e-7 rad (about
+// one ulp of pi) of the float64 atan2 over 4M arguments of magnitude
+// e^-20..e^5.
+__device__ inline float atan2_fit(float y, float x) {
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const float mx = __builtin_fmaxf(__builtin_fmaxf(ax, ay), 1e-30f);
+    const float mn = __builtin_fminf(ax, ay);
+    const float t = mn * __builtin_amdgcn_rcpf(mx);
+    const float s = t * t;
+    float p = 0x1.57b160p-9f;
+    p = __builtin_fmaf(p, s, -0x1.efda4ep-7f);
+    p = __builtin_fmaf(p, s, 0x1.50ddb2p-5f);
+    p = __builtin_fmaf(p, s, -0x1.2dbd08p-4f);
+    p = __builtin_fmaf(p, s, 0x1.b11b7ap-4f);
+    p = __builtin_fmaf(p, s, -0x1.228756p-3f);
+    p = __builtin_fmaf(p, s, 0x1.996740p-3f);
+    p = __builtin_fmaf(p, s, -0x1.55546cp-2f);
+    float a = __builtin_fmaf(t * s, p, t);
+    if (ay > ax) a = 0x1.921fb6p+0f - a;
+    if (x < 0.0f) a = 0x1.921fb6p+1f - a;
+    const float xy = x + y;  // NaN in, NaN out (fmaxf/fminf drop NaNs)
+    return xy != xy ? xy : __builtin_copysignf(a, y);
+}
+
 // Real-type traits: the FP32 form runs on float, the plain FP64 form on double.
 template <typename R>
 struct RT;
@@ -28,8 +89,8 @@ struct RT<float> {
     using key_t = uint32_t;
     static constexpr int key_top_bit = 30;  // keys are of non-negative floats
     __device__ static inline float pi4() { return 0.785398185253143310546875f; }  // f32(pi)/4
-    __device__ static inline void sincos_(float x, float *s, float *c) { sincosf(x, s, c); }
-    __device__ static inline float atan2_(float y, float x) { return atan2f(y, x); }
+    __device__ static inline void sincos_(float x, float *s, float *c) { sincos_fit(x, s, c); }
+    __device__ static inline float atan2_(float y, float x) { return atan2_fit(y, x); }
     __device__ static inline float sqrt_(float x) { return sqrtf(x); }
     __device__ static inline float pow_(float x, float y) { return powf(x, y); }
     // glibc hypotf: sqrt of the f64 sum of f64 squares, rounded once to f32

@@ -11,8 +11,9 @@
 //   phnm  [I][P] complex R   ph_nm in sorted pixel order
 //   bp    [I][P] R           bperp_mat (or [I] when the rows are invariant)
 //   grid, filt [I][C] complex R
-//   part  [I][nwi][2][24][n_j] complex R  per-window-row CLAP partial sums,
-//         alternate window segments in alternate planes
+//   part  [I][nwi][nplanes][4][n_j][6] complex R  per-window-row CLAP
+//         partial sums in groups of 6 window rows (one window step), each
+//         group column-major; overlapping window segments in different planes
 #pragma once
 
 #include "ps2_device.hpp"
@@ -514,7 +515,10 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         for (int q = 0; q < CLAP_WIN / 2; ++q)
             wv[q] = ((R)(arow + q) + bias) * (R)(1.0 / (CLAP_FFT * CLAP_FFT));
     }
-    cplx<R> *dst = part + ((((int64_t)f * prm.nwi + ix) * prm.nplanes + seg % prm.nplanes) * CLAP_WIN + lrc) * nj;
+    // plane of 6-row groups, each column-major: a flush of 6 columns writes one
+    // contiguous run of 36 entries per group
+    cplx<R> *dst = part + (((int64_t)f * prm.nwi + ix) * prm.nplanes + seg % prm.nplanes) * (int64_t)nj * CLAP_WIN +
+                   (int64_t)(lrc / CLAP_INC) * nj * CLAP_INC + lrc % CLAP_INC;
 
     for (int jx = jx0; jx < jx1; ++jx) {
         // re-read the low-pass table (L1-resident) every window instead of
@@ -670,11 +674,11 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
         if (out_row) {
             if (!last) {
 #pragma unroll
-                for (int e = 0; e < CLAP_INC; ++e) dst[cbeg + e] = {ar[e], ai[e]};
+                for (int e = 0; e < CLAP_INC; ++e) dst[(int64_t)(cbeg + e) * CLAP_INC] = {ar[e], ai[e]};
             } else {
 #pragma unroll
                 for (int e = 0; e < CLAP_WIN; ++e)
-                    if (cbeg + e < nj) dst[cbeg + e] = {ar[e], ai[e]};
+                    if (cbeg + e < nj) dst[(int64_t)(cbeg + e) * CLAP_INC] = {ar[e], ai[e]};
             }
         }
 #pragma unroll
@@ -702,62 +706,97 @@ __device__ inline cplx<R> unit_phase(cplx<R> v) {
 // Sum of the window rows covering each cell (ascending row order).  With
 // UNIT the cell's unit phase is stored instead: that is ph_patch of every
 // pixel in the cell (extract_patch_phase gathers then normalises each copy of
-// the same value, native_stage2.rs:1186-1196).
+// the same value, native_stage2.rs:1186-1196).  Workgroup = a tile of
+// COMB_TR = 24 rows [24 m, 24 m + 24) x COMB_TC columns of one interferogram;
+// every 6-row group of the tile lies in one 6-row group of each of the 4
+// window rows over it, a contiguous run of 6 x COMB_TC entries (the planes
+// hold 6-row groups column-major), and lanes take (group row, column, group)
+// fastest first, so each load reads whole runs.  Each cell adds its (at most
+// 4 x nplanes) entries in the fixed order window row ascending, segment
+// ascending; the sums go through an LDS tile and are stored row-major.
+// Grid: tiles x I, the interferogram slowest.
+constexpr int COMB_TR = 4 * CLAP_INC, COMB_TC = 32;
 template <typename R, bool UNIT, int MAXSEG>
 __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
                                                       cplx<R> *__restrict__ filt, int ni, int nj,
                                                       int I, int nwi, int nseg, int wseg, int nplanes,
     const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    static_assert(COMB_TR * COMB_TC % 256 == 0, "whole tile per pass");
+    constexpr int NIX = COMB_TR / CLAP_INC + CLAP_WIN / CLAP_INC - 1;   // window rows meeting a tile
+    __shared__ cplx<R> tile[COMB_TC][COMB_TR + 1];   // [column][row]
+    const int tj_n = (nj + COMB_TC - 1) / COMB_TC;
+    const int tiles = ((ni + COMB_TR - 1) / COMB_TR) * tj_n;
+    const int f = (int)(blockIdx.x / tiles), tl = (int)(blockIdx.x - (unsigned)f * tiles);
+    const int r0 = (tl / tj_n) * COMB_TR, c0 = (tl % tj_n) * COMB_TC;
+    const int ixa = r0 / CLAP_INC - (CLAP_WIN / CLAP_INC - 1);   // first window row meeting the tile
     const int64_t C = (int64_t)ni * nj;
-    if (gid >= C * I) return;
-    const int f = (int)(gid / C);
-    const int64_t cell = gid - (int64_t)f * C;
-    const int r = (int)(cell / nj), c = (int)(cell - (int64_t)r * nj);
-    int ix_hi = r / CLAP_INC;
-    if (ix_hi > nwi - 1) ix_hi = nwi - 1;
-    const int ix_lo = r >= CLAP_WIN - 1 ? (r - (CLAP_WIN - 1) + CLAP_INC - 1) / CLAP_INC : 0;
+    const cplx<R> *pf = part + (int64_t)f * nwi * nplanes * nj * CLAP_WIN;
     // segments whose columns [span s, span s + span + 18) contain c: s0..s1,
     // at most nplanes <= 4 of them
     const int span = CLAP_INC * wseg;
-    int s1 = c / span;
-    if (s1 > nseg - 1) s1 = nseg - 1;
-    int s0;
-    if constexpr (MAXSEG == 2) {   // wseg >= 3: s1, and s1 - 1 inside s1's first 18 columns
-        s0 = s1 > 0 && c < s1 * span + (CLAP_WIN - CLAP_INC) ? s1 - 1 : s1;
-    } else {
-        const int d = c - span - (CLAP_WIN - CLAP_INC);
-        s0 = d < 0 ? 0 : d / span + 1;
-    }
-    // at most 4 window rows (ix_hi - ix_lo <= 3) x MAXSEG (>= nplanes)
-    // segments: every load is issued before the first add, then the sum runs
-    // in the fixed order (ix ascending, segment ascending)
-    cplx<R> v[4][MAXSEG];
-    bool ok[4][MAXSEG];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int ix = ix_lo + q;
-        const int lr = r - ix * CLAP_INC;
-#pragma unroll
-        for (int t = 0; t < MAXSEG; ++t) {
-            const int sg = s0 + t;
-            ok[q][t] = ix <= ix_hi && sg <= s1;
-            const int plane = MAXSEG == 2 ? (sg & 1) : sg % nplanes;
-            v[q][t] = ok[q][t] ? part[((((int64_t)f * nwi + ix) * nplanes + plane) * CLAP_WIN + lr) * nj + c]
-                               : cplx<R>{(R)0, (R)0};
-        }
-    }
-    R sre = (R)0, sim = (R)0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int t = 0; t < MAXSEG; ++t)
-            if (ok[q][t]) {
-                sre += v[q][t].re;
-                sim += v[q][t].im;
+    for (int h = 0; h < COMB_TR * COMB_TC / 256; ++h) {
+        const int e = (int)threadIdx.x + 256 * h;
+        const int r6 = e % CLAP_INC, rest = e / CLAP_INC;
+        const int cl = rest % COMB_TC, rl = (rest / COMB_TC) * CLAP_INC + r6;
+        const int r = r0 + rl, c = c0 + cl;
+        cplx<R> out{(R)0, (R)0};
+        if (r < ni && c < nj) {
+            int ix_hi = r / CLAP_INC;
+            if (ix_hi > nwi - 1) ix_hi = nwi - 1;
+            const int ix_lo = r >= CLAP_WIN - 1 ? (r - (CLAP_WIN - 1) + CLAP_INC - 1) / CLAP_INC : 0;
+            int s1 = c / span;
+            if (s1 > nseg - 1) s1 = nseg - 1;
+            int s0;
+            if constexpr (MAXSEG == 2) {   // wseg >= 3: s1, and s1 - 1 inside s1's first 18 columns
+                s0 = s1 > 0 && c < s1 * span + (CLAP_WIN - CLAP_INC) ? s1 - 1 : s1;
+            } else {
+                const int d = c - span - (CLAP_WIN - CLAP_INC);
+                s0 = d < 0 ? 0 : d / span + 1;
             }
-    filt[gid] = UNIT ? unit_phase<R>({sre, sim}) : cplx<R>{sre, sim};
+            // the cell's window rows ix_lo..ix_hi (at most 4) are among the
+            // tile's NIX; every load is issued before the first add
+            cplx<R> v[NIX][MAXSEG];
+            bool ok[NIX][MAXSEG];
+#pragma unroll
+            for (int q = 0; q < NIX; ++q) {
+                const int ix = ixa + q;
+                const int lr = r - ix * CLAP_INC;
+#pragma unroll
+                for (int t = 0; t < MAXSEG; ++t) {
+                    const int sg = s0 + t;
+                    ok[q][t] = ix >= ix_lo && ix <= ix_hi && sg <= s1;
+                    const int plane = MAXSEG == 2 ? (sg & 1) : sg % nplanes;
+                    // (an address inside the planes also where the entry is not used)
+                    const int64_t at =
+                        ok[q][t] ? ((((int64_t)ix * nplanes + plane) * (CLAP_WIN / CLAP_INC) + lr / CLAP_INC) * nj + c) *
+                                           CLAP_INC + r6
+                                 : 0;
+                    v[q][t] = ok[q][t] ? pf[at] : cplx<R>{(R)0, (R)0};
+                }
+            }
+            R sre = (R)0, sim = (R)0;
+#pragma unroll
+            for (int q = 0; q < NIX; ++q)
+#pragma unroll
+                for (int t = 0; t < MAXSEG; ++t)
+                    if (ok[q][t]) {
+                        sre += v[q][t].re;
+                        sim += v[q][t].im;
+                    }
+            out = UNIT ? unit_phase<R>({sre, sim}) : cplx<R>{sre, sim};
+        }
+        tile[cl][rl] = out;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < COMB_TR * COMB_TC / 256; ++h) {
+        const int e = (int)threadIdx.x + 256 * h;
+        const int rl = e / COMB_TC, cl = e - rl * COMB_TC;
+        const int r = r0 + rl, c = c0 + cl;
+        if (r < ni && c < nj) filt[(int64_t)f * C + (int64_t)r * nj + c] = tile[cl][rl];
+    }
 }
 
 template <typename R>
@@ -1326,8 +1365,9 @@ __device__ inline int hist_bin(double v, const double *__restrict__ mids, HistPa
 
 // refine_candidate (native_stage2.rs:1259-1303) of row k from the selected
 // trial: offset sum, f64 least-squares step mopt, final residual sum.  Writes
-// K, C, coh and N_opt of the row and returns its coh.
-template <typename R>
+// K, C, coh and N_opt of the row and returns its coh.  ROWINV: one bperp row
+// shared by every pixel.
+template <typename R, bool ROWINV>
 __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm, const R *__restrict__ bp,
                                     const cplx<R> *__restrict__ filt, const int32_t *__restrict__ cell_of,
                                     const int32_t *__restrict__ tix, const R *__restrict__ rng,
@@ -1346,7 +1386,7 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     const int64_t P = prm.P, C = prm.C;
     const cplx<R> *nm = phnm + k;
     const cplx<R> *fl = filt + cell_of[k];
-    auto bat = [&](int i) { return prm.rowinv ? bp[i] : bp[(int64_t)i * P + k]; };
+    auto bat = [&](int i) { return ROWINV ? bp[i] : bp[(int64_t)i * P + k]; };
     const R range = rng[k];
     R k0 = (R)(RT<R>::pi4() / range * (R)(t - prm.tn));
     // offset = sum psdph e^{-i k0 bp} (:1261-1267) is the trial scan's sum at
@@ -1358,6 +1398,7 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     // current one
     cplx<R> cf = fl[0], cn = nm[0];
     R cb = bat(0);
+#pragma unroll 2
     for (int i = 0; i < I; ++i) {
         const int i1 = i + 1 < I ? i + 1 : i;
         const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P];
@@ -1380,6 +1421,7 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     cf = fl[0];
     cn = nm[0];
     cb = bat(0);
+#pragma unroll 2
     for (int i = 0; i < I; ++i) {
         const int i1 = i + 1 < I ? i + 1 : i;
         const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P];
@@ -1415,6 +1457,32 @@ struct RmsHistOut {
     HistParams hp;
 };
 
+// The workgroup's share of that step; `mine` marks the threads that own a
+// row (one per row).  Every thread of the workgroup must call it.
+__device__ inline void rms_hist_tail(const RmsHistOut &ro, bool mine, int64_t k, double coh) {
+    __shared__ double red[256];
+    __shared__ uint32_t lh[128];
+    if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
+    __syncthreads();
+    double d2 = 0.0;
+    if (mine) {
+        const double d = coh - ro.coh_prev[k];
+        d2 = d * d;
+        ro.coh_prev[k] = coh;
+        const int bin = hist_bin(coh, ro.mids, ro.hp);
+        if (bin >= 0) atomicAdd(&lh[bin], 1u);
+    }
+    red[threadIdx.x] = d2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ro.part[blockIdx.x] = red[0];
+    for (int q = threadIdx.x; q < ro.hp.nbins; q += 256)
+        if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm,
                                                 const R *__restrict__ bp,
@@ -1432,29 +1500,13 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool row = k < prm.P;
     double coh = 0.0;
-    if (row) coh = refine_row<R>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm);
-    if (!ro.coh_prev) return;
-    __shared__ double red[256];
-    __shared__ uint32_t lh[128];
-    if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
-    __syncthreads();
-    double d2 = 0.0;
     if (row) {
-        const double d = coh - ro.coh_prev[k];
-        d2 = d * d;
-        ro.coh_prev[k] = coh;
-        const int bin = hist_bin(coh, ro.mids, ro.hp);
-        if (bin >= 0) atomicAdd(&lh[bin], 1u);
+        if (prm.rowinv)
+            coh = refine_row<R, true>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm);
+        else
+            coh = refine_row<R, false>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm);
     }
-    red[threadIdx.x] = d2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) ro.part[blockIdx.x] = red[0];
-    for (int q = threadIdx.x; q < ro.hp.nbins; q += 256)
-        if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
+    if (ro.coh_prev) rms_hist_tail(ro, row, k, coh);
 }
 
 // ph_patch (unit filt) for every row and ph_res = arg(residual) of valid rows,

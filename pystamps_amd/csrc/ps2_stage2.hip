@@ -672,7 +672,8 @@ int launch_clap(ps2_session *s, bool unit) {
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
     auto combine = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, s->part.as<cplx<R>>(),
+        const dim3 g(blocks_for(s->ni, COMB_TR) * blocks_for(s->nj, COMB_TC) * (unsigned)s->I);
+        hipLaunchKernelGGL(kern, g, dim3(256), 0, s->stream, s->part.as<cplx<R>>(),
                            s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj, (int)s->I, s->nwi, prm.nseg,
                            prm.wseg, prm.nplanes, s->gate);
     };
