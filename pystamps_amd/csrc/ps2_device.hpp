@@ -28,6 +28,8 @@ struct cplx {
 // minimax iteration); q mod 4 swaps and negates.  Evaluated in float
 // arithmetic over 4M arguments in |x| <= 8192, both stay within 1.5 ulp of
 // the float64 values.  Wider arguments and non-finite ones go to sincosf.
+// sin(-0) comes out +0 (the reduction adds +0); the rotations that use it
+// sum the products with other terms, so no result's sign depends on it.
 __device__ inline void sincos_fit(float x, float *sp, float *cp) {
     if (__builtin_expect(!(__builtin_fabsf(x) <= 8192.0f), 0)) {
         sincosf(x, sp, cp);
