@@ -707,11 +707,11 @@ __device__ inline cplx<R> unit_phase(cplx<R> v) {
 // UNIT the cell's unit phase is stored instead: that is ph_patch of every
 // pixel in the cell (extract_patch_phase gathers then normalises each copy of
 // the same value, native_stage2.rs:1186-1196).  Workgroup = a tile of
-// COMB_TR = 24 rows [24 m, 24 m + 24) x COMB_TC columns of one interferogram;
-// every 6-row group of the tile lies in one 6-row group of each of the 4
-// window rows over it, a contiguous run of 6 x COMB_TC entries (the planes
-// hold 6-row groups column-major), and lanes take (group row, column, group)
-// fastest first, so each load reads whole runs.  Each cell adds its (at most
+// COMB_TR = 24 rows x COMB_TC columns of one interferogram; every 6-row group
+// of the tile lies in one 6-row group of each of the (at most 4) window rows
+// over it, a contiguous run of 6 x COMB_TC entries (the planes hold 6-row
+// groups column-major), and lanes take (group row, column, group) fastest
+// first, so each load reads whole runs.  Each cell adds its (at most
 // 4 x nplanes) entries in the fixed order window row ascending, segment
 // ascending; the sums go through an LDS tile and are stored row-major.
 // Grid: tiles x I, the interferogram slowest.
@@ -723,13 +723,11 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
     const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
     static_assert(COMB_TR * COMB_TC % 256 == 0, "whole tile per pass");
-    constexpr int NIX = COMB_TR / CLAP_INC + CLAP_WIN / CLAP_INC - 1;   // window rows meeting a tile
     __shared__ cplx<R> tile[COMB_TC][COMB_TR + 1];   // [column][row]
     const int tj_n = (nj + COMB_TC - 1) / COMB_TC;
     const int tiles = ((ni + COMB_TR - 1) / COMB_TR) * tj_n;
     const int f = (int)(blockIdx.x / tiles), tl = (int)(blockIdx.x - (unsigned)f * tiles);
     const int r0 = (tl / tj_n) * COMB_TR, c0 = (tl % tj_n) * COMB_TC;
-    const int ixa = r0 / CLAP_INC - (CLAP_WIN / CLAP_INC - 1);   // first window row meeting the tile
     const int64_t C = (int64_t)ni * nj;
     const cplx<R> *pf = part + (int64_t)f * nwi * nplanes * nj * CLAP_WIN;
     // segments whose columns [span s, span s + span + 18) contain c: s0..s1,
@@ -755,18 +753,20 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
                 const int d = c - span - (CLAP_WIN - CLAP_INC);
                 s0 = d < 0 ? 0 : d / span + 1;
             }
-            // the cell's window rows ix_lo..ix_hi (at most 4) are among the
-            // tile's NIX; every load is issued before the first add
-            cplx<R> v[NIX][MAXSEG];
-            bool ok[NIX][MAXSEG];
+            // at most 4 window rows (ix_hi - ix_lo <= 3) x MAXSEG (>= nplanes)
+            // segments; the 6 rows of a group share ix_lo, so for each q the
+            // group's lanes read one run.  Every load is issued before the
+            // first add.
+            cplx<R> v[4][MAXSEG];
+            bool ok[4][MAXSEG];
 #pragma unroll
-            for (int q = 0; q < NIX; ++q) {
-                const int ix = ixa + q;
+            for (int q = 0; q < 4; ++q) {
+                const int ix = ix_lo + q;
                 const int lr = r - ix * CLAP_INC;
 #pragma unroll
                 for (int t = 0; t < MAXSEG; ++t) {
                     const int sg = s0 + t;
-                    ok[q][t] = ix >= ix_lo && ix <= ix_hi && sg <= s1;
+                    ok[q][t] = ix <= ix_hi && sg <= s1;
                     const int plane = MAXSEG == 2 ? (sg & 1) : sg % nplanes;
                     // (an address inside the planes also where the entry is not used)
                     const int64_t at =
@@ -778,7 +778,7 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
             }
             R sre = (R)0, sim = (R)0;
 #pragma unroll
-            for (int q = 0; q < NIX; ++q)
+            for (int q = 0; q < 4; ++q)
 #pragma unroll
                 for (int t = 0; t < MAXSEG; ++t)
                     if (ok[q][t]) {
@@ -827,6 +827,7 @@ __device__ inline cplx<R> rotate_neg(cplx<R> v, R phase) {
 struct FitParams {
     int64_t P, C;
     int I, tn, T, rowinv;
+    int step_table;       // FP32 scan: unit steps from the session table (else from bperp)
     float rescore_bound;  // FP32 form only
 };
 
@@ -992,8 +993,8 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     const cplx<R> *nm = phnm + k;                      // nm[i * P]
     const cplx<R> *fl = filt + cell_of[k];             // fl[i * C]
     const R *b = prm.rowinv ? bp : bp + k;
-    const int64_t bs = prm.rowinv ? 0 : P;            // b[i * bs] (rowinv: b[i])
-    const cplx<R> *w1 = wt ? (prm.rowinv ? wt : wt + k) : nullptr;  // FP32: w1[i * bs]
+    const int64_t bs = prm.rowinv ? 1 : P;            // b[i * bs]
+    const cplx<R> *w1 = wt ? (prm.rowinv ? wt : wt + k) : nullptr;  // FP32 table: w1[i * bs]
     const int64_t C = prm.C;
     const R range = rng[k];
     const R pi4 = RT<R>::pi4();
@@ -1024,42 +1025,85 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
             // one chunk: trials 0..tn and their negatives.  Trial 0 adds v
             // itself and the recurrence starts at z = e^{-i phi}, which is what
             // the general loop computes from z = 1 (exact multiplies by 1, 0).
-            // The next interferogram's three loads are issued before the
-            // current one is processed.
+            // The unit step e^{-i phi} comes from the session's table or, where
+            // the host found the scan bound by HBM bandwidth (8-trial chunks
+            // over large patches, per-pixel bperp), is computed from the
+            // pixel's bperp: 4 bytes read instead of 8, with the same f32
+            // operations as the table, so the same bits.  The 8-trial loop
+            // keeps the loads of the next two interferograms in flight.
             R pre[TC], pim[TC], nre[TC], nim[TC];
 #pragma unroll
             for (int q = 0; q < TC; ++q) pre[q] = pim[q] = nre[q] = nim[q] = (R)0;
-            cplx<R> cf = fl[0], cn = nm[0], ce = w1[0];
-            for (int i = 0; i < I; ++i) {
-                const int i1 = i + 1 < I ? i + 1 : i;
-                const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P],
-                              ne = w1[(int64_t)i1 * bs];
-                const cplx<R> v = psdph_value<R>(cf, cn);
-                if (v.re == (R)0 && v.im == (R)0) valid = false;
-                denom += RT<R>::abs_(v.re, v.im);
-                pre[0] += v.re;
-                pim[0] += v.im;
-                const R wr = ce.re, wi = ce.im;
-                R zr = wr, zi = wi;
-#pragma unroll
-                for (int q = 1; q < TC; ++q) {
-                    if (q <= tn) {
-                        // A_q = sum v Re(z^q) in (pre, pim), B_q = sum v Im(z^q)
-                        // in (nre, nim): both trial sums v z^q = A + iB and
-                        // v conj(z)^q = A - iB from four FMAs
-                        pre[q] = fmaf(v.re, zr, pre[q]);
-                        pim[q] = fmaf(v.im, zr, pim[q]);
-                        nre[q] = fmaf(v.re, zi, nre[q]);
-                        nim[q] = fmaf(v.im, zi, nim[q]);
-                        const R nzr = fmaf(zr, wr, -zi * wi);
-                        const R nzi = fmaf(zr, wi, zi * wr);
-                        zr = nzr;
-                        zi = nzi;
+            auto run = [&](auto table) {
+                constexpr bool TABLE = decltype(table)::value;
+                using Src = typename std::conditional<TABLE, cplx<R>, R>::type;
+                auto load = [&](int i) -> Src {
+                    if constexpr (TABLE)
+                        return w1[(int64_t)i * bs];
+                    else
+                        return b[(int64_t)i * bs];
+                };
+                auto step = [&](Src e) -> cplx<R> {
+                    if constexpr (TABLE) {
+                        return e;
+                    } else {
+                        R sn, cs;
+                        RT<R>::sincos_((e / range) * pi4, &sn, &cs);
+                        return {cs, -sn};
                     }
+                };
+                const int i1 = I > 1 ? 1 : 0;
+                cplx<R> cf = fl[0], cn = nm[0];
+                Src ce = load(0);
+                cplx<R> mf = fl[(int64_t)i1 * C], mn = nm[(int64_t)i1 * P];
+                Src me = load(i1);
+                // (8 trials) unrolled by the period of the register rotation,
+                // so the copies below become renaming: a copy of a register
+                // that a load in flight targets would wait for that load
+#pragma unroll(TC == 8 ? 3 : 1)
+                for (int i = 0; i < I; ++i) {
+                    const int i2 = i + 2 < I ? i + 2 : I - 1;
+                    const cplx<R> nf = fl[(int64_t)i2 * C], nn = nm[(int64_t)i2 * P];
+                    const Src ne = load(i2);
+                    const cplx<R> v = psdph_value<R>(cf, cn);
+                    if (v.re == (R)0 && v.im == (R)0) valid = false;
+                    denom += RT<R>::abs_(v.re, v.im);
+                    pre[0] += v.re;
+                    pim[0] += v.im;
+                    const cplx<R> w = step(ce);
+                    const R wr = w.re, wi = w.im;
+                    R zr = wr, zi = wi;
+#pragma unroll
+                    for (int q = 1; q < TC; ++q) {
+                        if (q <= tn) {
+                            // A_q = sum v Re(z^q) in (pre, pim), B_q = sum v
+                            // Im(z^q) in (nre, nim): both trial sums v z^q =
+                            // A + iB and v conj(z)^q = A - iB from four FMAs
+                            pre[q] = fmaf(v.re, zr, pre[q]);
+                            pim[q] = fmaf(v.im, zr, pim[q]);
+                            nre[q] = fmaf(v.re, zi, nre[q]);
+                            nim[q] = fmaf(v.im, zi, nim[q]);
+                            const R nzr = fmaf(zr, wr, -zi * wi);
+                            const R nzi = fmaf(zr, wi, zi * wr);
+                            zr = nzr;
+                            zi = nzi;
+                        }
+                    }
+                    cf = mf;
+                    cn = mn;
+                    ce = me;
+                    mf = nf;
+                    mn = nn;
+                    me = ne;
                 }
-                cf = nf;
-                cn = nn;
-                ce = ne;
+            };
+            if constexpr (TC == 8) {
+                if (prm.step_table)
+                    run(std::true_type{});
+                else
+                    run(std::false_type{});
+            } else {
+                run(std::true_type{});
             }
             if (!valid) {
                 tix[k] = -1;
@@ -1103,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                     zr = (R)1;
                     zi = (R)0;
                 } else {
-                    const R phi = ((prm.rowinv ? b[i] : b[i * bs]) / range) * pi4;
+                    const R phi = (b[i * bs] / range) * pi4;
                     R sa, ca;
                     RT<R>::sincos_(phi * (R)t0, &sa, &ca);
                     zr = ca;
@@ -1124,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                     }
                 }
             } else {
-                const R phi = ((prm.rowinv ? b[i] : b[i * bs]) / range) * pi4;
+                const R phi = (b[i * bs] / range) * pi4;
 #pragma unroll
                 for (int q = 0; q < TC; ++q) {
                     const int t = t0 + q;
@@ -1394,20 +1438,28 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     const cplx<R> off = offset[k];
     const R ocr = off.re, oci = -off.im;
     double num = 0.0, den = 0.0;
-    // both passes issue the next interferogram's loads before processing the
-    // current one
+    // both passes keep the loads of the next two interferograms in flight
+    // while the current one is processed; the loops are unrolled by the
+    // period of that register rotation so the copies become renaming (a copy
+    // of a register that a load in flight targets would wait for the load)
+    const int i1 = I > 1 ? 1 : 0;
     cplx<R> cf = fl[0], cn = nm[0];
     R cb = bat(0);
-#pragma unroll 2
+    cplx<R> mf = fl[(int64_t)i1 * C], mn = nm[(int64_t)i1 * P];
+    R mb = bat(i1);
+#pragma unroll 3
     for (int i = 0; i < I; ++i) {
-        const int i1 = i + 1 < I ? i + 1 : i;
-        const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P];
-        const R nb = bat(i1);
+        const int i2 = i + 2 < I ? i + 2 : I - 1;
+        const cplx<R> nf = fl[(int64_t)i2 * C], nn = nm[(int64_t)i2 * P];
+        const R nb = bat(i2);
         const cplx<R> v = psdph_value<R>(cf, cn);
         const R bi = cb;
-        cf = nf;
-        cn = nn;
-        cb = nb;
+        cf = mf;
+        cn = mn;
+        cb = mb;
+        mf = nf;
+        mn = nn;
+        mb = nb;
         const double wgt = (double)RT<R>::abs_(v.re, v.im);
         const double wb = wgt * (double)bi;
         den += wb * wb;
@@ -1421,15 +1473,21 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     cf = fl[0];
     cn = nm[0];
     cb = bat(0);
-#pragma unroll 2
+    mf = fl[(int64_t)i1 * C];
+    mn = nm[(int64_t)i1 * P];
+    mb = bat(i1);
+#pragma unroll 3
     for (int i = 0; i < I; ++i) {
-        const int i1 = i + 1 < I ? i + 1 : i;
-        const cplx<R> nf = fl[(int64_t)i1 * C], nn = nm[(int64_t)i1 * P];
-        const R nb = bat(i1);
+        const int i2 = i + 2 < I ? i + 2 : I - 1;
+        const cplx<R> nf = fl[(int64_t)i2 * C], nn = nm[(int64_t)i2 * P];
+        const R nb = bat(i2);
         const cplx<R> r = rotate_neg<R>(psdph_value<R>(cf, cn), k0 * cb);
-        cf = nf;
-        cn = nn;
-        cb = nb;
+        cf = mf;
+        cn = mn;
+        cb = mb;
+        mf = nf;
+        mn = nn;
+        mb = nb;
         mre += r.re;
         mim += r.im;
         dn += RT<R>::abs_(r.re, r.im);

@@ -330,6 +330,15 @@ struct ps2_session {
     size_t rsz() const { return form == PS2_FORM_FP64 ? sizeof(double) : sizeof(float); }
 };
 
+// FP32 trial scan: unit steps e^{-i phi} from a per-(pixel, ifg) table, or
+// computed in the scan from the pixel's bperp.  Computing them is the better
+// choice where the scan is bound by HBM bandwidth: 8-trial chunks (fewer than
+// 8 positive trials) over large patches with per-pixel bperp.  Invariant
+// bperp rows keep a one-row table.
+static inline bool scan_step_table(const ps2_session *s) {
+    return s->form == PS2_FORM_FP32 && (s->rowinv || s->tn >= 8 || s->P < 65536);
+}
+
 // Host -> device copy on the session's stream, complete on return (the
 // library issues nothing on the legacy default stream, which would wait on
 // other threads' streams, including one being captured).
@@ -494,11 +503,15 @@ int alloc_state(ps2_session *s) {
             return rc;
     }
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
-    if (sizeof(R) == 4)
+    // the FP32 unit-step table (one row with invariant bperp rows), unless
+    // the scan computes the steps itself; the chunked scan (48 or more
+    // positive trials) always reads it
+    const bool table = sizeof(R) == 4 && (scan_step_table(s) || s->tn >= 48);
+    if (table)
         if ((rc = s->wt.alloc(sizeof(cplx<R>) * (s->rowinv ? I : P * I)))) return rc;
     hipLaunchKernelGGL(k_trial_setup<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
                        s->bp.as<R>(), s->rowinv, P, (int)I, s->rng.as<R>(),
-                       sizeof(R) == 4 ? s->wt.as<cplx<R>>() : nullptr);
+                       table ? s->wt.as<cplx<R>>() : nullptr);
     PS2_LAUNCHED();
     if ((rc = s->offset.alloc(sizeof(cplx<R>) * P))) return rc;
     if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
@@ -693,6 +706,7 @@ FitParams fit_params(ps2_session *s) {
     f.tn = s->tn;
     f.T = s->T;
     f.rowinv = s->rowinv;
+    f.step_table = scan_step_table(s) ? 1 : 0;
     f.rescore_bound = s->rescore_bound;
     return f;
 }

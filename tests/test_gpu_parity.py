@@ -347,6 +347,33 @@ def test_row_invariant_and_snr_paths(S, ora):
     assert g["i_loop"] == r["i_loop"]
     assert np.array_equal(g["trial_ix"], r["trial_ix"])
     np.testing.assert_allclose(g["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
+    # the FP32 form with one bperp row: the scan takes its unit steps from the
+    # per-interferogram table (a stride slip there shows as wrong trials)
+    g32 = S.run_stage2_hip(prep, form="fp32")
+    assert g32["i_loop"] == r["i_loop"]
+    agree = g32["trial_ix"] == r["trial_ix"]
+    dcoh = np.abs(g32["coh_ps"] - r["coh_ps"])
+    print("row-invariant fp32 vs ref64: trial agreement", agree.mean(), "coh p99.9",
+          np.quantile(dcoh, 0.999))
+    assert agree.mean() >= 0.999
+    assert np.quantile(dcoh, 0.999) < 1e-5
+
+
+def test_row_invariant_fit_fp32_vs_ref64(S, ora):
+    """One trial fit (no loop) with row-invariant bperp and a realistic filtered
+    grid: the FP32 trial choice equals ref64's wherever ref64 is not tied."""
+    base = small_prep(n_ps=6000, n_ifg=24, seed=0x5EED1004)
+    prep = dataclasses.replace(base, bperp_mat=None, row_invariant_bperp=True,
+                               row_bperp_nm=base.bperp_nm.copy())
+    f64 = _filt_input(ora, prep, ora.REF64)
+    with S.Stage2Session(prep, form="fp32") as s:
+        o = s.fit(f64.astype(np.complex64))
+    patch = ora.extract_patch_phase(prep, f64.astype(np.complex64), mode=ora.REF32)
+    r64 = ora.fit_rows(prep, patch.astype(np.complex128), mode=ora.REF64)
+    agree = o["trial_ix"] == r64["trial_ix"]
+    print("row-invariant fit fp32 vs ref64 trial agreement", agree.mean())
+    assert agree.mean() >= 0.9999
+    np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
 
 
 # --- batched patches ---------------------------------------------------------------
