@@ -52,23 +52,29 @@ def kernel_work(prep, name):
     FFT2 at 2 * 5 N^2 log2 N^2 = 102,400, 7-tap separable smoothing 28,672,
     pointwise gain 12,288); bytes = grid read + the partial planes written
     (each window row writes its 24 output rows: ~4x the grid, complex64); the
-    combine reads those planes and writes the filtered grid.  Trial scan: one complex multiply-add (8 flop) per pixel,
-    interferogram and trial; bytes = ph_nm + trial step (8 + 8 B) per pixel and
-    interferogram, the filtered grid read once.  Refine: two passes over
+    combine reads those planes and writes the filtered grid.  Trial scan: one
+    complex multiply-add (8 flop) per pixel, interferogram and trial; bytes =
+    ph_nm (8 B) + the unit trial step per pixel and interferogram (8 B from the
+    step table, or 4 B of bperp where the 8-trial scan of a large patch
+    computes the step), the filtered grid read once.  Refine: two passes over
     ph_nm + bperp (12 B) and the filtered grid.  Fill: ph_nm + bperp read,
-    grid written."""
+    grid written.  With row-invariant bperp the per-element bperp / step bytes
+    drop out (one row, cache resident)."""
     P, I, T = prep.n_ps, prep.n_ifg, len(prep.trial_values)
     C = prep.n_i * prep.n_j
+    rowinv = bool(prep.row_invariant_bperp)
+    bpb = 0.0 if rowinv else 4.0                                   # bperp bytes per element
+    stepb = 0.0 if rowinv else (4.0 if (T - 1) // 2 < 8 and P >= 65536 else 8.0)
     W = clap_windows(prep.n_i) * clap_windows(prep.n_j)
     part = 8.0 * clap_windows(prep.n_i) * 24 * prep.n_j * I
     if name == "clap_strip":
         return 143_360.0 * W * I, 8.0 * C * I + part
     if name == "scan":
-        return 8.0 * P * I * T, 16.0 * P * I + 8.0 * C * I
+        return 8.0 * P * I * T, (8.0 + stepb) * P * I + 8.0 * C * I
     if name == "refine":   # (+ the fused RMS/histogram step: coh_prev read and written)
-        return 2 * 20.0 * P * I, 2 * 12.0 * P * I + 8.0 * C * I + 16.0 * P
+        return 2 * 20.0 * P * I, 2 * (8.0 + bpb) * P * I + 8.0 * C * I + 16.0 * P
     if name == "fill_grid":
-        return 16.0 * P * I, 12.0 * P * I + 8.0 * C * I
+        return 16.0 * P * I, (8.0 + bpb) * P * I + 8.0 * C * I
     if name == "clap_combine":
         return 0.0, part + 8.0 * C * I
     if name == "weights":
