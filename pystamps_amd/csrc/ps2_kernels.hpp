@@ -1756,6 +1756,13 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
     __shared__ double s_scale;
     if (ctl->stopped) return;
     const int t = threadIdx.x;
+    // every global input is read here, before the first barrier, so the
+    // phases below wait on one round of loads instead of one per phase
+    const double h_t = t < CTL_BINS ? (double)hist[t] : 0.0;
+    const double nr_t = t < CTL_BINS ? ctl->nr[t] : 0.0;
+    const double gcs0 = ctl->gcs;
+    const int i_loop0 = ctl->i_loop;
+    const uint32_t nresc = rescore_count ? *rescore_count : 0u;
     // gamma-change RMS: fixed assignment of block partials to threads, then a
     // fixed tree (deterministic)
     double acc = 0.0;
@@ -1768,10 +1775,10 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
     }
     if (t == 0) {
         const double rms = sqrt(red[0] / (double)(cp.P > 1 ? cp.P : 1));
-        const double change = rms - ctl->gcs;
+        const double change = rms - gcs0;
         ctl->gcs = rms;
-        if (rescore_count) ctl->rescored += *rescore_count;
-        s_stop = (fabs(change) < cp.conv || ctl->i_loop >= cp.max_iter) ? 1 : 0;
+        if (rescore_count) ctl->rescored += nresc;
+        s_stop = (fabs(change) < cp.conv || i_loop0 >= cp.max_iter) ? 1 : 0;
     }
     __syncthreads();
     if (s_stop) {
@@ -1786,17 +1793,17 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
     };
     if (!cp.psquare) {
         clear_counters();
-        if (t == 0) ctl->i_loop += 1;
+        if (t == 0) ctl->i_loop = i_loop0 + 1;
         return;
     }
     // Nr scaling on the low-coherence bins (native_stage2.rs:365-374)
     const int lct = cp.lct < 0 ? 0 : (cp.lct > CTL_BINS ? CTL_BINS : cp.lct);
     if (t < CTL_BINS) {
-        na[t] = (double)hist[t];
-        nr[t] = ctl->nr[t];
+        na[t] = h_t;
+        nr[t] = nr_t;
     }
-    __syncthreads();
     clear_counters();
+    __syncthreads();
     if (t == 0) {
         double denom = 0.0, num = 0.0;
         for (int b = 0; b < lct; ++b) denom += nr[b];
@@ -1843,7 +1850,7 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
         for (int k = 0; k < 79 && k <= n; ++k) sacc += kCtlInterpTaps[k] * up[n - k];
         prand_hi[q] = sacc * 10.0;
     }
-    if (t == 0) ctl->i_loop += 1;
+    if (t == 0) ctl->i_loop = i_loop0 + 1;
 }
 
 }  // namespace ps2

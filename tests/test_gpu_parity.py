@@ -209,6 +209,32 @@ def test_fit_fp32_vs_ref32_and_ref64(S, ora, prep_c1):
     np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
 
 
+def test_fit_fp32_large_patch_steps_from_bperp(S, ora):
+    """80,000 rows with per-pixel bperp and 13 trials: the 8-trial FP32 scan
+    computes its unit steps from bperp (the path for patches of 65,536 rows or
+    more) instead of reading the step table.  Trial choice and sums as in the
+    table path: ref32's trial wherever ref32 is not itself tied, ref64's
+    coherence within 1e-5."""
+    prep = small_prep(n_ps=80_000, n_ifg=20, extent=12_000.0, seed=0x5EED1901)
+    assert prep.n_ps >= 65_536 and len(prep.trial_values) < 16 and not prep.row_invariant_bperp
+    f32 = _filt_input(ora, prep, ora.REF32)
+    with S.Stage2Session(prep, form="fp32") as s:
+        o = s.fit(f32)
+    patch = ora.extract_patch_phase(prep, f32, mode=ora.REF32)
+    r32 = ora.fit_rows(prep, patch, mode=ora.REF32)
+    r64 = ora.fit_rows(prep, patch.astype(np.complex128), mode=ora.REF64)
+    bound = 2.0 * (8 + 8 + 2 * prep.n_ifg) * 2.0 ** -24
+    clear = r32["trial_margin"] > bound
+    print("large-patch fit fp32: near-ties", int((~clear).sum()), "rescored", o["n_rescored"])
+    assert np.array_equal(o["N_opt"], r32["N_opt"])
+    assert np.array_equal(o["trial_ix"][clear], r32["trial_ix"][clear])
+    ok = _close(o["K_ps"], r32["K_ps"], F64_ATOL, F64_RTOL)
+    assert np.all(ok[clear])
+    agree = o["trial_ix"] == r64["trial_ix"]
+    assert agree.mean() >= 0.9999
+    np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
+
+
 def test_fit_real_row_fp64_and_fp32(S, ora):
     """The reference's real-data 75-interferogram row (tests/golden), fitted by
     the native-path kernels: 48 pixels carry the row with different constant
