@@ -264,6 +264,13 @@ int staging_d2h(void *dst_host, const void *src_device, size_t bytes, hipStream_
 
 using namespace ps2;
 
+// CLAP strip segmentation of a session (clap_segments)
+struct ClapSegs {
+    int nseg, wseg, nplanes;
+};
+
+constexpr int CLAP_TEAMS = 2;   // teams (interferograms) per CLAP wavefront
+
 struct ps2_session {
     std::mutex mu;
     int form = PS2_FORM_FP32;
@@ -276,6 +283,7 @@ struct ps2_session {
     int tn = 0, T = 0, rowinv = 0;
     int nwi = 0, nwj = 0;
     bool windows = false;
+    ClapSegs segs{1, 1, 4};   // set with the device state
     double alpha = 1.0, beta = 0.3, nr_max_nz_ix = 1.0, conv = 1e-4;
     int64_t lct = 31, max_iter = 25;
     int weighting = PS2_WEIGHT_PSQUARE;
@@ -469,26 +477,38 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
 }
 
 // Window segments of the CLAP strips: each window row is split into nseg
-// segments of wseg windows until there are ~4 wavefronts per SIMD (1024
-// SIMDs).  Segment s covers columns [6 wseg s, 6 wseg (s + 1) + 18), so it
-// overlaps the next ceil(3 / wseg) segments; segments write their partial
-// sums into nplanes = 1 + ceil(3 / wseg) planes in turn (2 from 3 windows per
-// segment on), so no two overlapping segments share a plane.
-struct ClapSegs {
-    int nseg, wseg, nplanes;
-};
-
-constexpr int CLAP_TEAMS = 2;   // teams (interferograms) per CLAP wavefront
-
-static ClapSegs clap_segments(int nwi, int nwj, int ngroups) {
-    const int rows = std::max(1, nwi * ngroups);
-    int nseg = std::max(1, (4096 + rows - 1) / rows);
-    nseg = std::min(nseg, std::max(1, nwj));
-    ClapSegs c;
-    c.wseg = (nwj + nseg - 1) / nseg;
-    c.nseg = (nwj + c.wseg - 1) / c.wseg;
-    c.nplanes = 1 + (3 + c.wseg - 1) / c.wseg;
-    return c;
+// segments of wseg windows; one strip (a wavefront) walks one segment of one
+// window row for one pair of interferograms.  The strips run in rounds of
+// `slots` (CUs x strip wavefronts resident per CU), and a strip's time is
+// about wseg windows plus a quarter window of start and flush, so the launch
+// takes ~ ceil(strips / slots) x (wseg + 1/4) window times (per XCD: strip u
+// runs on XCD u % 8).  Each segment also adds 18 overlap columns per window
+// row to the partial planes that the combine reads: 2-7.5e-5 x I x nwi
+// window times per segment measured on the combine (500k x 60, 4M x 120,
+// 1M x 200); the model takes 3e-5.  nseg minimises the sum: a last round that holds a few strips
+// costs as much as a full one.  Segment s covers columns
+// [6 wseg s, 6 wseg (s + 1) + 18), so it overlaps the next ceil(3 / wseg)
+// segments; segments write their partial sums into nplanes = 1 + ceil(3 / wseg)
+// planes in turn (2 from 3 windows per segment on), so no two overlapping
+// segments share a plane.
+static ClapSegs clap_segments(int nwi, int nwj, int ngroups, int I, int slots) {
+    const int64_t rows = std::max<int64_t>(1, (int64_t)nwi * ngroups);
+    const double seg_cost = 3e-5 * (double)I * (double)nwi;
+    const int64_t xslots = std::max<int64_t>(1, slots / PS2_NUM_XCD);
+    const int nw = std::max(1, nwj);
+    ClapSegs best{1, nw, 1 + (3 + nw - 1) / nw};
+    double best_cost = -1.0;
+    for (int nseg = 1; nseg <= nw; ++nseg) {
+        const int wseg = (nw + nseg - 1) / nseg;
+        if ((nw + wseg - 1) / wseg != nseg) continue;   // same segments as a smaller nseg
+        const int64_t per_xcd = (rows * nseg + PS2_NUM_XCD - 1) / PS2_NUM_XCD;
+        const double cost = (double)((per_xcd + xslots - 1) / xslots) * (wseg + 0.25) + seg_cost * nseg;
+        if (best_cost < 0.0 || cost < best_cost) {   // ties keep the longer segments
+            best_cost = cost;
+            best = {nseg, wseg, 1 + (3 + wseg - 1) / wseg};
+        }
+    }
+    return best;
 }
 
 template <typename R>
@@ -498,8 +518,14 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->grid.alloc(sizeof(cplx<R>) * C * I))) return rc;
     if ((rc = s->filt.alloc(sizeof(cplx<R>) * C * I))) return rc;
     if (s->windows) {
-        const ClapSegs cs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS));
-        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * cs.nplanes * CLAP_WIN * s->nj * I)))
+        // strip slots: CUs x CLAP wavefronts resident per CU (register bound)
+        int n_cu = 0, per_cu = 0;
+        PS2_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+        PS2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&k_clap_strip<R, CLAP_TEAMS, true>), 32 * CLAP_TEAMS, 0));
+        s->segs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS), (int)I,
+                                std::max(1, n_cu) * std::max(1, per_cu));
+        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * s->segs.nplanes * CLAP_WIN * s->nj * I)))
             return rc;
     }
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
@@ -651,7 +677,7 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
     p.nwi = s->nwi;
     p.nwj = s->nwj;
     p.ngroups = (int)((s->I + teams - 1) / teams);
-    const ClapSegs cs = clap_segments(s->nwi, s->nwj, p.ngroups);
+    const ClapSegs cs = s->segs;
     p.wseg = cs.wseg;
     p.nseg = cs.nseg;
     p.nplanes = cs.nplanes;
