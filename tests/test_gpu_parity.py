@@ -159,6 +159,29 @@ def test_clap_fp32_vs_ref32(S, ora, prep_small):
     assert err.max() <= 1e-6 * np.abs(r).max()
 
 
+def test_clap_fp32_vs_ref32_full_size_grid(S, ora):
+    """CLAP on a 299 x 299 grid with 60 interferograms (the 500k x 60
+    workload's grid): the strips cover several windows each, so the partial
+    planes alternate between 2 planes and the combine takes its 2-segment
+    path; every cell within 1e-6 of the plane's largest value of ref32."""
+    prep = small_prep(n_ps=30_000, n_ifg=60, extent=15_000.0, seed=0x5EED1902)
+    assert (prep.n_i, prep.n_j) == (299, 299)
+    g = _grid_input(prep).astype(np.complex64)
+    with S.Stage2Session(prep, form="fp32") as s:
+        f = s.clap(g)
+    r = ora.clap_filter_grid_stack(prep, g, mode=ora.REF32)
+    err = np.abs(f - r)
+    print("clap fp32 299x299x60 max|diff|", err.max(), "max|ref|", np.abs(r).max())
+    assert np.array_equal(np.isnan(f), np.isnan(r))
+    assert err.max() <= 1e-6 * np.abs(r).max()
+    # the FP64 form on the same grid (its own segmentation: one wavefront per SIMD)
+    g64 = g.astype(np.complex128)
+    with S.Stage2Session(prep, form="fp64") as s:
+        f64 = s.clap(g64)
+    r64 = ora.clap_filter_grid_stack(prep, g64, mode=ora.REF64)
+    np.testing.assert_allclose(f64, r64, rtol=0, atol=1e-12 * np.abs(r64).max())
+
+
 def _filt_input(ora, prep, mode):
     """A realistic filtered grid: the oracle's CLAP of the first fill."""
     k = np.zeros(prep.n_ps)
