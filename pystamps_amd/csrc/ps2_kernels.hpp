@@ -1307,15 +1307,22 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
             // sums, v z^t = A + iB (trial t) and v conj(z)^t = A - iB (trial
             // -t); the first argmax over ascending trial index is kept with
             // the smaller index winning exact ties
+            // (only the nu = ceil(I / 64) slots that hold interferograms: the
+            // others are zero and stay out of the serial chain)
+            const int nu = (I + 63) / 64;
             double zr[RESCORE_PER_LANE], zi[RESCORE_PER_LANE], wr[RESCORE_PER_LANE], wi[RESCORE_PER_LANE];
 #pragma unroll
             for (int u = 0; u < RESCORE_PER_LANE; ++u) {
                 zr[u] = 1.0;
                 zi[u] = 0.0;
-                double sn, cs;
-                sincos((vb[u] / range) * 0.78539816339744830962, &sn, &cs);
-                wr[u] = cs;
-                wi[u] = -sn;
+                wr[u] = 1.0;
+                wi[u] = 0.0;
+                if (u < nu) {
+                    double sn, cs;
+                    sincos((vb[u] / range) * 0.78539816339744830962, &sn, &cs);
+                    wr[u] = cs;
+                    wi[u] = -sn;
+                }
             }
             auto offer = [&](double sr, double si, int idx) {
                 const double c = hypot(sr, si) / denom;
@@ -1330,14 +1337,16 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
                 double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
 #pragma unroll
                 for (int u = 0; u < RESCORE_PER_LANE; ++u) {
-                    ar = fma(vr[u], zr[u], ar);
-                    ai = fma(vi[u], zr[u], ai);
-                    br = fma(vr[u], zi[u], br);
-                    bi = fma(vi[u], zi[u], bi);
-                    const double nzr = zr[u] * wr[u] - zi[u] * wi[u];
-                    const double nzi = zr[u] * wi[u] + zi[u] * wr[u];
-                    zr[u] = nzr;
-                    zi[u] = nzi;
+                    if (u < nu) {
+                        ar = fma(vr[u], zr[u], ar);
+                        ai = fma(vi[u], zr[u], ai);
+                        br = fma(vr[u], zi[u], br);
+                        bi = fma(vi[u], zi[u], bi);
+                        const double nzr = zr[u] * wr[u] - zi[u] * wi[u];
+                        const double nzi = zr[u] * wi[u] + zi[u] * wr[u];
+                        zr[u] = nzr;
+                        zi[u] = nzi;
+                    }
                 }
                 wave_sum4_f64(ar, ai, br, bi);
                 offer(ar - bi, ai + br, prm.tn + t);
