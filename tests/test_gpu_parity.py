@@ -258,6 +258,43 @@ def test_fit_fp32_large_patch_steps_from_bperp(S, ora):
     np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
 
 
+@pytest.mark.parametrize("tn, n_ifg", [(12, 24), (20, 24), (30, 24), (41, 70), (55, 24)])
+def test_fit_trial_chunks_fp64_and_fp32(S, ora, tn, n_ifg):
+    """The scan's chunk paths by positive trial count: the 16/24/32/48-trial
+    one-pass kernels and, beyond 47, the chunked scan re-anchored per chunk
+    (70 interferograms also take the FP64 re-score past one 64-lane slot).
+    FP64 form vs ref64 exact; FP32 form: ref32's trial wherever ref32 is not
+    itself tied at the form's re-score bound, ref64's coherence within 1e-5."""
+    from pystamps_amd.prepare import trial_values
+    base = small_prep(n_ps=6000, n_ifg=n_ifg, seed=0x5EED1005)
+    wraps = tn / 8.0
+    prep = dataclasses.replace(base, n_trial_wraps=wraps, trial_values=trial_values(wraps))
+    assert len(prep.trial_values) == 2 * tn + 1
+    f64 = _filt_input(ora, prep, ora.REF64)
+    with S.Stage2Session(prep, form="fp64") as s:
+        o64 = s.fit(f64)
+    r64 = ora.fit_rows(prep, ora.extract_patch_phase(prep, f64, mode=ora.REF64), mode=ora.REF64)
+    assert np.array_equal(o64["trial_ix"], r64["trial_ix"])
+    np.testing.assert_allclose(o64["K_ps"], r64["K_ps"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(o64["coh_ps"], r64["coh_ps"], rtol=0, atol=1e-10)
+    f32 = _filt_input(ora, prep, ora.REF32)
+    with S.Stage2Session(prep, form="fp32") as s:
+        o32 = s.fit(f32)
+    patch = ora.extract_patch_phase(prep, f32, mode=ora.REF32)
+    r32 = ora.fit_rows(prep, patch, mode=ora.REF32)
+    r64b = ora.fit_rows(prep, patch.astype(np.complex128), mode=ora.REF64)
+    tc = next((c for c in (8, 16, 24, 32, 48) if tn < c), 48)
+    bound = 2.0 * (tc + 8 + 2 * prep.n_ifg) * 2.0 ** -24
+    clear = r32["trial_margin"] > bound
+    print(f"tn {tn}: ref32 near-ties", int((~clear).sum()), "rescored", o32["n_rescored"])
+    assert (~clear).mean() < 0.01
+    assert np.array_equal(o32["trial_ix"][clear], r32["trial_ix"][clear])
+    assert np.all(_close(o32["K_ps"], r32["K_ps"], F64_ATOL, F64_RTOL)[clear])
+    agree = o32["trial_ix"] == r64b["trial_ix"]
+    assert agree.mean() >= 0.9995
+    np.testing.assert_allclose(o32["coh_ps"][agree], r64b["coh_ps"][agree], rtol=0, atol=1e-5)
+
+
 def test_fit_real_row_fp64_and_fp32(S, ora):
     """The reference's real-data 75-interferogram row (tests/golden), fitted by
     the native-path kernels: 48 pixels carry the row with different constant
