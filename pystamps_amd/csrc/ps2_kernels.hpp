@@ -907,8 +907,9 @@ __global__ __launch_bounds__(256) void k_grid_to_cells(const cplx<R> *__restrict
 // computed once per session: the pixel's bperp range max - min (1 when zero)
 // and, for the FP32 form, the unit trial step e^{-i phi} with
 // phi = (bp / range) * pi/4 per interferogram, with the scan's own f32
-// arithmetic.  wt may be NULL (FP64 form).  With row-invariant bperp every
-// pixel has the same range and wt holds one step per interferogram.
+// arithmetic.  wt may be NULL (FP64 form, or an FP32 scan that computes the
+// steps itself).  With row-invariant bperp every pixel has the same range and
+// wt holds one step per interferogram.
 template <typename R>
 __global__ __launch_bounds__(256) void k_trial_setup(const R *__restrict__ bp, int rowinv,
                                                      int64_t P, int I, R *__restrict__ rng,
