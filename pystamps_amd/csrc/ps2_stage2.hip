@@ -860,11 +860,14 @@ int launch_iteration(ps2_session *s) {
 }
 
 // The gamma loop from the initial state (native_stage2.rs:242-258) to the
-// stop decision.  Default: one iteration is captured as a HIP graph once per
-// session and launched gamma_max_iterations times back to back; the control
-// kernel's stop flag turns the remaining launches into early returns, and the
-// host waits once.  With per-kernel timing or verbose output the iterations
+// stop decision.  Default: gamma_max_iterations iterations are captured as one
+// HIP graph once per session (one iteration, launched that many times, when
+// there are more than LOOP_GRAPH_MAX) and launched; the control kernel's stop
+// flag turns the remaining kernels into early returns, and the host waits
+// once.  With per-kernel timing or verbose output the iterations
 // are launched one by one and the host reads the control block after each.
+constexpr int64_t LOOP_GRAPH_MAX = 64;
+
 template <typename R>
 int run_loop(ps2_session *s) {
     const int64_t P = s->P;
@@ -885,11 +888,16 @@ int run_loop(ps2_session *s) {
     s->gate = &s->ctl.as<LoopCtl>()->stopped;
     const int64_t max_iter = s->ctl_params.max_iter;
     bool stepwise = s->timing || s->verbose || s->no_graph;
+    // the graph holds every iteration (up to LOOP_GRAPH_MAX of them, else one
+    // iteration replayed): each graph launch costs ~30 us more than the
+    // ~10 us between two of its kernels
+    const int64_t per_graph = max_iter <= LOOP_GRAPH_MAX ? max_iter : 1;
     if (!stepwise && !s->graph_exec) {
         // no allocation anywhere in the library while the capture is open
         std::unique_lock<std::shared_mutex> cap(capture_mutex());
         PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        rc = launch_iteration<R>(s);
+        rc = 0;
+        for (int64_t it = 0; it < per_graph && !rc; ++it) rc = launch_iteration<R>(s);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s->stream, &g);
         cap.unlock();
@@ -910,7 +918,7 @@ int run_loop(ps2_session *s) {
         }
     }
     if (!stepwise) {
-        for (int64_t it = 0; it < max_iter; ++it) PS2_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+        for (int64_t it = 0; it < max_iter; it += per_graph) PS2_HIP(hipGraphLaunch(s->graph_exec, s->stream));
     } else {
         for (int64_t it = 0; it < max_iter; ++it) {
             const auto t_it = std::chrono::steady_clock::now();
