@@ -1061,7 +1061,8 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                 // (8 trials) unrolled by the period of the register rotation,
                 // so the copies below become renaming: a copy of a register
                 // that a load in flight targets would wait for that load
-#pragma unroll(TC == 8 ? 3 : 1)
+                constexpr int ROT_UNROLL = TC == 8 ? 3 : 1;
+#pragma unroll ROT_UNROLL
                 for (int i = 0; i < I; ++i) {
                     const int i2 = i + 2 < I ? i + 2 : I - 1;
                     const cplx<R> nf = fl[(int64_t)i2 * C], nn = nm[(int64_t)i2 * P];
