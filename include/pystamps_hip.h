@@ -204,9 +204,10 @@ int ps2_hip_select_columns_f64(const double *src, int64_t n_rows, int64_t n_cols
  * every input already in HBM, and returns after the stop decision; outputs
  * stay on the device until ps2_hip_session_download.  The loop stays on the
  * device: a control kernel takes the stop decision and builds the P-square
- * table, one iteration is captured as a HIP graph on the session's first run
- * and replayed gamma_max_iterations times (iterations after the stop return at
- * once), and the host waits once per loop.  With timing or verbose output the
+ * table, gamma_max_iterations iterations are captured as one HIP graph on the
+ * session's first run (one iteration, launched that many times, beyond 64
+ * iterations) and launched (kernels after the stop return at once), and the
+ * host waits once per loop.  With timing or verbose output the
  * same kernels are launched iteration by iteration instead.
  * Threads: every entry point may be called from several host threads at once.
  * A session is used by one call at a time (calls on the same session
