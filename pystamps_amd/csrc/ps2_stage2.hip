@@ -380,13 +380,16 @@ int validate(const ps2_patch_in *in) {
         if (in->trial_values[t] != (double)(t - tn))
             return arg_error("trial_values must be the consecutive integers -tn..tn (trial_values(), native_stage2.rs:1305-1308)");
     if (tn > 4096) return arg_error("trial count too large");
+    // size limits first (before any per-pixel read), as divisions so that no
+    // product of the caller's extents can overflow
+    const int64_t lim = (int64_t)1 << 31;
+    if (in->n_i > lim / in->n_j || in->n_ps > lim / in->n_ifg || in->n_i * in->n_j > lim / in->n_ifg)
+        return arg_error("patch too large for one session (n_ps*n_ifg and n_i*n_j*n_ifg must not exceed 2^31)");
     const int64_t C = in->n_i * in->n_j;
     for (int64_t p = 0; p < in->n_ps; ++p)
         if (in->grid_lin[p] < 0 || in->grid_lin[p] >= C)
             return arg_error("grid_lin[" + std::to_string(p) + "] outside the n_i x n_j grid");
     if (in->weighting == PS2_WEIGHT_SNR && !in->amp) return arg_error("amp is required for SNR weighting");
-    if (in->n_ps * in->n_ifg > (int64_t)1 << 31 || C * in->n_ifg > (int64_t)1 << 31)
-        return arg_error("patch too large for one session (n_ps*n_ifg and cells*n_ifg must stay below 2^31)");
     return PS2_OK;
 }
 

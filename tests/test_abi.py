@@ -78,6 +78,11 @@ def test_header_structs_match_ctypes_layout():
     (lambda h: (setattr(h.s, "row_invariant", 0), setattr(h.s, "bperp_mat", None)), "bperp_mat is required"),
     (lambda h: (setattr(h.s, "weighting", _native.WEIGHT_SNR), setattr(h.s, "amp", None)), "amp is required"),
     (lambda h: setattr(h.s, "d_a", None), "required input array is NULL"),
+    # size limits are checked before any per-pixel array is read, so these
+    # oversized extents never touch the 500-row arrays they point at
+    (lambda h: setattr(h.s, "n_ps", (1 << 31) // 6 + 1), "patch too large"),
+    (lambda h: (setattr(h.s, "n_i", 1 << 40), setattr(h.s, "n_j", 1 << 40)), "patch too large"),
+    (lambda h: (setattr(h.s, "n_i", 1 << 16), setattr(h.s, "n_j", 1 << 14)), "patch too large"),
 ])
 def test_patch_validation_messages_on_host(mutate, message):
     """Inputs the library rejects before any device call, with the message."""
