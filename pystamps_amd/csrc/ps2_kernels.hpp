@@ -960,6 +960,20 @@ __device__ inline void scan_sums_from_ab(R (&pre)[TC], R (&pim)[TC], R (&nre)[TC
     }
 }
 
+// e / d with the bits of the IEEE quotient, from rcp = 1 / d (correctly
+// rounded, once per pixel): q0 = e rcp is within 1.5 ulp of e / d; one
+// correction q + (e - d q) rcp (the residual is exact with an FMA) brings it
+// within 1 ulp, and a second is then correctly rounded (Markstein's theorem),
+// for every quotient in the normal range.  Five instructions instead of the
+// eleven of a division.  The corrections are written negated,
+// -((-r) rcp + (-q)), which keeps the sign of a zero quotient.
+__device__ inline float div_from_rcp(float e, float d, float rcp) {
+    float q = e * rcp;
+    q = -fmaf(-fmaf(-q, d, e), rcp, -q);
+    q = -fmaf(-fmaf(-q, d, e), rcp, -q);
+    return q;
+}
+
 // Trial scan of topofit_row (native_stage2.rs:1207-1256) for one pixel per
 // thread.  Trials are processed in chunks of TC positive trials t and their
 // negatives -t.
@@ -973,7 +987,9 @@ __device__ inline void scan_sums_from_ab(R (&pre)[TC], R (&pim)[TC], R (&nre)[TC
 // trial sum at the selected trial).  rng and wt come from k_trial_setup.
 // SINGLE (FP32 form, tn < TC): only the single-chunk path is compiled, so the
 // kernel needs fewer registers and more wavefronts fit per SIMD.
-template <typename R, int TC, bool SINGLE = false>
+// NQ (single-chunk path only): tn known at compile time (NQ = tn > 0), so the
+// trial loop has no per-trial test of a runtime count; 0 reads tn at run time.
+template <typename R, int TC, bool SINGLE = false, int NQ = 0>
 __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                                               const R *__restrict__ bp,
                                               const cplx<R> *__restrict__ wt,
@@ -998,6 +1014,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     const cplx<R> *w1 = wt ? (prm.rowinv ? wt : wt + k) : nullptr;  // FP32 table: w1[i * bs]
     const int64_t C = prm.C;
     const R range = rng[k];
+    const R rcp = (R)1 / range;   // (the on-the-fly unit step of the 8-trial scan)
     const R pi4 = RT<R>::pi4();
     const int tn = prm.tn;
 
@@ -1021,16 +1038,20 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     };
 
     static_assert(!SINGLE || sizeof(R) == 4, "the single-chunk scan is the FP32 form's");
+    static_assert(NQ == 0 || (SINGLE && NQ < TC), "a compile-time trial count needs the single-chunk scan");
     if constexpr (sizeof(R) == 4) {
         if (SINGLE || tn < TC) {
+            const int qmax = NQ > 0 ? NQ : tn;   // the host passes NQ = tn
             // one chunk: trials 0..tn and their negatives.  Trial 0 adds v
             // itself and the recurrence starts at z = e^{-i phi}, which is what
             // the general loop computes from z = 1 (exact multiplies by 1, 0).
             // The unit step e^{-i phi} comes from the session's table or, where
             // the host found the scan bound by HBM bandwidth (8-trial chunks
             // over large patches, per-pixel bperp), is computed from the
-            // pixel's bperp: 4 bytes read instead of 8, with the same f32
-            // operations as the table, so the same bits.  The 8-trial loop
+            // pixel's bperp: 4 bytes read instead of 8, with the table's
+            // values (the quotient bperp / range has the IEEE bits, see
+            // div_from_rcp, and the rest are the same f32 operations).  The
+            // 8-trial loop
             // keeps the loads of the next two interferograms in flight.
             R pre[TC], pim[TC], nre[TC], nim[TC];
 #pragma unroll
@@ -1049,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                         return e;
                     } else {
                         R sn, cs;
-                        RT<R>::sincos_((e / range) * pi4, &sn, &cs);
+                        RT<R>::sincos_(div_from_rcp(e, range, rcp) * pi4, &sn, &cs);
                         return {cs, -sn};
                     }
                 };
@@ -1077,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                     R zr = wr, zi = wi;
 #pragma unroll
                     for (int q = 1; q < TC; ++q) {
-                        if (q <= tn) {
+                        if (q <= qmax) {
                             // A_q = sum v Re(z^q) in (pre, pim), B_q = sum v
                             // Im(z^q) in (nre, nim): both trial sums v z^q =
                             // A + iB and v conj(z)^q = A - iB from four FMAs
@@ -1112,10 +1133,10 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
                 return;
             }
             if (denom == (R)0) denom = (R)1;
-            scan_sums_from_ab<R, TC>(pre, pim, nre, nim, 1, tn);
+            scan_sums_from_ab<R, TC>(pre, pim, nre, nim, 1, qmax);
 #pragma unroll
             for (int q = 0; q < TC; ++q) {
-                if (q <= tn) {
+                if (q <= qmax) {
                     offer(RT<R>::abs_(pre[q], pim[q]) / denom, q + tn, pre[q], pim[q]);
                     if (q != 0) offer(RT<R>::abs_(nre[q], nim[q]) / denom, tn - q, nre[q], nim[q]);
                 }

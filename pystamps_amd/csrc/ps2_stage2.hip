@@ -773,9 +773,19 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     if constexpr (sizeof(R) == 4) {
         // FP32: every trial fits one chunk (tn < tc) except beyond 47
         // positive trials, which take 48-trial chunks
-        if (tc == 8)
-            go(k_scan<R, 8, true>);
-        else if (tc == 16)
+        if (tc == 8) {
+            // the trial count as a compile-time constant (no per-trial test)
+            switch (s->tn) {
+            case 1: go(k_scan<R, 8, true, 1>); break;
+            case 2: go(k_scan<R, 8, true, 2>); break;
+            case 3: go(k_scan<R, 8, true, 3>); break;
+            case 4: go(k_scan<R, 8, true, 4>); break;
+            case 5: go(k_scan<R, 8, true, 5>); break;
+            case 6: go(k_scan<R, 8, true, 6>); break;
+            case 7: go(k_scan<R, 8, true, 7>); break;
+            default: go(k_scan<R, 8, true>); break;
+            }
+        } else if (tc == 16)
             go(k_scan<R, 16, true>);
         else if (tc == 24)
             go(k_scan<R, 24, true>);
