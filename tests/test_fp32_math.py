@@ -129,3 +129,39 @@ def test_atan2_fit_axes_zeros_and_nan():
     np.testing.assert_allclose(a[:8], ref[:8], rtol=0, atol=3e-7)
     assert np.signbit(a[1]) and not np.signbit(a[0])
     assert np.isnan(a[8]) and np.isnan(a[9])
+
+
+def _fma_f32(a, b, c):
+    # the product of two float32 values is exact in long double (64-bit
+    # significand); one rounding to long double then float32 for the sum
+    L = np.longdouble
+    return (L(a) * L(b) + L(c)).astype(F32)
+
+
+def _div_from_rcp(e, d):
+    # the 8-trial scan's quotient (div_from_rcp in ps2_kernels.hpp): per-pixel
+    # reciprocal, two corrections written negated to keep a zero's sign
+    rcp = (F32(1) / d).astype(F32)
+    q = (e * rcp).astype(F32)
+    q = -_fma_f32(-_fma_f32(-q, d, e), rcp, -q)
+    q = -_fma_f32(-_fma_f32(-q, d, e), rcp, -q)
+    return q
+
+
+@pytest.mark.skipif(np.finfo(np.longdouble).nmant < 63, reason="needs an 80-bit long double for exact FMA emulation")
+def test_scan_quotient_has_ieee_bits():
+    """bperp / range from the reciprocal with two FMA corrections equals the
+    IEEE float32 quotient bit for bit (Markstein's theorem), so the scan that
+    computes its unit steps from bperp sees the step table's values."""
+    rng = np.random.default_rng(5)
+    n = 2_000_000
+    cases = [
+        (rng.uniform(-3000, 3000, n), rng.uniform(7e5, 1.2e6, n)),          # bperp, slant range
+        (rng.uniform(1, 2, n) * 2.0 ** rng.integers(-100, 100, n) * rng.choice([-1, 1], n),
+         rng.uniform(1, 2, n) * 2.0 ** rng.integers(-24, 24, n)),          # any normal quotient
+        (np.concatenate([np.zeros(500), -np.zeros(500)]), rng.uniform(7e5, 1.2e6, 1000)),
+    ]
+    for e, d in cases:
+        e, d = e.astype(F32), d.astype(F32)
+        q = _div_from_rcp(e, d)
+        np.testing.assert_array_equal(q.view(np.uint32), (e / d).view(np.uint32))
