@@ -258,11 +258,44 @@ def test_fit_fp32_large_patch_steps_from_bperp(S, ora):
     np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
 
 
-@pytest.mark.parametrize("tn, n_ifg", [(12, 24), (20, 24), (30, 24), (41, 70), (55, 24)])
+def _first_rows(prep, n):
+    """The patch's first n pixels on the same grid (per-pixel arrays cut)."""
+    kw = {"n_ps": n}
+    for f in dataclasses.fields(prep):
+        v = getattr(prep, f.name)
+        if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[0] == prep.n_ps:
+            kw[f.name] = np.ascontiguousarray(v[:n])
+    return dataclasses.replace(prep, **kw)
+
+
+def test_fit_fp32_steps_from_bperp_equal_table_bits(S, ora):
+    """The unit steps the 8-trial FP32 scan computes from bperp (patches of
+    65,536 rows or more) are the step table's values bit for bit: the first
+    65,535 rows of an 80,000-row patch, fitted on their own (step table), give
+    the trial, K and coherence bits they get within the whole patch."""
+    prep = small_prep(n_ps=80_000, n_ifg=20, extent=12_000.0, seed=0x5EED1901)
+    n = 65_535
+    sub = _first_rows(prep, n)
+    assert sub.n_ps == n and sub.ph_nm.shape[0] == n and sub.grid_lin.shape[0] == n
+    f32 = _filt_input(ora, prep, ora.REF32)
+    with S.Stage2Session(prep, form="fp32") as s:
+        whole = s.fit(f32)
+    with S.Stage2Session(sub, form="fp32") as s:
+        part = s.fit(f32)
+    assert np.array_equal(whole["trial_ix"][:n], part["trial_ix"])
+    for key in ("K_ps", "coh_ps"):
+        a = np.ascontiguousarray(whole[key][:n], dtype=np.float64)
+        b = np.ascontiguousarray(part[key], dtype=np.float64)
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), key
+
+
+@pytest.mark.parametrize("tn, n_ifg", [(1, 24), (2, 24), (3, 24), (4, 24), (5, 24), (6, 24), (7, 24),
+                                       (12, 24), (20, 24), (30, 24), (41, 70), (55, 24)])
 def test_fit_trial_chunks_fp64_and_fp32(S, ora, tn, n_ifg):
-    """The scan's chunk paths by positive trial count: the 16/24/32/48-trial
-    one-pass kernels and, beyond 47, the chunked scan re-anchored per chunk
-    (70 interferograms also take the FP64 re-score past one 64-lane slot).
+    """The scan's chunk paths by positive trial count: the 8-trial kernel
+    compiled for each trial count 1..7, the 16/24/32/48-trial one-pass kernels
+    and, beyond 47, the chunked scan re-anchored per chunk (70 interferograms
+    also take the FP64 re-score past one 64-lane slot).
     FP64 form vs ref64 exact; FP32 form: ref32's trial wherever ref32 is not
     itself tied at the form's re-score bound, ref64's coherence within 1e-5."""
     from pystamps_amd.prepare import trial_values
