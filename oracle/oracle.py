@@ -69,44 +69,75 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
-        L.ora_stage2_run.argtypes = [C.POINTER(_In), C.POINTER(_Out), C.c_int, C.c_int]
-        L.ora_fill_phase_weight.argtypes = [C.POINTER(_In), _f64p, _f64p, C.c_void_p, C.c_int, C.c_int]
-        L.ora_accumulate_grid.argtypes = [C.POINTER(_In), C.c_void_p, C.c_void_p, C.c_int]
-        L.ora_clap_filter_grid_stack.argtypes = [C.POINTER(_In), C.c_void_p, C.c_void_p, C.c_int, C.c_int]
-        L.ora_extract_patch_phase.argtypes = [C.POINTER(_In), C.c_void_p, C.c_void_p, C.c_int, C.c_int]
-        L.ora_fit_rows.argtypes = [C.POINTER(_In), C.c_void_p, _f64p, _f64p, _f64p, _f64p,
-                                   C.c_void_p, _i32p, _f32p, C.c_int, C.c_int]
-        L.ora_rms_difference.argtypes = [_f64p, _f64p, C.c_int64]
-        L.ora_rms_difference.restype = C.c_double
-        L.ora_hist_with_centers.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p]
-        L.ora_psquare_weighting.argtypes = [_f64p, _f64p, C.c_int64, C.c_int64, C.c_double,
-                                            _f64p, C.c_int64, _f64p, _f64p]
-        L.ora_rng_uniform.argtypes = [C.c_uint32, C.c_int64, _f64p]
-        L.ora_trial_values.argtypes = [C.c_double, _f64p, C.c_int64]
-        L.ora_trial_values.restype = C.c_int64
-        L.ora_low_pass.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _i64p]
-        L.ora_grid_indices.argtypes = [_f32p, C.c_int64, C.c_double, _f32p, _i64p, _i64p]
-        L.ora_clap_window_weight.argtypes = [C.c_int64, _f64p]
-        L.ora_random_coherence_histogram.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p,
-                                                     C.c_int64, C.c_uint32, _f64p, _f64p, C.c_int]
-        L.ora_random_coherence_histogram_sb.argtypes = [_f64p, C.c_int64, _i32p, _i32p, C.c_int64,
-                                                        _f64p, C.c_int64, _f64p, C.c_int64,
-                                                        C.c_uint32, _f64p, _f64p, C.c_int]
-        L.ora_py_near_max_candidates.argtypes = [_f64p, C.c_int64, _i64p]
-        L.ora_py_near_max_candidates.restype = C.c_int64
-        L.ora_py_select_candidate.argtypes = [_i64p, _f64p, _f64p, C.c_int64, C.c_int64]
-        L.ora_py_select_candidate.restype = C.c_int64
-        L.ora_py_topofit_rows.argtypes = [_f64p, _f64p, C.c_int, C.c_int64, C.c_int64, C.c_double,
-                                          _f64p, _f64p, _f64p, _f64p]
-        L.ora_py_grid_accumulate.argtypes = [_f32p, _i64p, C.c_int64, C.c_int64, C.c_int64, _f32p]
-        L.ora_py_grid_accumulate.restype = C.c_int
-        L.ora_topofit_phase_row_coh.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64]
-        L.ora_topofit_phase_row_coh.restype = C.c_double
-        L.ora_fft32_2d.argtypes = [_f64p, C.c_int]
-        L.ora_snr_weighting.argtypes = [_f32p, C.c_void_p, C.c_int64, C.c_int64, _f64p, C.c_int]
-        _lib = L
+        _lib = _bind(C.CDLL(str(LIB_PATH)))
     return _lib
+
+
+def build_variant(out_path, defines=()) -> Path:
+    """Compile the oracle with extra -D defines (the planted-mistake builds of
+    tests/test_oracle_pins.py) into out_path; the default library is untouched."""
+    out_path = Path(out_path)
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE), f"LIB={out_path}",
+                    "EXTRA=" + " ".join(f"-D{d}" for d in defines)], check=True)
+    return out_path
+
+
+class use_library:
+    """Context manager: the module's functions call the library at `path`
+    (a build_variant output) instead of the default oracle."""
+
+    def __init__(self, path):
+        self.path = str(path)
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = _bind(C.CDLL(self.path))
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
+def _bind(L):
+    L.ora_stage2_run.argtypes = [C.POINTER(_In), C.POINTER(_Out), C.c_int, C.c_int]
+    L.ora_fill_phase_weight.argtypes = [C.POINTER(_In), _f64p, _f64p, C.c_void_p, C.c_int, C.c_int]
+    L.ora_accumulate_grid.argtypes = [C.POINTER(_In), C.c_void_p, C.c_void_p, C.c_int]
+    L.ora_clap_filter_grid_stack.argtypes = [C.POINTER(_In), C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    L.ora_extract_patch_phase.argtypes = [C.POINTER(_In), C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    L.ora_fit_rows.argtypes = [C.POINTER(_In), C.c_void_p, _f64p, _f64p, _f64p, _f64p,
+                               C.c_void_p, _i32p, _f32p, C.c_int, C.c_int]
+    L.ora_rms_difference.argtypes = [_f64p, _f64p, C.c_int64]
+    L.ora_rms_difference.restype = C.c_double
+    L.ora_hist_with_centers.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p]
+    L.ora_psquare_weighting.argtypes = [_f64p, _f64p, C.c_int64, C.c_int64, C.c_double,
+                                        _f64p, C.c_int64, _f64p, _f64p]
+    L.ora_rng_uniform.argtypes = [C.c_uint32, C.c_int64, _f64p]
+    L.ora_trial_values.argtypes = [C.c_double, _f64p, C.c_int64]
+    L.ora_trial_values.restype = C.c_int64
+    L.ora_low_pass.argtypes = [C.c_double, C.c_double, C.c_double, _f64p, _i64p]
+    L.ora_grid_indices.argtypes = [_f32p, C.c_int64, C.c_double, _f32p, _i64p, _i64p]
+    L.ora_clap_window_weight.argtypes = [C.c_int64, _f64p]
+    L.ora_random_coherence_histogram.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p,
+                                                 C.c_int64, C.c_uint32, _f64p, _f64p, C.c_int]
+    L.ora_random_coherence_histogram_sb.argtypes = [_f64p, C.c_int64, _i32p, _i32p, C.c_int64,
+                                                    _f64p, C.c_int64, _f64p, C.c_int64,
+                                                    C.c_uint32, _f64p, _f64p, C.c_int]
+    L.ora_py_near_max_candidates.argtypes = [_f64p, C.c_int64, _i64p]
+    L.ora_py_near_max_candidates.restype = C.c_int64
+    L.ora_py_select_candidate.argtypes = [_i64p, _f64p, _f64p, C.c_int64, C.c_int64]
+    L.ora_py_select_candidate.restype = C.c_int64
+    L.ora_py_topofit_rows.argtypes = [_f64p, _f64p, C.c_int, C.c_int64, C.c_int64, C.c_double,
+                                      _f64p, _f64p, _f64p, _f64p]
+    L.ora_py_grid_accumulate.argtypes = [_f32p, _i64p, C.c_int64, C.c_int64, C.c_int64, _f32p]
+    L.ora_py_grid_accumulate.restype = C.c_int
+    L.ora_topofit_phase_row_coh.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64]
+    L.ora_topofit_phase_row_coh.restype = C.c_double
+    L.ora_fft32_2d.argtypes = [_f64p, C.c_int]
+    L.ora_snr_weighting.argtypes = [_f32p, C.c_void_p, C.c_int64, C.c_int64, _f64p, C.c_int]
+    return L
 
 
 def _p(a, t):
@@ -360,6 +391,23 @@ def fft32_2d(x, inverse=False):
     buf = a.view(np.float64).copy()
     lib().ora_fft32_2d(_p(buf, _f64p), int(inverse))
     return buf.view(np.complex128).reshape(32, 32)
+
+
+def snr_weighting(amp, ph_res, mode=REF64):
+    """snr_weighting (native_stage2.rs:1875-1897) of amp [P][I] and ph_res [P][I]."""
+    a = _c(amp, np.float32)
+    rt, _ = _rdt(mode)
+    pr = _c(ph_res, rt)
+    P, I = a.shape
+    out = np.zeros(P)
+    lib().ora_snr_weighting(_p(a, _f32p), pr.ctypes.data, P, I, _p(out, _f64p), mode)
+    return out
+
+
+def rms_difference(a, b):
+    x = _c(a, np.float64)
+    y = _c(b, np.float64)
+    return lib().ora_rms_difference(_p(x, _f64p), _p(y, _f64p), x.size)
 
 
 def cpu_threads_default():

@@ -16,24 +16,40 @@
  * its own radix-2 FFT (pinned against a brute-force DFT in tests); num-complex
  * 0.4.6 for complex mul/norm/arg, restated literally below.
  *
- * Pinning (what checks each function; see tests/test_oracle_*.py):
- *   rng_uniform                 golden values, native_stage2.rs:2463-2484
- *   psquare_weighting           golden values, native_stage2.rs:2486-2509,
+ * Pinning (what checks each function; tests/test_oracle_golden.py = G,
+ * tests/test_oracle_pins.py = P):
+ *   rng_uniform                 G: golden values, native_stage2.rs:2463-2484
+ *   psquare_weighting           G: golden values, native_stage2.rs:2486-2509;
  *                               tail/rounding cases tests/test_stage2_trial_wraps.py:55-108
- *   hist_with_centers           golden case, native_stage2.rs:2511-2519
- *   random_coherence_histogram  golden 100-bin counts, native_stage2.rs:2521-2559
- *   trial_values                golden, tests/test_stage2_ported.py:703-707
- *   grid_indices                golden boundary, tests/test_stage2_trial_wraps.py:111-127
- *   low_pass, window weight     closed forms (native_stage2.rs:1014-1037, 2013-2043)
- *   fft / clap                  brute-force DFT; beta=0 => per-window linear filter
- *                               by low_pass checked by direct circular convolution;
- *                               zero input => zero; n_win<=0 => zeros
- *   accumulate_grid             brute-force per-cell sums; f32 cancellation case
+ *   hist_with_centers           G: golden case, native_stage2.rs:2511-2519
+ *   random_coherence_histogram  G: golden 100-bin counts, native_stage2.rs:2521-2559;
+ *                               small-baseline branch composed by numpy (exact counts)
+ *   trial_values                G: golden, tests/test_stage2_ported.py:703-707
+ *   grid_indices                G: golden boundary, tests/test_stage2_trial_wraps.py:111-127
+ *   low_pass, window weight     G: closed forms (native_stage2.rs:1014-1037, 2013-2043)
+ *   fft                         G: brute-force DFT
+ *   clap                        P: the whole gain path (|F|, fftshift, 7x7 gausswin
+ *                               smoothing, ifftshift, median, (H/m)^alpha - 1 clamped,
+ *                               beta blend with low_pass, inverse FFT, window weights,
+ *                               edge-window shifts, NaN cells) against numpy.fft +
+ *                               scipy.signal.convolve2d + np.median, alpha 1 and 0.7,
+ *                               asymmetric low_pass; beta = 0 against a direct
+ *                               circular-convolution sum by ifft2(low_pass);
+ *                               G: all-pass, shift equivariance, linearity, zero, empty
+ *   fill                        P: ph_nm = e^{+i bp K} cancels the ramp (output = w);
+ *                               NaN K => NaN
+ *   accumulate_grid             G: brute-force per-cell sums; f32 cancellation case
  *                               tests/test_stage2_ported.py:864-907
- *   topofit / refine            noise-free closed forms (K on a trial node => exact
- *                               trial, coh=1, C recovered), brute force FP64 scan
- *   fill / extract / rms        per-element closed forms
- *   whole loop                  3-PS fixture known outcome (native_stage2.rs:2698-2769)
+ *   extract                     P: own cell's value, direction kept, |.| = 1, zeros kept
+ *   rms_difference              P: numpy sqrt(mean(d^2))
+ *   snr_weighting               P: closed forms (2 sqrt 2, sqrt(2/3), sigma = 0)
+ *   topofit / refine            G: noise-free closed forms (K on a trial node => exact
+ *                               trial, coh=1, C recovered), off-node refine, brute-force
+ *                               FP64 scan
+ *   whole loop                  G: 3-PS fixture known outcome (native_stage2.rs:2698-2769);
+ *                               P: two iterations composed in numpy from the pinned steps
+ *                               (RMS, Nr scaling, P-square) equal the loop's state
+ * Each P pin is shown to fail on a planted mistake (ORA_MUTANT builds below).
  */
 #include "stage2_oracle.h"
 
@@ -46,6 +62,25 @@
 #endif
 
 #define ORA_PI 3.14159265358979323846
+
+/* Planted mistakes, compiled in only by tests/test_oracle_pins.py (-DORA_MUTANT=n)
+ * to show that the pin tests catch each of them.  0 (the default): none. */
+#ifndef ORA_MUTANT
+#define ORA_MUTANT 0
+#endif
+#define ORA_MUT_SMOOTH_FIRST 1    /* 7x7 smoothing before the fftshift */
+#define ORA_MUT_MEDIAN_UPPER 2    /* median = upper middle value only */
+#define ORA_MUT_LP_TRANSPOSED 3   /* low_pass read column-major */
+#define ORA_MUT_NO_ISHIFT 4       /* ifftshift left out */
+#define ORA_MUT_SHIFT_OFF1 5      /* fftshift by n/2 - 1 */
+#define ORA_MUT_NO_CLAMP 6        /* negative gain not clamped to 0 */
+#define ORA_MUT_WEIGHT_UNSHIFTED 7 /* edge window weight without the shift */
+#define ORA_MUT_FILL_SIGN 8       /* fill ramp e^{+i bp K} */
+#define ORA_MUT_EXTRACT_NO_NORM 9 /* gathered phase not normalised */
+#define ORA_MUT_SNR_NO_HALF 10    /* sigma without the 1/2 */
+#define ORA_MUT_RMS_NO_MEAN 11    /* sqrt of the sum, not the mean */
+#define ORA_MUT_ALPHA_IGNORED 12  /* (H/m)^alpha taken as H/m */
+#define ORA_MUT_NR_UNSCALED 13    /* loop: Nr not rescaled before P-square */
 #define COH_BIN_COUNT 100
 
 /* native_stage2.rs:33-113: audited MATLAB interp(.,10) FIR taps. */
@@ -392,6 +427,9 @@ double ora_rms_difference(const double *a, const double *b, int64_t n) {
         s += d * d;
     }
     double denom = (double)(n > 1 ? n : 1);
+#if ORA_MUTANT == ORA_MUT_RMS_NO_MEAN
+    denom = 1.0;
+#endif
     return sqrt(s / denom);
 }
 
@@ -603,9 +641,16 @@ static void clap_one_ifg(const clap_ctx *c, const double *g_re, const double *g_
         fft2(&p, br, bi, 0, tr, ti);
         for (int64_t k = 0; k < nn; ++k) habs[k] = hypot(br[k], bi[k]);
         /* fftshift by (n/2, n/2) */
+        int64_t fsh = n / 2;
+#if ORA_MUTANT == ORA_MUT_SHIFT_OFF1
+        fsh = n / 2 - 1;
+#endif
         for (int64_t r = 0; r < n; ++r)
             for (int64_t q = 0; q < n; ++q)
-                hsh[r * n + q] = habs[((r + n / 2) % n) * n + ((q + n / 2) % n)];
+                hsh[r * n + q] = habs[((r + fsh) % n) * n + ((q + fsh) % n)];
+#if ORA_MUTANT == ORA_MUT_SMOOTH_FIRST
+        memcpy(hsh, habs, sizeof(double) * (size_t)nn);
+#endif
         /* separable 'same' convolution, zero fill */
         for (int64_t r = 0; r < n; ++r)
             for (int64_t q = 0; q < n; ++q) {
@@ -630,15 +675,34 @@ static void clap_one_ifg(const clap_ctx *c, const double *g_re, const double *g_
         for (int64_t r = 0; r < n; ++r)
             for (int64_t q = 0; q < n; ++q)
                 hsh[r * n + q] = hflt[((r + ish) % n) * n + ((q + ish) % n)];
+#if ORA_MUTANT == ORA_MUT_NO_ISHIFT
+        memcpy(hsh, hflt, sizeof(double) * (size_t)nn);
+#elif ORA_MUTANT == ORA_MUT_SMOOTH_FIRST
+        /* smoothed unshifted |F|, then the two shifts cancel */
+        memcpy(hsh, hflt, sizeof(double) * (size_t)nn);
+#endif
         memcpy(hmed, hsh, sizeof(double) * (size_t)nn);
         qsort(hmed, (size_t)nn, sizeof(double), cmp_f64);
         double med = (nn % 2 == 0) ? (hmed[nn / 2 - 1] + hmed[nn / 2]) / 2.0 : hmed[nn / 2];
+#if ORA_MUTANT == ORA_MUT_MEDIAN_UPPER
+        med = hmed[nn / 2];
+#endif
         for (int64_t k = 0; k < nn; ++k) {
             double v = hsh[k];
             if (med != 0.0) v /= med;
+#if ORA_MUTANT == ORA_MUT_ALPHA_IGNORED
+            v = v - 1.0;
+#else
             v = c->alpha_is_one ? v - 1.0 : pow(v, c->alpha) - 1.0;
+#endif
+#if ORA_MUTANT != ORA_MUT_NO_CLAMP
             if (v < 0.0) v = 0.0;
+#endif
+#if ORA_MUTANT == ORA_MUT_LP_TRANSPOSED
+            double g = v * c->beta + c->low_pass[(k % n) * n + k / n];
+#else
             double g = v * c->beta + c->low_pass[k];
+#endif
             br[k] *= g;
             bi[k] *= g;
         }
@@ -647,6 +711,9 @@ static void clap_one_ifg(const clap_ctx *c, const double *g_re, const double *g_
             for (int64_t lj = 0; lj < nw; ++lj) {
                 if (li < win->row_shift || lj < win->col_shift) continue;
                 double wt = c->weight[(li - win->row_shift) * nw + (lj - win->col_shift)];
+#if ORA_MUTANT == ORA_MUT_WEIGHT_UNSHIFTED
+                wt = c->weight[li * nw + lj];
+#endif
                 if (wt == 0.0) continue;
                 double f = wt * inv_scale;
                 int64_t dst = (win->i1 + li) * n_j + (win->j1 + lj);
@@ -903,6 +970,9 @@ void ora_snr_weighting(const float *amp, const void *ph_res, int64_t n_ps, int64
         g /= d;
         a2 /= d;
         double sigma = sqrt(0.5 * (a2 - g * g));
+#if ORA_MUTANT == ORA_MUT_SNR_NO_HALF
+        sigma = sqrt(a2 - g * g);
+#endif
         out[r] = sigma != 0.0 ? g / sigma : 0.0;
     }
 }
