@@ -120,7 +120,7 @@ int ps2_hip_device_count(void);
 /* Whole stage-2 gamma loop for one patch: replaces the loop of
  * run_stage2_native_inner (native_stage2.rs:242-406) and fills the write_pm1
  * arguments (native_stage2.rs:319-336).  Inputs are checked on the host
- * before any device call; a patch whose n_ps*n_ifg or n_i*n_j*n_ifg exceeds
+ * before any device call; a patch whose n_ps*n_ifg or n_i*n_j*n_ifg reaches
  * 2^31 is rejected (split it into patches, as the reference's patching does). */
 int ps2_hip_run(const ps2_patch_in *in, ps2_patch_out *out, const ps2_run_opts *opts);
 

@@ -381,10 +381,11 @@ int validate(const ps2_patch_in *in) {
             return arg_error("trial_values must be the consecutive integers -tn..tn (trial_values(), native_stage2.rs:1305-1308)");
     if (tn > 4096) return arg_error("trial count too large");
     // size limits first (before any per-pixel read), as divisions so that no
-    // product of the caller's extents can overflow
-    const int64_t lim = (int64_t)1 << 31;
+    // product of the caller's extents can overflow; every product stays below
+    // 2^31, so pixel and cell counts fit the int32 counting sort
+    const int64_t lim = ((int64_t)1 << 31) - 1;
     if (in->n_i > lim / in->n_j || in->n_ps > lim / in->n_ifg || in->n_i * in->n_j > lim / in->n_ifg)
-        return arg_error("patch too large for one session (n_ps*n_ifg and n_i*n_j*n_ifg must not exceed 2^31)");
+        return arg_error("patch too large for one session (n_ps*n_ifg and n_i*n_j*n_ifg must be below 2^31)");
     const int64_t C = in->n_i * in->n_j;
     for (int64_t p = 0; p < in->n_ps; ++p)
         if (in->grid_lin[p] < 0 || in->grid_lin[p] >= C)

@@ -81,6 +81,12 @@ def test_header_structs_match_ctypes_layout():
     # size limits are checked before any per-pixel array is read, so these
     # oversized extents never touch the 500-row arrays they point at
     (lambda h: setattr(h.s, "n_ps", (1 << 31) // 6 + 1), "patch too large"),
+    # 2^31 pixels in one interferogram, and exactly 2^31 elements: the int32
+    # counting sort of the cells would overflow
+    (lambda h: (setattr(h.s, "n_ps", 1 << 31), setattr(h.s, "n_ifg", 1)), "patch too large"),
+    (lambda h: (setattr(h.s, "n_ps", 1 << 30), setattr(h.s, "n_ifg", 2)), "patch too large"),
+    (lambda h: (setattr(h.s, "n_i", 1 << 15), setattr(h.s, "n_j", 1 << 15), setattr(h.s, "n_ifg", 2)),
+     "patch too large"),
     (lambda h: (setattr(h.s, "n_i", 1 << 40), setattr(h.s, "n_j", 1 << 40)), "patch too large"),
     (lambda h: (setattr(h.s, "n_i", 1 << 16), setattr(h.s, "n_j", 1 << 14)), "patch too large"),
 ])
