@@ -2,9 +2,10 @@
 """Stage-2 gamma-estimation benchmark on one MI355X.
 
 Metric: PS-candidate x interferogram x gamma-iteration per second of the
-stage-2 gamma loop (first fill .. final stop decision), with the patch's
+stage-2 gamma loop (first fill .. final stop decision, then the write_pm1
+outputs of the last iteration in the caller's layouts), with the patch's
 inputs already resident in HBM.  A step is one whole gamma loop over the
-patch (to convergence or gamma_max_iterations).
+patch (to convergence or gamma_max_iterations) with its outputs.
 
 Default workload: 500,000 candidates in 15x15 km, 60 interferograms, bperp
 U[-250, 250] m, 40 % coherent pixels, StaMPS defaults (50 m grid => 299x299
@@ -79,6 +80,11 @@ def kernel_work(prep, name):
         return 0.0, part + 8.0 * C * I
     if name == "weights":
         return 0.0, 24.0 * P
+    if name == "outputs":
+        # once per loop: ph_nm + bperp read, ph_weight + ph_patch + ph_res
+        # written (8 + 8 + 4 B) in row order, the filtered grid read, the grid
+        # transposed (read + write), five f64 vectors and the trial index
+        return 0.0, (8.0 + bpb + 20.0) * P * I + 24.0 * C * I + 92.0 * P
     return 0.0, 0.0
 
 
@@ -97,10 +103,40 @@ def aggregate_roofline(prep, kt, peak_tflops):
 
 KERNEL_SYMBOLS = {"clap_strip": "k_clap_strip", "scan": "k_scan", "refine": "k_refine",
                   "fill_grid": "k_fill_grid", "clap_combine": "k_clap_combine",
-                  "rescore": "k_rescore", "weights": "k_weights"}
+                  "rescore": "k_rescore", "weights": "k_weights", "outputs": "k_final_outputs"}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage2_fp32_500k_60ifg_pmc_traffic.md")
+STATS_SUMMARY = os.path.join(ROOT, "profiles", "stage2_fp32_500k_60ifg_kernel_stats.md")
+
+
+def summary_avg_us(path, kernel):
+    """Average launch duration (us) of `kernel` in a committed rocprofv3
+    --kernel-trace --stats summary table (| kernel | calls | total ms | avg us | ...)."""
+    sym = KERNEL_SYMBOLS.get(kernel, kernel)
+    try:
+        with open(path) as fh:
+            for line in fh:
+                cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+                if len(cols) >= 4 and sym in cols[0]:
+                    try:
+                        return float(cols[3])
+                    except ValueError:
+                        continue
+    except OSError:
+        pass
+    return None
+
+
+def cpu_threads():
+    """Host threads for the CPU baseline: every core this process may run on
+    (its affinity set), within the job's CPU share when the host sets one
+    through OMP_NUM_THREADS."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
 
 
 def summary_traffic(path, kernel):
@@ -248,7 +284,7 @@ def main():
     avg_s = kt[dom][0] / kt[dom][1] * 1e-3
     flops, nbytes = kernel_work(prep, dom)
     if flops / max(nbytes, 1.0) > PEAK_FP32_TFLOPS * 1e3 / PEAK_HBM_GBS:
-        roof = {"bound": "mfma", "engine": "fp32 VALU (no matrix op on this path)",
+        roof = {"bound": "valu", "engine": "fp32 vector ALU (no matrix op on this path)",
                 "achieved": flops / avg_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s"}
     else:
         roof = {"bound": "hbm", "achieved": nbytes / avg_s / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
@@ -267,6 +303,16 @@ def main():
         roof["traffic"] = None
     roof["kernel"] = dom
     roof["avg_launch_us"] = avg_s * 1e6
+    roof["avg_source"] = ("HIP events on the session stream, per launch, over loops launched "
+                          "kernel by kernel after the timed loops")
+    if args.config == 2 and not args.n_ps:
+        pavg = summary_avg_us(STATS_SUMMARY, dom)
+        if pavg:
+            roof["profile_avg_launch_us"] = pavg
+            roof["profile_source"] = os.path.relpath(STATS_SUMMARY, ROOT)
+            roof["frac_from_profile"] = (flops / (pavg * 1e-6) / 1e12 / PEAK_FP32_TFLOPS
+                                         if roof["unit"] == "TFLOP/s"
+                                         else nbytes / (pavg * 1e-6) / 1e9 / PEAK_HBM_GBS)
     roof["loop_time_weighted_frac"] = aggregate_roofline(prep, kt, PEAK_FP32_TFLOPS)
 
     result = {
@@ -288,6 +334,7 @@ def main():
                    "form": "fp32 bulk + fp64 near-tie re-score"},
         "roofline": roof,
         "rescored_rows_per_loop": r32["out"]["n_rescored"],
+        "rechecked_rows_per_loop": r32["out"]["n_rechecked"],
         "end_to_end_seconds": {"total": e2e, "prep_incl_gpu_nr": prep_s, "upload": r32["setup"],
                                "loop": r32["wall"] / args.steps, "download": r32["download"]},
         "kernel_us_per_launch": {k: round(v[0] / v[1] * 1e3, 1) for k, v in r32["kt"].items() if v[1]},
@@ -317,7 +364,7 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         # one warm-up, then the median of args.cpu_runs timed loops
         O.run(prep, mode=O.REF32, threads=threads, diagnostics=False)
         runs = []
@@ -329,6 +376,8 @@ def main():
         cpu_v = P * I * ro["i_loop"] / cpu_s
         result["cpu_baseline"] = {
             "value": cpu_v, "unit": "PS·ifg·iter/s", "cores": threads, "kind": "port",
+            "cores_visible": len(os.sched_getaffinity(0)), "host_cpus": os.cpu_count(),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
             "sample": f"one whole gamma loop ({ro['i_loop']} iterations) of the same {P} x {I} "
                       f"workload, median of {args.cpu_runs} after a warm-up; oracle in the "
                       f"reference's precision map, OpenMP over rows/ifgs",

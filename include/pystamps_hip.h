@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PS2_HIP_ABI_VERSION 1
+#define PS2_HIP_ABI_VERSION 2
 
 /* return codes */
 #define PS2_OK 0
@@ -46,7 +46,7 @@ extern "C" {
 
 /* computation forms */
 #define PS2_FORM_FP64 0     /* plain FP64 form: every step in f64 */
-#define PS2_FORM_FP32 1     /* FP32 bulk + FP64 near-tie trial re-score */
+#define PS2_FORM_FP32 1     /* FP32 bulk + FP64 near-tie trial re-score and P-square re-decision */
 
 /* weighting (Stage2Parms::filter_weighting, native_stage2.rs:364-384) */
 #define PS2_WEIGHT_PSQUARE 0
@@ -96,6 +96,8 @@ typedef struct {
     /* diagnostics */
     int32_t *trial_ix;        /* [n_ps] selected trial index, -1 for invalid rows */
     int64_t n_rescored;       /* FP32 form: rows re-scored in FP64, summed over iterations */
+    int64_t n_rechecked;      /* FP32 form, P-square: rows refined again in FP64 because their
+                                 coherence lay near a lookup / histogram boundary, summed */
     double loop_seconds;      /* device time of the gamma loop (first fill .. stop decision) */
 } ps2_patch_out;
 
@@ -208,8 +210,9 @@ int ps2_hip_select_columns_f64(const double *src, int64_t n_rows, int64_t n_cols
  * device: a control kernel takes the stop decision and builds the P-square
  * table, gamma_max_iterations iterations are captured as one HIP graph on the
  * session's first run (one iteration, launched that many times, beyond 64
- * iterations) and launched (kernels after the stop return at once), and the
- * host waits once per loop.  With timing or verbose output the
+ * iterations) and launched (kernels after the stop return at once); the run
+ * ends with the write_pm1 outputs of the last iteration arranged in the
+ * caller's layouts on the device, and the host waits once per loop.  With timing or verbose output the
  * same kernels are launched iteration by iteration instead.
  * Threads: every entry point may be called from several host threads at once.
  * A session is used by one call at a time (calls on the same session
@@ -253,13 +256,14 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
 /* Per-kernel device time of the session's runs and single steps so far (the reference prints
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
- * rescore, refine, rms_hist, weights.  ms[k] is the summed HIP-event time of
- * slot k over launches[k] launches.  The gamma-change RMS and coherence
- * histogram run inside the refine kernel, so slot rms_hist records no
- * launches (it is kept for the ABI).  Timing events are recorded only when
+ * rescore, refine, rms_hist, weights, outputs.  ms[k] is the summed HIP-event
+ * time of slot k over launches[k] launches.  The gamma-change RMS and
+ * coherence histogram run inside the refine kernel, so slot rms_hist records
+ * no launches (it is kept for the ABI); outputs is the once-per-loop kernel
+ * that writes the write_pm1 arguments in the caller's layouts.  Timing events are recorded only when
  * ps2_hip_session_set_timing(s, 1) was called before the runs; such runs
  * launch the iterations one by one (no graph), with the same results. */
-#define PS2_NUM_KERNEL_SLOTS 8
+#define PS2_NUM_KERNEL_SLOTS 9
 int ps2_hip_session_set_timing(ps2_session *s, int32_t enable);
 int ps2_hip_session_kernel_times(ps2_session *s, double *ms, int64_t *launches);
 
@@ -270,7 +274,9 @@ int ps2_hip_session_kernel_times(ps2_session *s, double *ms, int64_t *launches);
  *  clap:      clap_filter_grid_stack (:742-922) of host ph_grid; ph_filt out.
  *  fit:       extract_patch_phase + fit_stage2_rows (:1186-1205, :659-720) of
  *             host ph_filt; the out struct's k/c/coh/n_opt/ph_res/ph_patch/
- *             trial_ix are filled (ph_res / ph_patch as float / complex64). */
+ *             trial_ix are filled (ph_res / ph_patch as float / complex64).
+ *             It replaces the session's loop outputs: download and
+ *             export_device then need another session_run. */
 int ps2_hip_step_fill_grid(ps2_session *s, const double *k_ps, const double *weighting,
                            void *ph_grid_out);
 int ps2_hip_step_clap(ps2_session *s, const void *ph_grid, void *ph_filt_out);

@@ -51,7 +51,8 @@ class _Out(C.Structure):
         ("ph_res", C.c_void_p), ("ph_patch", C.c_void_p), ("ph_grid", C.c_void_p),
         ("ph_weight", C.c_void_p), ("nr", _f64p), ("gamma_change_save", C.c_double),
         ("i_loop", C.c_int64), ("trial_ix", _i32p), ("trial_margin", _f32p),
-        ("weighting_last", _f64p), ("k_prev", _f64p),
+        ("weighting_last", _f64p), ("k_prev", _f64p), ("trial_margin_min", _f32p),
+        ("decision_margin_min", _f64p),
     ]
 
 
@@ -210,6 +211,8 @@ def run(prep, mode=REF32, threads=0, diagnostics=True):
         o["trial_margin"] = np.zeros(P, np.float32)
         o["weighting_last"] = np.zeros(P)
         o["k_prev"] = np.zeros(P)
+        o["trial_margin_min"] = np.zeros(P, np.float32)
+        o["decision_margin_min"] = np.zeros(P)
     s = _Out()
     s.k_ps, s.c_ps = _p(o["K_ps"], _f64p), _p(o["C_ps"], _f64p)
     s.coh_ps, s.n_opt = _p(o["coh_ps"], _f64p), _p(o["N_opt"], _f64p)
@@ -224,6 +227,8 @@ def run(prep, mode=REF32, threads=0, diagnostics=True):
         s.trial_margin = _p(o["trial_margin"], _f32p)
         s.weighting_last = _p(o["weighting_last"], _f64p)
         s.k_prev = _p(o["k_prev"], _f64p)
+        s.trial_margin_min = _p(o["trial_margin_min"], _f32p)
+        s.decision_margin_min = _p(o["decision_margin_min"], _f64p)
     rc = L.ora_stage2_run(C.byref(inp.s), C.byref(s), mode, threads)
     if rc != 0:
         raise RuntimeError(f"oracle stage 2 failed with code {rc}")

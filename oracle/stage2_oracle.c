@@ -241,6 +241,30 @@ static int64_t center_index(double v, const double *c, int64_t n, int equal) {
     return k;
 }
 
+/* Distance of a coherence to the nearest discrete choice boundary of the
+ * P-square update: the lookup's round-half points (n + 1/2) / 1000
+ * (native_stage2.rs:1674-1682) and the histogram's bin boundaries
+ * (:1593-1606).  Diagnostic only (tests tell near-ties from errors with it). */
+static double decision_distance(double v, const double *c, int64_t n) {
+    double x = 1000.0 * v;
+    double dist = fabs(x - floor(x) - 0.5) * 1e-3;
+    if (n < 2 || !isfinite(v)) return dist;
+    if (centers_equal_spacing(c, n)) {
+        double d = n < 3 ? 1.0 : (c[n - 1] - c[0]) / (double)(n - 1);
+        if (d != 0.0) {
+            double y = (v - (c[0] + c[1]) / 2.0) / d;
+            double e = fabs(y - rint(y)) * d;
+            if (e < dist) dist = e;
+        }
+    } else {
+        for (int64_t k = 0; k + 1 < n; ++k) {
+            double e = fabs(v - (c[k] + c[k + 1]) / 2.0);
+            if (e < dist) dist = e;
+        }
+    }
+    return dist;
+}
+
 void ora_hist_with_centers(const double *values, int64_t n, const double *centers, int64_t nc,
                            double *out) {
     for (int64_t k = 0; k < nc; ++k) out[k] = 0.0;

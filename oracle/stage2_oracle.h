@@ -72,6 +72,10 @@ typedef struct {
     float *trial_margin;      /* [n_ps] (best - second best) coherence of the last scan */
     double *weighting_last;   /* [n_ps] weighting used by the last fill */
     double *k_prev;           /* [n_ps] K used by the last fill */
+    float *trial_margin_min;  /* [n_ps] smallest trial margin over the loop's iterations */
+    double *decision_margin_min; /* [n_ps] smallest distance of coh to a P-square lookup
+                                    (n + 1/2)/1000 or histogram bin boundary over the
+                                    iterations whose coherence sets weights (not the last) */
 } ora_stage2_out;
 
 /* Whole gamma loop (native_stage2.rs:242-406). Returns 0 or a negative code. */

@@ -14,14 +14,14 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = Path(os.environ.get("PS2_HIP_LIB") or PKG / "libpystamps_hip.so")
 HEADER = PKG.parent / "include" / "pystamps_hip.h"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 FORM_FP64 = 0
 FORM_FP32 = 1
 WEIGHT_PSQUARE = 0
 WEIGHT_SNR = 1
 COH_BINS = 100
 KERNEL_SLOTS = ("fill_grid", "clap_strip", "clap_combine", "scan", "rescore",
-                "refine", "rms_hist", "weights")
+                "refine", "rms_hist", "weights", "outputs")
 
 _f64p = C.POINTER(C.c_double)
 _f32p = C.POINTER(C.c_float)
@@ -49,7 +49,7 @@ class PatchOut(C.Structure):
         ("coh_ps_save", _f64p), ("ph_res", _f32p), ("ph_patch", _f32p), ("ph_grid", _f32p),
         ("ph_weight", _f32p), ("nr", _f64p), ("gamma_change_save", C.c_double),
         ("i_loop", C.c_int64), ("trial_ix", _i32p), ("n_rescored", C.c_int64),
-        ("loop_seconds", C.c_double),
+        ("n_rechecked", C.c_int64), ("loop_seconds", C.c_double),
     ]
 
 

@@ -194,37 +194,9 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
 // a workgroup takes FINAL_TP sorted pixels x FINAL_TF interferograms, reads
 // the planes along the pixel axis, and writes each pixel's run of
 // interferograms as one contiguous row segment (order[k] scatters whole rows,
-// not single elements).  Grid (ceil(P / FINAL_TP), ceil(I / FINAL_TF)).
+// not single elements).  Tile shape of k_final_outputs.
 constexpr int FINAL_TP = 32;
 constexpr int FINAL_TF = 64;
-
-// ph_weight output of the last fill.
-template <typename R>
-__global__ __launch_bounds__(256) void k_final_weight(const cplx<R> *__restrict__ phnm,
-                                                      const R *__restrict__ bp, int bp_rowinv,
-                                                      const double *__restrict__ K,
-                                                      const double *__restrict__ W,
-                                                      const int64_t *__restrict__ order,
-                                                      float2 *__restrict__ out, int64_t P, int I) {
-    __shared__ float2 tile[FINAL_TP][FINAL_TF + 1];
-    const int64_t k0 = (int64_t)blockIdx.x * FINAL_TP;
-    const int f0 = blockIdx.y * FINAL_TF;
-    const int np = (int)min<int64_t>(FINAL_TP, P - k0), nf = min(FINAL_TF, I - f0);
-    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
-        const int kl = e % FINAL_TP, fl = e / FINAL_TP;
-        if (kl < np && fl < nf) {
-            const int64_t k = k0 + kl, g = (int64_t)(f0 + fl) * P + k;
-            const R b = bp_rowinv ? bp[f0 + fl] : bp[g];
-            const cplx<R> v = weight_value<R>(phnm[g], b, K[k], W[k]);
-            tile[kl][fl] = make_float2((float)v.re, (float)v.im);
-        }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
-        const int kl = e / FINAL_TF, fl = e % FINAL_TF;
-        if (kl < np && fl < nf) out[order[k0 + kl] * I + f0 + fl] = tile[kl][fl];
-    }
-}
 
 // ---------------------------------------------------------------------------
 // CLAP (native_stage2.rs:742-1184).
@@ -878,31 +850,6 @@ __global__ __launch_bounds__(256) void k_unpermute(const T *__restrict__ src, co
     if (k < P) dst[order[k]] = src[k];
 }
 
-// grid planes [I][C] -> complex64 cells [C][I] (the caller's ph_grid layout),
-// through a 32 x 32 LDS tile so that both sides are coalesced
-template <typename R>
-__global__ __launch_bounds__(256) void k_grid_to_cells(const cplx<R> *__restrict__ grid, float2 *__restrict__ out,
-                                                       int64_t C, int I) {
-    __shared__ float2 tile[32][33];
-    const int64_t c0 = (int64_t)blockIdx.x * 32;
-    const int f0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int r = ty; r < 32; r += 8) {
-        const int f = f0 + r;
-        const int64_t c = c0 + tx;
-        if (f < I && c < C) {
-            const cplx<R> v = grid[(int64_t)f * C + c];
-            tile[r][tx] = make_float2((float)v.re, (float)v.im);
-        }
-    }
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int64_t c = c0 + r;
-        const int f = f0 + tx;
-        if (f < I && c < C) out[c * I + f] = tile[tx][r];
-    }
-}
-
 // Loop-invariant inputs of the trial scan (native_stage2.rs:1215-1225),
 // computed once per session: the pixel's bperp range max - min (1 when zero)
 // and, for the FP32 form, the unit trial step e^{-i phi} with
@@ -1449,7 +1396,8 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
                                     const int32_t *__restrict__ tix, const R *__restrict__ rng,
                                     const cplx<R> *__restrict__ offset, double *__restrict__ Kout,
                                     double *__restrict__ Cout, double *__restrict__ cohout,
-                                    double *__restrict__ nopt, const FitParams &prm) {
+                                    double *__restrict__ nopt, const FitParams &prm,
+                                    float *max_angle) {
     const int t = tix[k];
     if (t < 0) {
         Kout[k] = __builtin_nan("");
@@ -1501,7 +1449,10 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     }
     if (den == 0.0) den = 1.0;
     k0 = (R)((double)k0 + num / den);
-    R mre = (R)0, mim = (R)0, dn = (R)0;
+    // residual sums over the interferograms in f64 (the FP32 form's terms are
+    // f32; f64 accumulation keeps the I-term sum's rounding out of coh)
+    double mre = 0.0, mim = 0.0, dn = 0.0;
+    R amax = (R)0;   // largest |k0 bp| of this pass (bounds the f32 angle rounding)
     cf = fl[0];
     cn = nm[0];
     cb = bat(0);
@@ -1513,23 +1464,26 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
         const int i2 = i + 2 < I ? i + 2 : I - 1;
         const cplx<R> nf = fl[(int64_t)i2 * C], nn = nm[(int64_t)i2 * P];
         const R nb = bat(i2);
-        const cplx<R> r = rotate_neg<R>(psdph_value<R>(cf, cn), k0 * cb);
+        const R ang = k0 * cb;
+        const cplx<R> r = rotate_neg<R>(psdph_value<R>(cf, cn), ang);
+        amax = fmax(amax, fabs(ang));
         cf = mf;
         cn = mn;
         cb = mb;
         mf = nf;
         mn = nn;
         mb = nb;
-        mre += r.re;
-        mim += r.im;
-        dn += RT<R>::abs_(r.re, r.im);
+        mre += (double)r.re;
+        mim += (double)r.im;
+        dn += (double)RT<R>::abs_(r.re, r.im);
     }
-    if (dn == (R)0) dn = (R)1;
+    if (dn == 0.0) dn = 1.0;
     Kout[k] = (double)k0;
-    Cout[k] = (double)RT<R>::atan2_(mim, mre);
-    const double coh = (double)(RT<R>::abs_(mre, mim) / dn);
+    Cout[k] = (double)RT<R>::atan2_((R)mim, (R)mre);
+    const double coh = sqrt(mre * mre + mim * mim) / dn;
     cohout[k] = coh;
     nopt[k] = 1.0;
+    *max_angle = (float)amax;
     return coh;
 }
 
@@ -1573,6 +1527,123 @@ __device__ inline void rms_hist_tail(const RmsHistOut &ro, bool mine, int64_t k,
         if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
 }
 
+// FP64 re-decision of the P-square inputs (FP32 form).  The loop turns each
+// coherence into two discrete choices: the P-square lookup index
+// round_half_away(1000 coh) (native_stage2.rs:1674-1682) and the coherence
+// histogram bin (hist_with_centers :1593-1606).  A row whose FP32 coherence
+// lies within the refine's error bound of a choice boundary is refined again
+// in FP64 (one wavefront per row, lanes over interferograms, from the same
+// trial index, filtered phase and ph_nm, with the caller's f64 baselines), as
+// refine_candidate (:1259-1303) does in the FP64 oracle; its K, C and coh are
+// then the FP64 values.
+struct RecheckIn {
+    const double *bp64;   // [I][P] (or [I] with invariant rows): the caller's f64 bperp
+    int enabled;          // FP32 form with P-square weighting
+};
+
+// distance of coh to the nearest lookup or histogram boundary
+__device__ inline double decision_distance(double coh, const double *__restrict__ mids, const HistParams &hp) {
+    const double x = 1000.0 * coh;
+    double dist = fabs(x - floor(x) - 0.5) * 1e-3;   // round-half boundaries n + 1/2
+    if (hp.equal) {
+        if (hp.d != 0.0) {
+            const double y = (coh - hp.cutoff0) / hp.d;   // bin edges cutoff0 + j d
+            dist = fmin(dist, fabs(y - rint(y)) * hp.d);
+        }
+    } else {
+        for (int b = 0; b + 1 < hp.nbins; ++b) dist = fmin(dist, fabs(coh - mids[b]));
+    }
+    return dist;
+}
+
+// FP64 refine of row k by one wavefront (every lane returns the row's coh)
+__device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__restrict__ phnm,
+                                             const double *__restrict__ bp64,
+                                             const cplx<float> *__restrict__ filt,
+                                             const int32_t *__restrict__ cell_of,
+                                             const int32_t *__restrict__ tix, double *__restrict__ Kout,
+                                             double *__restrict__ Cout, double *__restrict__ cohout,
+                                             const FitParams &prm, int lane) {
+    const int I = prm.I;
+    const int64_t P = prm.P, C = prm.C;
+    const cplx<float> *nm = phnm + k;
+    const cplx<float> *fl = filt + cell_of[k];
+    auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
+    // psdph in f64: conj(unit filtered phase) x ph_nm
+    auto val = [&](int i, double &re, double &im) {
+        const cplx<float> f = fl[(int64_t)i * C], n = nm[(int64_t)i * P];
+        double fr = (double)f.re, fi = (double)f.im;
+        const double a = hypot(fr, fi);
+        if (a != 0.0) {
+            fr /= a;
+            fi /= a;
+        }
+        const double nr = (double)n.re, ni = (double)n.im;
+        re = fr * nr + fi * ni;
+        im = fr * ni - fi * nr;
+    };
+    double mn = INFINITY, mx = -INFINITY;
+    for (int i = lane; i < I; i += 64) {
+        const double b = bat(i);
+        mn = fmin(mn, b);
+        mx = fmax(mx, b);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, o, 64));
+        mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    double range = mx - mn;
+    if (range == 0.0) range = 1.0;
+    double k0 = 0.78539816339744830962 / range * (double)(tix[k] - prm.tn);
+    double ore = 0.0, oim = 0.0;
+    for (int i = lane; i < I; i += 64) {
+        double re, im, sn, cs;
+        val(i, re, im);
+        sincos(k0 * bat(i), &sn, &cs);
+        ore += re * cs + im * sn;
+        oim += im * cs - re * sn;
+    }
+    ore = wave_sum_f64(ore);
+    oim = wave_sum_f64(oim);
+    double num = 0.0, den = 0.0;
+    for (int i = lane; i < I; i += 64) {
+        double re, im, sn, cs;
+        val(i, re, im);
+        const double b = bat(i);
+        sincos(k0 * b, &sn, &cs);
+        const double rr = re * cs + im * sn, ri = im * cs - re * sn;
+        const double w = hypot(re, im), wb = w * b;
+        den += wb * wb;
+        const double pr = rr * ore + ri * oim, pim = ri * ore - rr * oim;   // r conj(offset)
+        num += wb * (w * atan2(pim, pr));
+    }
+    num = wave_sum_f64(num);
+    den = wave_sum_f64(den);
+    if (den == 0.0) den = 1.0;
+    k0 += num / den;
+    double mre = 0.0, mim = 0.0, dn = 0.0;
+    for (int i = lane; i < I; i += 64) {
+        double re, im, sn, cs;
+        val(i, re, im);
+        sincos(k0 * bat(i), &sn, &cs);
+        const double rr = re * cs + im * sn, ri = im * cs - re * sn;
+        mre += rr;
+        mim += ri;
+        dn += hypot(rr, ri);
+    }
+    double unused = 0.0;
+    wave_sum4_f64(mre, mim, dn, unused);
+    if (dn == 0.0) dn = 1.0;
+    const double coh = hypot(mre, mim) / dn;
+    if (lane == 0) {
+        Kout[k] = k0;
+        Cout[k] = atan2(mim, mre);
+        cohout[k] = coh;
+    }
+    return coh;
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm,
                                                 const R *__restrict__ bp,
@@ -1585,64 +1656,191 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
                                                 double *__restrict__ Cout,
                                                 double *__restrict__ cohout,
                                                 double *__restrict__ nopt, FitParams prm, RmsHistOut ro,
-    const int32_t *__restrict__ gate) {
+                                                RecheckIn rc, uint32_t *__restrict__ recheck_count,
+                                                const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool row = k < prm.P;
     double coh = 0.0;
+    float amax = 0.0f;
     if (row) {
         if (prm.rowinv)
-            coh = refine_row<R, true>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm);
+            coh = refine_row<R, true>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt,
+                                      prm, &amax);
         else
-            coh = refine_row<R, false>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm);
+            coh = refine_row<R, false>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt,
+                                       prm, &amax);
+    }
+    if constexpr (sizeof(R) == 4) {
+        if (rc.enabled) {
+            // rows near a lookup / bin boundary: bound on |coh_f32 - coh_f64| of
+            // this refine, 2 (8 + max |k0 bp|) 2^-24 (per term: psdph product,
+            // rotation, angle rounding <= |k0 bp| ulp; sums in f64)
+            __shared__ int n_flag;
+            __shared__ int flag_tid[256];
+            __shared__ double new_coh[256];
+            if (threadIdx.x == 0) n_flag = 0;
+            __syncthreads();
+            const double bound = 2.0 * (8.0 + (double)amax) * 5.9604644775390625e-08;
+            const bool flag = row && tix[k] >= 0 && decision_distance(coh, ro.mids, ro.hp) < bound;
+            if (flag) flag_tid[atomicAdd(&n_flag, 1)] = threadIdx.x;
+            __syncthreads();
+            const int nf = n_flag;
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            for (int q = wave; q < nf; q += 4) {
+                const int t = flag_tid[q];
+                const double c = refine_row_f64_wave((int64_t)blockIdx.x * blockDim.x + t, phnm, rc.bp64, filt,
+                                                     cell_of, tix, Kout, Cout, cohout, prm, lane);
+                if (lane == 0) new_coh[t] = c;
+            }
+            __syncthreads();
+            if (flag) coh = new_coh[threadIdx.x];
+            if (threadIdx.x == 0 && nf > 0 && recheck_count) atomicAdd(recheck_count, (uint32_t)nf);
+        }
     }
     if (ro.coh_prev) rms_hist_tail(ro, row, k, coh);
 }
 
-// ph_patch (unit filt) for every row and ph_res = arg(residual) of valid rows,
-// in the caller's row order (tiled like k_final_weight).
+// The loop's outputs in the caller's layouts (the write_pm1 arguments,
+// native_stage2.rs:319-336), one launch after the last iteration:
+//  - ph_weight of the last fill (fill_phase_weight :629-657, from the K and
+//    weighting that fill used), ph_patch (the unit filtered cell value,
+//    extract_patch_phase :1186-1205) and ph_res = arg(residual) of valid rows
+//    (:714-716), [P][I] row order.  A pixel tile is FINAL_TP sorted pixels x
+//    FINAL_TF interferograms: the inputs are read once along the planes
+//    (coalesced over pixels), each output goes out through one LDS tile as
+//    whole row segments (coalesced over interferograms);
+//  - K, C, coh, N_opt, coh_ps_save and the trial index in row order (the
+//    first interferogram tile of each pixel tile);
+//  - ph_grid [C][I] from the [I][C] planes (32 x 32 transposed tiles, in the
+//    blocks after the pixel tiles).
+// Any output pointer may be NULL.
+struct FinalOut {
+    float2 *weight;        // [P][I]
+    float2 *patch;         // [P][I]
+    float *res;            // [P][I]
+    double *vec;           // [5][P]: K, C, coh, N_opt, coh_ps_save
+    int32_t *tix;          // [P]
+    float2 *grid;          // [C][I]
+    int64_t n_pix_blocks;  // pixel tiles: ceil(P / FINAL_TP) x ceil(I / FINAL_TF)
+    int n_ifg_tiles;       // ceil(I / FINAL_TF)
+    int n_grid_ifg_tiles;  // ceil(I / 32)
+};
+
+struct FinalIn {
+    const double *K, *Cc, *coh, *nopt, *coh_prev;   // sorted order, [P]
+    const double *Kw, *Ww;                          // K and weighting of the last fill
+    const int32_t *tix;
+};
+
 template <typename R>
-__global__ __launch_bounds__(256) void k_final_patch_res(const cplx<R> *__restrict__ phnm,
-                                                         const R *__restrict__ bp,
-                                                         const cplx<R> *__restrict__ filt,
-                                                         const int32_t *__restrict__ cell_of,
-                                                         const double *__restrict__ K,
-                                                         const double *__restrict__ nopt,
-                                                         const int64_t *__restrict__ order,
-                                                         float2 *__restrict__ patch_out,
-                                                         float *__restrict__ res_out, int64_t P,
-                                                         int I, int rowinv, int64_t C) {
-    __shared__ float2 tp[FINAL_TP][FINAL_TF + 1];
-    __shared__ float tr[FINAL_TP][FINAL_TF + 1];
-    const int64_t k0 = (int64_t)blockIdx.x * FINAL_TP;
-    const int f0 = blockIdx.y * FINAL_TF;
-    const int np = (int)min<int64_t>(FINAL_TP, P - k0), nf = min(FINAL_TF, I - f0);
-    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
-        const int kl = e % FINAL_TP, fl = e / FINAL_TP;
-        if (kl < np && fl < nf) {
-            const int64_t k = k0 + kl;
-            const int f = f0 + fl;
-            const int64_t g = (int64_t)f * P + k;
-            const cplx<R> fv = filt[(int64_t)f * C + cell_of[k]];
-            tp[kl][fl] = make_float2((float)fv.re, (float)fv.im);
-            float pr = 0.0f;
-            if (res_out && nopt[k] == 1.0) {
-                const cplx<R> v = psdph_value<R>(fv, phnm[g]);
-                const R b = rowinv ? bp[f] : bp[g];
-                const cplx<R> r = rotate_neg<R>(v, (R)K[k] * b);
-                pr = (float)RT<R>::atan2_(r.im, r.re);
+__global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict__ phnm,
+                                                       const R *__restrict__ bp, int rowinv,
+                                                       const cplx<R> *__restrict__ filt,
+                                                       const cplx<R> *__restrict__ grid,
+                                                       const int32_t *__restrict__ cell_of,
+                                                       const int64_t *__restrict__ order, FinalIn in,
+                                                       FinalOut o, int64_t P, int I, int64_t C) {
+    __shared__ float2 tile[FINAL_TP][FINAL_TF + 1];
+    const int64_t b = blockIdx.x;
+    if (b >= o.n_pix_blocks) {   // ph_grid transpose tile
+        float2(*t)[33] = reinterpret_cast<float2(*)[33]>(&tile[0][0]);
+        const int64_t gb = b - o.n_pix_blocks;
+        const int64_t c0 = (gb / o.n_grid_ifg_tiles) * 32;
+        const int f0 = (int)(gb % o.n_grid_ifg_tiles) * 32;
+        const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+        for (int r = ty; r < 32; r += 8) {
+            const int f = f0 + r;
+            const int64_t c = c0 + tx;
+            if (f < I && c < C) {
+                const cplx<R> v = grid[(int64_t)f * C + c];
+                t[r][tx] = make_float2((float)v.re, (float)v.im);
             }
-            tr[kl][fl] = pr;
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {
+            const int64_t c = c0 + r;
+            const int f = f0 + tx;
+            if (f < I && c < C) o.grid[c * I + f] = t[tx][r];
+        }
+        return;
+    }
+    const int64_t k0 = (b / o.n_ifg_tiles) * FINAL_TP;
+    const int f0 = (int)(b % o.n_ifg_tiles) * FINAL_TF;
+    const int np = (int)min<int64_t>(FINAL_TP, P - k0), nf = min(FINAL_TF, I - f0);
+    if (f0 == 0 && (int)threadIdx.x < np) {
+        const int64_t k = k0 + threadIdx.x, r = order[k];
+        if (o.vec) {
+            o.vec[r] = in.K[k];
+            o.vec[P + r] = in.Cc[k];
+            o.vec[2 * P + r] = in.coh[k];
+            o.vec[3 * P + r] = in.nopt[k];
+            if (in.coh_prev) o.vec[4 * P + r] = in.coh_prev[k];
+        }
+        if (o.tix) o.tix[r] = in.tix[k];
+    }
+    // read side: a thread keeps one pixel of the tile (kl) and takes the
+    // interferograms fl0 + 8 q; indices are clamped into the tile so that every
+    // load of the thread is issued at once (out-of-tile values are not stored)
+    constexpr int PER = FINAL_TP * FINAL_TF / 256;
+    const int kl = threadIdx.x % FINAL_TP, fl0 = threadIdx.x / FINAL_TP;
+    const int64_t k = k0 + min(kl, np - 1);
+    const int32_t cell = cell_of[k];
+    const R kres = (R)in.K[k];
+    const bool valid = in.nopt[k] == 1.0;
+    const double kw = o.weight ? in.Kw[k] : 0.0, ww = o.weight ? in.Ww[k] : 0.0;
+    cplx<R> nm[PER], fv[PER];
+    R bv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int f = f0 + min(fl0 + 8 * q, nf - 1);
+        const int64_t g = (int64_t)f * P + k;
+        nm[q] = phnm[g];
+        bv[q] = rowinv ? bp[f] : bp[g];
+        fv[q] = filt[(int64_t)f * C + cell];
+    }
+    float2 wv[PER], pv[PER];
+    float rv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        wv[q] = make_float2(0.0f, 0.0f);
+        if (o.weight) {
+            const cplx<R> v = weight_value<R>(nm[q], bv[q], kw, ww);
+            wv[q] = make_float2((float)v.re, (float)v.im);
+        }
+        pv[q] = make_float2((float)fv[q].re, (float)fv[q].im);
+        rv[q] = 0.0f;
+        if (o.res && valid) {
+            const cplx<R> r = rotate_neg<R>(psdph_value<R>(fv[q], nm[q]), kres * bv[q]);
+            rv[q] = (float)RT<R>::atan2_(r.im, r.re);
         }
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < FINAL_TP * FINAL_TF; e += 256) {
-        const int kl = e / FINAL_TF, fl = e % FINAL_TF;
-        if (kl < np && fl < nf) {
-            const int64_t o = order[k0 + kl] * I + f0 + fl;
-            if (patch_out) patch_out[o] = tp[kl][fl];
-            if (res_out) res_out[o] = tr[kl][fl];
+    // write side: thread t stores column t % 64 of rows t / 64 + 4 q
+    const int fw = threadIdx.x % FINAL_TF, kw0 = threadIdx.x / FINAL_TF;
+    int64_t orow[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) orow[q] = order[k0 + min(kw0 + 4 * q, np - 1)] * I + f0 + fw;
+    // one output at a time through the tile, written as row segments
+    for (int which = 0; which < 3; ++which) {
+        if ((which == 0 && !o.weight) || (which == 1 && !o.patch) || (which == 2 && !o.res)) continue;
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            tile[kl][fl0 + 8 * q] = which == 0 ? wv[q] : which == 1 ? pv[q] : make_float2(rv[q], 0.0f);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int kr = kw0 + 4 * q;
+            if (kr < np && fw < nf) {
+                const float2 v = tile[kr][fw];
+                if (which == 0)
+                    o.weight[orow[q]] = v;
+                else if (which == 1)
+                    o.patch[orow[q]] = v;
+                else
+                    o.res[orow[q]] = v.x;
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -1726,6 +1924,7 @@ struct LoopCtl {
     int32_t stopped;
     int32_t i_loop;       // current iteration (1-based), final count once stopped
     uint64_t rescored;    // FP32 form: rows re-scored in FP64, summed
+    uint64_t rechecked;   // FP32 form: rows refined again in FP64 (P-square choices), summed
     double gcs;           // gamma_change_save
     double nr[CTL_BINS];  // Nr, scaled each P-square update
 };
@@ -1794,7 +1993,8 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
     const double nr_t = t < CTL_BINS ? ctl->nr[t] : 0.0;
     const double gcs0 = ctl->gcs;
     const int i_loop0 = ctl->i_loop;
-    const uint32_t nresc = rescore_count ? *rescore_count : 0u;
+    const uint32_t nresc = rescore_count ? rescore_count[0] : 0u;
+    const uint32_t nrech = rescore_count ? rescore_count[1] : 0u;
     // gamma-change RMS: fixed assignment of block partials to threads, then a
     // fixed tree (deterministic)
     double acc = 0.0;
@@ -1809,7 +2009,10 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
         const double rms = sqrt(red[0] / (double)(cp.P > 1 ? cp.P : 1));
         const double change = rms - gcs0;
         ctl->gcs = rms;
-        if (rescore_count) ctl->rescored += nresc;
+        if (rescore_count) {
+            ctl->rescored += nresc;
+            ctl->rechecked += nrech;
+        }
         s_stop = (fabs(change) < cp.conv || i_loop0 >= cp.max_iter) ? 1 : 0;
     }
     __syncthreads();

@@ -56,11 +56,6 @@ static int arg_error(const std::string &msg) {
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// launch grid of the row-order output kernels (k_final_*)
-static inline dim3 final_grid(int64_t P, int64_t I) {
-    return dim3(blocks_for(P, FINAL_TP), blocks_for(I, FINAL_TF));
-}
-
 // Device buffer that frees itself.
 struct DBuf {
     void *p = nullptr;
@@ -293,8 +288,9 @@ struct ps2_session {
     HistRule hist_rule;
     // device state
     DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, wt, offset;
-    DBuf ex_k, ex_c, ex_coh, ex_nopt, ex_patch, ex_res;  // device export (row order)
-    DBuf dl_bulk, dl_vec;  // download scratch: finalized planes and row-order vectors
+    // outputs in the caller's layouts (row order), written by k_final_outputs
+    // at the end of every loop (and by the fit step)
+    DBuf o_weight, o_patch, o_res, o_vec, o_tix, o_grid;
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;
@@ -316,10 +312,11 @@ struct ps2_session {
     int64_t slot_launches[PS2_NUM_KERNEL_SLOTS] = {};
     // results of the last run
     bool ran = false;
+    bool loop_outputs = false;   // o_* hold the last loop's outputs (not a fit step's)
     int64_t i_loop = 0;
     double gamma_change_save = 0.0, loop_seconds = 0.0;
     std::vector<double> nr;
-    int64_t n_rescored = 0;
+    int64_t n_rescored = 0, n_rechecked = 0;
 
     ~ps2_session() {
         if (stream) {
@@ -572,6 +569,13 @@ int alloc_state(ps2_session *s) {
         }
         for (int k = 0; k < 7; ++k) cp.gsum += cp.g7[k];
     }
+    // output buffers in the caller's layouts
+    if ((rc = s->o_weight.alloc(sizeof(float2) * P * I))) return rc;
+    if ((rc = s->o_patch.alloc(sizeof(float2) * P * I))) return rc;
+    if ((rc = s->o_res.alloc(sizeof(float) * P * I))) return rc;
+    if ((rc = s->o_vec.alloc(sizeof(double) * 5 * P))) return rc;
+    if ((rc = s->o_tix.alloc(sizeof(int32_t) * P))) return rc;
+    if ((rc = s->o_grid.alloc(sizeof(float2) * C * I))) return rc;
     if ((rc = s->mids.alloc(sizeof(double) * 128))) return rc;
     std::vector<double> mids(128, 0.0);
     for (size_t k = 0; k < s->hist_rule.mids.size() && k < 128; ++k) mids[k] = s->hist_rule.mids[k];
@@ -582,7 +586,7 @@ int alloc_state(ps2_session *s) {
 // --- per-kernel timing ---------------------------------------------------------
 
 enum Slot { SLOT_FILL = 0, SLOT_CLAP_STRIP, SLOT_CLAP_COMBINE, SLOT_SCAN, SLOT_RESCORE, SLOT_REFINE,
-            SLOT_RMS, SLOT_WEIGHTS };
+            SLOT_RMS, SLOT_WEIGHTS, SLOT_OUTPUTS };
 
 struct SlotTimer {
     ps2_session *s;
@@ -815,23 +819,29 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     // inside the loop the refine kernel also runs the gamma-change RMS partials,
     // coh_prev <- coh and the coherence histogram (k_control reads them)
     RmsHistOut ro{};
+    ro.mids = s->mids.as<double>();
+    ro.hp.P = s->P;
+    ro.hp.equal = s->hist_rule.equal ? 1 : 0;
+    ro.hp.cutoff0 = s->hist_rule.cutoff0;
+    ro.hp.d = s->hist_rule.d;
+    ro.hp.nbins = PS2_COH_BINS;
     if (in_loop) {
         ro.coh_prev = s->cohprev.as<double>();
         ro.part = s->rms_part.as<double>();
         ro.hist = s->hist.as<uint32_t>();
-        ro.mids = s->mids.as<double>();
-        ro.hp.P = s->P;
-        ro.hp.equal = s->hist_rule.equal ? 1 : 0;
-        ro.hp.cutoff0 = s->hist_rule.cutoff0;
-        ro.hp.d = s->hist_rule.d;
-        ro.hp.nbins = PS2_COH_BINS;
     }
+    // FP32 form, P-square weighting: rows near a lookup / bin boundary are
+    // refined again in FP64 inside the refine kernel
+    RecheckIn rc{};
+    rc.bp64 = sizeof(R) == 4 ? s->bp64.as<double>() : nullptr;
+    rc.enabled = sizeof(R) == 4 && s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
     SlotTimer tm(s, SLOT_REFINE);
     hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                        s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                        s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
                        s->Cc.as<double>(),
-                       s->coh.as<double>(), s->nopt.as<double>(), fp, ro, s->gate);
+                       s->coh.as<double>(), s->nopt.as<double>(), fp, ro, rc,
+                       sizeof(R) == 4 ? s->rescore_count.as<uint32_t>() + 1 : nullptr, s->gate);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -873,8 +883,43 @@ int launch_iteration(ps2_session *s) {
     return PS2_OK;
 }
 
+// The outputs in the caller's layouts (k_final_outputs).  loop: every
+// write_pm1 argument of the loop; otherwise (the fit step) ph_patch, ph_res,
+// K/C/coh/N_opt and the trial index.
+template <typename R>
+int launch_outputs(ps2_session *s, bool loop) {
+    const int64_t P = s->P, I = s->I, C = s->C;
+    SlotTimer tm(s, SLOT_OUTPUTS);
+    FinalIn fi;
+    fi.K = s->K.as<double>();
+    fi.Cc = s->Cc.as<double>();
+    fi.coh = s->coh.as<double>();
+    fi.nopt = s->nopt.as<double>();
+    fi.coh_prev = loop ? s->cohprev.as<double>() : nullptr;
+    fi.Kw = s->Kprev.as<double>();
+    fi.Ww = s->Wprev.as<double>();
+    fi.tix = s->tix.as<int32_t>();
+    FinalOut fo;
+    fo.weight = loop ? s->o_weight.as<float2>() : nullptr;
+    fo.patch = s->o_patch.as<float2>();
+    fo.res = s->o_res.as<float>();
+    fo.vec = s->o_vec.as<double>();
+    fo.tix = s->o_tix.as<int32_t>();
+    fo.grid = loop ? s->o_grid.as<float2>() : nullptr;
+    fo.n_ifg_tiles = (int)blocks_for(I, FINAL_TF);
+    fo.n_pix_blocks = (int64_t)blocks_for(P, FINAL_TP) * fo.n_ifg_tiles;
+    fo.n_grid_ifg_tiles = (int)blocks_for(I, 32);
+    const int64_t n_grid_blocks = loop ? (int64_t)blocks_for(C, 32) * fo.n_grid_ifg_tiles : 0;
+    hipLaunchKernelGGL(k_final_outputs<R>, dim3((unsigned)(fo.n_pix_blocks + n_grid_blocks)), dim3(256), 0,
+                       s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->filt.as<cplx<R>>(),
+                       s->grid.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->d_order.as<int64_t>(), fi, fo, P,
+                       (int)I, C);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
 // The gamma loop from the initial state (native_stage2.rs:242-258) to the
-// stop decision.  Default: gamma_max_iterations iterations are captured as one
+// stop decision, and the outputs of the last iteration.  Default: gamma_max_iterations iterations are captured as one
 // HIP graph once per session (one iteration, launched that many times, when
 // there are more than LOOP_GRAPH_MAX) and launched; the control kernel's stop
 // flag turns the remaining kernels into early returns, and the host waits
@@ -906,12 +951,15 @@ int run_loop(ps2_session *s) {
     // iteration replayed): each graph launch costs ~30 us more than the
     // ~10 us between two of its kernels
     const int64_t per_graph = max_iter <= LOOP_GRAPH_MAX ? max_iter : 1;
+    // a graph that holds every iteration ends with the output kernel
+    const bool outputs_in_graph = per_graph == max_iter;
     if (!stepwise && !s->graph_exec) {
         // no allocation anywhere in the library while the capture is open
         std::unique_lock<std::shared_mutex> cap(capture_mutex());
         PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         rc = 0;
         for (int64_t it = 0; it < per_graph && !rc; ++it) rc = launch_iteration<R>(s);
+        if (!rc && outputs_in_graph) rc = launch_outputs<R>(s, true);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s->stream, &g);
         cap.unlock();
@@ -933,6 +981,10 @@ int run_loop(ps2_session *s) {
     }
     if (!stepwise) {
         for (int64_t it = 0; it < max_iter; it += per_graph) PS2_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+        if (!outputs_in_graph && (rc = launch_outputs<R>(s, true))) {
+            s->gate = nullptr;
+            return rc;
+        }
     } else {
         for (int64_t it = 0; it < max_iter; ++it) {
             const auto t_it = std::chrono::steady_clock::now();
@@ -957,6 +1009,8 @@ int run_loop(ps2_session *s) {
             }
             if (hc->stopped) break;
         }
+        s->gate = nullptr;
+        if ((rc = launch_outputs<R>(s, true))) return rc;
     }
     PS2_HIP(hipEventRecord(s->ev[1], s->stream));
     PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
@@ -970,77 +1024,42 @@ int run_loop(ps2_session *s) {
     s->gamma_change_save = hc->gcs;
     s->nr.assign(hc->nr, hc->nr + CTL_BINS);
     s->n_rescored = (int64_t)hc->rescored;
+    s->n_rechecked = (int64_t)hc->rechecked;
     s->ran = true;
+    s->loop_outputs = true;
     return PS2_OK;
 }
 
-// download helpers ------------------------------------------------------------
-// Every output is put in the caller's row order and layout on the device (into
-// session-owned scratch) and then moved by one bulk copy.
+// download: the outputs are already in the caller's layouts on the device
+// (k_final_outputs); each goes down in one bulk copy.
 
-int dl_scratch(ps2_session *s, size_t bulk_bytes) {
-    int rc;
-    if (s->dl_vec.bytes < sizeof(double) * s->P && (rc = s->dl_vec.alloc(sizeof(double) * s->P))) return rc;
-    if (s->dl_bulk.bytes < bulk_bytes && (rc = s->dl_bulk.alloc(bulk_bytes))) return rc;
-    return PS2_OK;
-}
-
-template <typename T>
-int download_row_vec(ps2_session *s, const DBuf &src, T *dst) {
-    if (!dst) return PS2_OK;
-    hipLaunchKernelGGL(k_unpermute<T>, dim3(blocks_for(s->P, 256)), dim3(256), 0, s->stream, src.as<T>(),
-                       s->d_order.as<int64_t>(), s->dl_vec.as<T>(), s->P);
-    PS2_LAUNCHED();
-    return staging().d2h(dst, s->dl_vec.p, sizeof(T) * s->P, s->stream);
-}
-
-template <typename R>
 int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
     const int64_t P = s->P, I = s->I, C = s->C;
-    int rc;
     const size_t plane = (size_t)P * I;
-    size_t bulk = (out->ph_patch || out->ph_res) ? plane * (sizeof(float2) + sizeof(float)) : 0;
-    if (loop_outputs && out->ph_weight) bulk = std::max(bulk, plane * sizeof(float2));
-    if (loop_outputs && out->ph_grid) bulk = std::max(bulk, (size_t)C * I * sizeof(float2));
-    if ((rc = dl_scratch(s, bulk))) return rc;
-    if ((rc = download_row_vec(s, s->K, out->k_ps))) return rc;
-    if ((rc = download_row_vec(s, s->Cc, out->c_ps))) return rc;
-    if ((rc = download_row_vec(s, s->coh, out->coh_ps))) return rc;
-    if ((rc = download_row_vec(s, s->nopt, out->n_opt))) return rc;
-    if ((rc = download_row_vec(s, s->tix, out->trial_ix))) return rc;
-    if (out->ph_patch || out->ph_res) {
-        float2 *pbuf = s->dl_bulk.as<float2>();
-        float *rbuf = reinterpret_cast<float *>(pbuf + plane);
-        hipLaunchKernelGGL(k_final_patch_res<R>, final_grid(P, I), dim3(256), 0, s->stream,
-                           s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
-                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
-                           s->d_order.as<int64_t>(), out->ph_patch ? pbuf : nullptr,
-                           out->ph_res ? rbuf : nullptr, P, (int)I, s->rowinv, s->C);
-        PS2_LAUNCHED();
-        if (out->ph_patch && (rc = staging().d2h(out->ph_patch, pbuf, sizeof(float2) * plane, s->stream)))
+    int rc;
+    double *const vec_dst[5] = {out->k_ps, out->c_ps, out->coh_ps, out->n_opt,
+                                loop_outputs ? out->coh_ps_save : nullptr};
+    for (int v = 0; v < 5; ++v)
+        if (vec_dst[v] && (rc = staging().d2h(vec_dst[v], s->o_vec.as<double>() + (size_t)v * P,
+                                              sizeof(double) * P, s->stream)))
             return rc;
-        if (out->ph_res && (rc = staging().d2h(out->ph_res, rbuf, sizeof(float) * plane, s->stream))) return rc;
-    }
+    if (out->trial_ix && (rc = staging().d2h(out->trial_ix, s->o_tix.p, sizeof(int32_t) * P, s->stream)))
+        return rc;
+    if (out->ph_patch && (rc = staging().d2h(out->ph_patch, s->o_patch.p, sizeof(float2) * plane, s->stream)))
+        return rc;
+    if (out->ph_res && (rc = staging().d2h(out->ph_res, s->o_res.p, sizeof(float) * plane, s->stream)))
+        return rc;
     if (!loop_outputs) return PS2_OK;
-    if ((rc = download_row_vec(s, s->cohprev, out->coh_ps_save))) return rc;
-    if (out->ph_weight) {
-        hipLaunchKernelGGL(k_final_weight<R>, final_grid(P, I), dim3(256), 0, s->stream,
-                           s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->Kprev.as<double>(),
-                           s->Wprev.as<double>(), s->d_order.as<int64_t>(), s->dl_bulk.as<float2>(), P, (int)I);
-        PS2_LAUNCHED();
-        if ((rc = staging().d2h(out->ph_weight, s->dl_bulk.p, sizeof(float2) * plane, s->stream))) return rc;
-    }
-    if (out->ph_grid && C * I > 0) {   // device [I][C] -> caller [C][I]
-        hipLaunchKernelGGL(k_grid_to_cells<R>, dim3((unsigned)((C + 31) / 32), (unsigned)((I + 31) / 32)),
-                           dim3(256), 0, s->stream, s->grid.as<cplx<R>>(), s->dl_bulk.as<float2>(), C, (int)I);
-        PS2_LAUNCHED();
-        if ((rc = staging().d2h(out->ph_grid, s->dl_bulk.p, sizeof(float2) * C * I, s->stream))) return rc;
-    }
+    if (out->ph_weight && (rc = staging().d2h(out->ph_weight, s->o_weight.p, sizeof(float2) * plane, s->stream)))
+        return rc;
+    if (out->ph_grid && (rc = staging().d2h(out->ph_grid, s->o_grid.p, sizeof(float2) * C * I, s->stream)))
+        return rc;
     if (out->nr)
         for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
     out->gamma_change_save = s->gamma_change_save;
     out->i_loop = s->i_loop;
     out->n_rescored = s->n_rescored;
+    out->n_rechecked = s->n_rechecked;
     out->loop_seconds = s->loop_seconds;
     return PS2_OK;
 }
@@ -1190,13 +1209,12 @@ int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_se
 int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out) {
     if (!s || !out) return arg_error("session or output is NULL");
     std::lock_guard<std::mutex> lk(s->mu);
-    if (!s->ran) {
-        set_error("session_download before session_run");
+    if (!s->ran || !s->loop_outputs) {
+        set_error("session_download before session_run (or after a single step)");
         return PS2_E_STATE;
     }
     PS2_HIP(hipSetDevice(s->device));
-    return s->form == PS2_FORM_FP64 ? download_outputs<double>(s, out, true)
-                                    : download_outputs<float>(s, out, true);
+    return download_outputs(s, out, true);
 }
 
 void ps2_hip_session_destroy(ps2_session *s) { delete s; }
@@ -1208,44 +1226,22 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out) {
         set_error("session_export_device before session_run");
         return PS2_E_STATE;
     }
+    if (!s->loop_outputs) {
+        set_error("session_export_device after a single step: run the loop again");
+        return PS2_E_STATE;
+    }
     PS2_HIP(hipSetDevice(s->device));
     const int64_t P = s->P, I = s->I;
-    int rc;
-    for (DBuf *b : {&s->ex_k, &s->ex_c, &s->ex_coh, &s->ex_nopt})
-        if (!b->p && (rc = b->alloc(sizeof(double) * P))) return rc;
-    if (!s->ex_patch.p && (rc = s->ex_patch.alloc(sizeof(float2) * P * I))) return rc;
-    if (!s->ex_res.p && (rc = s->ex_res.alloc(sizeof(float) * P * I))) return rc;
-    const unsigned nb = blocks_for(P, 256);
-    const std::pair<DBuf *, DBuf *> vecs[] = {
-        {&s->K, &s->ex_k}, {&s->Cc, &s->ex_c}, {&s->coh, &s->ex_coh}, {&s->nopt, &s->ex_nopt}};
-    for (const auto &pr : vecs) {
-        hipLaunchKernelGGL(k_unpermute<double>, dim3(nb), dim3(256), 0, s->stream, pr.first->as<double>(),
-                           s->d_order.as<int64_t>(), pr.second->as<double>(), P);
-        PS2_LAUNCHED();
-    }
-    if (s->form == PS2_FORM_FP64)
-        hipLaunchKernelGGL(k_final_patch_res<double>, final_grid(P, I), dim3(256), 0, s->stream,
-                           s->phnm.as<cplx<double>>(), s->bp.as<double>(), s->filt.as<cplx<double>>(),
-                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
-                           s->d_order.as<int64_t>(), s->ex_patch.as<float2>(), s->ex_res.as<float>(), P,
-                           (int)I, s->rowinv, s->C);
-    else
-        hipLaunchKernelGGL(k_final_patch_res<float>, final_grid(P, I), dim3(256), 0, s->stream,
-                           s->phnm.as<cplx<float>>(), s->bp.as<float>(), s->filt.as<cplx<float>>(),
-                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
-                           s->d_order.as<int64_t>(), s->ex_patch.as<float2>(), s->ex_res.as<float>(), P,
-                           (int)I, s->rowinv, s->C);
-    PS2_LAUNCHED();
     PS2_HIP(hipStreamSynchronize(s->stream));
     out->device = s->device;
     out->n_ps = P;
     out->n_ifg = I;
-    out->k_ps = s->ex_k.as<double>();
-    out->c_ps = s->ex_c.as<double>();
-    out->coh_ps = s->ex_coh.as<double>();
-    out->n_opt = s->ex_nopt.as<double>();
-    out->ph_patch = s->ex_patch.as<float>();
-    out->ph_res = s->ex_res.as<float>();
+    out->k_ps = s->o_vec.as<double>();
+    out->c_ps = s->o_vec.as<double>() + P;
+    out->coh_ps = s->o_vec.as<double>() + 2 * P;
+    out->n_opt = s->o_vec.as<double>() + 3 * P;
+    out->ph_patch = s->o_patch.as<float>();
+    out->ph_res = s->o_res.as<float>();
     for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
     out->gamma_change_save = s->gamma_change_save;
     out->i_loop = s->i_loop;
@@ -1396,14 +1392,17 @@ int ps2_hip_step_fit(ps2_session *s, const void *ph_filt, ps2_patch_out *out) {
     PS2_LAUNCHED();
     rc = s->form == PS2_FORM_FP64 ? launch_fit<double>(s) : launch_fit<float>(s);
     if (rc) return rc;
-    uint32_t cnt = 0;
+    rc = s->form == PS2_FORM_FP64 ? launch_outputs<double>(s, false) : launch_outputs<float>(s, false);
+    if (rc) return rc;
+    s->loop_outputs = false;
+    uint32_t cnt[2] = {0u, 0u};
     if (s->form == PS2_FORM_FP32)
-        PS2_HIP(hipMemcpyAsync(&cnt, s->rescore_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+        PS2_HIP(hipMemcpyAsync(cnt, s->rescore_count.p, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
     PS2_HIP(hipStreamSynchronize(s->stream));
     collect_timers(s);
-    out->n_rescored = cnt;
-    return s->form == PS2_FORM_FP64 ? download_outputs<double>(s, out, false)
-                                    : download_outputs<float>(s, out, false);
+    out->n_rescored = cnt[0];
+    out->n_rechecked = cnt[1];
+    return download_outputs(s, out, false);
 }
 
 }  // extern "C"

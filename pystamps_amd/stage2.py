@@ -118,6 +118,7 @@ class _OutHolder:
         d["gamma_change_save"] = self.s.gamma_change_save
         d["i_loop"] = int(self.s.i_loop)
         d["n_rescored"] = int(self.s.n_rescored)
+        d["n_rechecked"] = int(self.s.n_rechecked)
         d["loop_seconds"] = float(self.s.loop_seconds)
         d["Nr_max_nz_ix"] = prep.nr_max_nz_ix
         d["step_number"] = 1.0
@@ -250,6 +251,7 @@ class Stage2Session:
         _check(self.L.ps2_hip_step_fit(self.h, f.ctypes.data, C.byref(out.s)))
         d = out.d
         d["n_rescored"] = int(out.s.n_rescored)
+        d["n_rechecked"] = int(out.s.n_rechecked)
         return d
 
     def close(self):

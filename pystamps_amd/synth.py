@@ -14,6 +14,9 @@ dropping the master equals the configured one.
 from __future__ import annotations
 
 import math
+import os
+import threading
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, replace
 
 import numpy as np
@@ -87,6 +90,24 @@ def _atmosphere(rng, extent_m, corr_m, std, spacing=100.0):
     return field * (std / sd if sd > 0 else 0.0)
 
 
+def _bilinear_add(out, cells, n_side, x, y, tmp, spacing=100.0):
+    """out[p, f] += bilinear sample of interferogram f's field at (x_p, y_p)
+    for every f at once; cells [n*n][I] holds the fields lattice-point-major,
+    so each corner is one contiguous row gather per pixel."""
+    gx = x / spacing
+    gy = y / spacing
+    i0 = np.floor(gy).astype(np.int64)
+    j0 = np.floor(gx).astype(np.int64)
+    fy = gy - i0
+    fx = gx - j0
+    q = i0 * n_side + j0
+    for w, off in (((1 - fy) * (1 - fx), 0), ((1 - fy) * fx, 1), (fy * (1 - fx), n_side),
+                   (fy * fx, n_side + 1)):
+        np.take(cells, q + off, axis=0, out=tmp)
+        tmp *= w[:, None]
+        out += tmp
+
+
 def _bilinear(field, x, y, spacing=100.0):
     gx = x / spacing
     gy = y / spacing
@@ -98,8 +119,21 @@ def _bilinear(field, x, y, spacing=100.0):
             + fy * ((1 - fx) * field[i0 + 1, j0] + fx * field[i0 + 1, j0 + 1]))
 
 
-def generate(cfg: SynthConfig, chunk: int = 250_000) -> RawPatch:
-    """Raw stage-2 inputs for one patch (ph1/ps1/bp1/parms content)."""
+GEN_CHUNK = 32_768   # rows per noise stream (fixed: the data does not depend on the thread count)
+
+
+def _threads() -> int:
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, min(n, 16))
+
+
+def generate(cfg: SynthConfig) -> RawPatch:
+    """Raw stage-2 inputs for one patch (ph1/ps1/bp1/parms content).  Row
+    chunks of GEN_CHUNK pixels draw their phase noise from their own child
+    stream of the seed and are built on host threads."""
     rng = np.random.default_rng(cfg.seed)
     P, I = cfg.n_ps, cfg.n_ifg
     x = rng.uniform(0.0, cfg.extent_m, P)
@@ -112,24 +146,50 @@ def generate(cfg: SynthConfig, chunk: int = 250_000) -> RawPatch:
     xn = (x - x.min()) / span
     scale = 4.0 * math.pi / (LAMBDA_M * SLANT_RANGE_M * math.sin(math.radians(INCIDENCE_DEG)))
 
+    # the master column (index 0) is written by the chunks too: first touches
+    # of these large arrays then happen on every thread, not on one
     ph = np.empty((P, I + 1), dtype=np.complex64)
     bperp_mat = np.empty((P, I + 1), dtype=np.float64)
-    ph[:, 0] = 1.0
-    bperp_mat[:, 0] = 0.0
-    for a in range(0, P, chunk):
-        b = min(P, a + chunk)
+    n_side = atmo[0].shape[1]
+    cells = np.ascontiguousarray(np.stack(atmo).reshape(I, -1).T)   # [n*n][I]
+    n_chunks = (P + GEN_CHUNK - 1) // GEN_CHUNK
+    streams = np.random.SeedSequence(cfg.seed).spawn(n_chunks)
+    local = threading.local()
+
+    def buffers(m):
+        # per-thread work arrays, reused by every chunk of the thread
+        if getattr(local, "m", 0) < m:
+            local.m = m
+            local.phase = np.empty((m, I))
+            local.tmp = np.empty((m, I))
+        return local.phase[:m], local.tmp[:m]
+
+    def chunk(c):
+        a, b = c * GEN_CHUNK, min(P, (c + 1) * GEN_CHUNK)
+        crng = np.random.default_rng(streams[c])
+        phase, tmp = buffers(b - a)
         bp = bperp_ifg[None, :] * (1.0 + 0.02 * xn[a:b, None])
-        phase = scale * bp * k_dem[a:b, None]
-        for f in range(I):
-            phase[:, f] += _bilinear(atmo[f], x[a:b], y[a:b])
+        np.multiply(scale * bp, k_dem[a:b, None], out=phase)
+        _bilinear_add(phase, cells, n_side, x[a:b], y[a:b], tmp)
         coh = coherent[a:b]
-        noise = rng.uniform(-math.pi, math.pi, (b - a, I))
+        crng.random(out=tmp)                              # noise: uniform [-pi, pi),
+        tmp *= 2.0 * math.pi                              # N(0, sigma) where coherent
+        tmp -= math.pi
         nc = int(coh.sum())
         if nc:
-            noise[coh] = rng.normal(0.0, cfg.noise_std, (nc, I))
-        phase += noise
-        ph[a:b, 1:] = np.exp(1j * phase).astype(np.complex64)
+            tmp[coh] = crng.normal(0.0, cfg.noise_std, (nc, I))
+        phase += tmp
+        ph[a:b, 0] = 1.0
+        bperp_mat[a:b, 0] = 0.0
+        out = ph[a:b, 1:]
+        np.cos(phase, out=tmp)
+        out.real = tmp
+        np.sin(phase, out=tmp)
+        out.imag = tmp
         bperp_mat[a:b, 1:] = bp
+
+    with ThreadPoolExecutor(max_workers=_threads()) as ex:
+        list(ex.map(chunk, range(n_chunks)))
     xy = np.empty((P, 3), dtype=np.float32)
     xy[:, 0] = np.arange(1, P + 1, dtype=np.float32)
     xy[:, 1] = x.astype(np.float32)
