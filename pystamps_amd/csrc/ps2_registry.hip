@@ -26,8 +26,23 @@ constexpr double kPi4 = 0.78539816339744830962;
 constexpr double kNearMaxTol = 2.0e-4;
 
 struct PyFitParams {
-    int64_t P;
+    int64_t P;      // rows in this launch
+    int64_t row0;   // first row of this launch
     int I, T, tn, rowinv;
+};
+
+// Element (row k, interferogram i) of the fit's arrays, for either layout:
+// ifg-major planes [I][n] (c_i = n, c_k = 1) or the caller's row-major
+// [n][I] (c_i = 1, c_k = I).  bperp has b_k = 0 when every row shares it.
+struct RowView {
+    const double2 *cpx;
+    int64_t c_i, c_k;
+    const double *bp;
+    int64_t b_i, b_k;
+    float2 *res;   // may be null
+    int64_t r_i, r_k;
+    __device__ double2 v(int64_t k, int i) const { return cpx[(int64_t)i * c_i + k * c_k]; }
+    __device__ double b(int64_t k, int i) const { return bp[(int64_t)i * b_i + k * b_k]; }
 };
 
 // exp(-i ph) * (re + i im)
@@ -39,24 +54,23 @@ __device__ inline void rot(double re, double im, double ph, double &orr, double 
 }
 
 // _ps_topofit_refine_candidate (ported.py:2480-2513) of row k; writes the
-// residual when res != nullptr.
-__device__ void py_refine(const double2 *cpx, const double *bp, int64_t bs, int64_t P, int64_t k,
-                          int I, double den_lin, double k0, double &K, double &C, double &coh,
-                          float2 *res, int64_t rs) {
+// residual when write_res.
+__device__ void py_refine(const RowView &rv, int64_t k, int I, double den_lin, double k0, double &K,
+                          double &C, double &coh, bool write_res) {
     double ore = 0.0, oim = 0.0;
     for (int i = 0; i < I; ++i) {
-        const double2 v = cpx[(int64_t)i * P + k];
+        const double2 v = rv.v(k, i);
         if (v.x == 0.0 && v.y == 0.0) continue;
         double rr, ri;
-        rot(v.x, v.y, k0 * bp[i * bs], rr, ri);
+        rot(v.x, v.y, k0 * rv.b(k, i), rr, ri);
         ore += rr;
         oim += ri;
     }
     double num = 0.0;
     for (int i = 0; i < I; ++i) {
-        const double2 v = cpx[(int64_t)i * P + k];
+        const double2 v = rv.v(k, i);
         if (v.x == 0.0 && v.y == 0.0) continue;
-        const double b = bp[i * bs];
+        const double b = rv.b(k, i);
         double rr, ri;
         rot(v.x, v.y, k0 * b, rr, ri);
         const double w = hypot(v.x, v.y);
@@ -66,15 +80,15 @@ __device__ void py_refine(const double2 *cpx, const double *bp, int64_t bs, int6
     const double kk = k0 + num / den_lin;
     double mr = 0.0, mi = 0.0, d2 = 0.0;
     for (int i = 0; i < I; ++i) {
-        const double2 v = cpx[(int64_t)i * P + k];
+        const double2 v = rv.v(k, i);
         double rr = 0.0, ri = 0.0;
         if (v.x != 0.0 || v.y != 0.0) {
-            rot(v.x, v.y, kk * bp[i * bs], rr, ri);
+            rot(v.x, v.y, kk * rv.b(k, i), rr, ri);
             mr += rr;
             mi += ri;
             d2 += hypot(rr, ri);
         }
-        if (res) res[(int64_t)i * rs + k] = make_float2((float)rr, (float)ri);
+        if (write_res) rv.res[(int64_t)i * rv.r_i + k * rv.r_k] = make_float2((float)rr, (float)ri);
     }
     if (d2 == 0.0) d2 = 1.0;
     K = kk;
@@ -82,26 +96,22 @@ __device__ void py_refine(const double2 *cpx, const double *bp, int64_t bs, int6
     coh = hypot(mr, mi) / d2;
 }
 
-// one thread per row; cpx, bp ifg-major ([I][P]; bp [I] when rowinv);
-// cohs [T][P] scratch; res [I][P] complex64 (may be null)
-__global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ cpx,
-                                                    const double *__restrict__ bpv,
-                                                    double *__restrict__ cohs,
+// one thread per row k = row0 + local; cohs [T][P] scratch of the launch's
+// rows; Kout/Cout/cohout indexed by k
+__global__ __launch_bounds__(256) void k_py_topofit(RowView rv, double *__restrict__ cohs,
                                                     double *__restrict__ Kout,
                                                     double *__restrict__ Cout,
-                                                    double *__restrict__ cohout,
-                                                    float2 *__restrict__ res, PyFitParams p) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= p.P) return;
+                                                    double *__restrict__ cohout, PyFitParams p) {
+    const int64_t local = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= p.P) return;
+    const int64_t k = p.row0 + local;
     const int I = p.I, T = p.T;
-    const double *bp = p.rowinv ? bpv : bpv + k;
-    const int64_t bs = p.rowinv ? 1 : p.P;
     double mn = INFINITY, mx = -INFINITY, denom = 0.0, den_lin = 0.0;
     int m = 0;
     for (int i = 0; i < I; ++i) {
-        const double2 v = cpx[(int64_t)i * p.P + k];
+        const double2 v = rv.v(k, i);
         if (v.x == 0.0 && v.y == 0.0) continue;
-        const double b = bp[i * bs];
+        const double b = rv.b(k, i);
         mn = fmin(mn, b);
         mx = fmax(mx, b);
         const double w = hypot(v.x, v.y);
@@ -111,8 +121,8 @@ __global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ 
     }
     if (m == 0) {
         Kout[k] = Cout[k] = cohout[k] = __builtin_nan("");
-        if (res)
-            for (int i = 0; i < I; ++i) res[(int64_t)i * p.P + k] = make_float2(0.0f, 0.0f);
+        if (rv.res)
+            for (int i = 0; i < I; ++i) rv.res[(int64_t)i * rv.r_i + k * rv.r_k] = make_float2(0.0f, 0.0f);
         return;
     }
     double range = mx - mn;
@@ -125,15 +135,15 @@ __global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ 
         const double tm = (double)(t - p.tn);
         double sr = 0.0, si = 0.0;
         for (int i = 0; i < I; ++i) {
-            const double2 v = cpx[(int64_t)i * p.P + k];
+            const double2 v = rv.v(k, i);
             if (v.x == 0.0 && v.y == 0.0) continue;
             double rr, ri;
-            rot(v.x, v.y, (bp[i * bs] / range * kPi4) * tm, rr, ri);
+            rot(v.x, v.y, (rv.b(k, i) / range * kPi4) * tm, rr, ri);
             sr += rr;
             si += ri;
         }
         const double c = hypot(sr, si) / denom;
-        cohs[(int64_t)t * p.P + k] = c;
+        cohs[(int64_t)t * p.P + local] = c;
         if (c > cmax) {
             cmax = c;
             amax = t;
@@ -144,16 +154,16 @@ __global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ 
     int n = 0, first = -1, last = -1, coarse_best = -1, refined_best = -1;
     double coarse_v = -INFINITY, refined_v = -INFINITY;
     for (int t = 0; t < T; ++t) {
-        const double c = cohs[(int64_t)t * p.P + k];
+        const double c = cohs[(int64_t)t * p.P + local];
         bool lm;
         if (T == 1)
             lm = true;
         else if (t == 0)
-            lm = c >= cohs[(int64_t)1 * p.P + k];
+            lm = c >= cohs[(int64_t)1 * p.P + local];
         else if (t == T - 1)
-            lm = c >= cohs[(int64_t)(T - 2) * p.P + k];
+            lm = c >= cohs[(int64_t)(T - 2) * p.P + local];
         else
-            lm = c >= cohs[(int64_t)(t - 1) * p.P + k] && c >= cohs[(int64_t)(t + 1) * p.P + k];
+            lm = c >= cohs[(int64_t)(t - 1) * p.P + local] && c >= cohs[(int64_t)(t + 1) * p.P + local];
         if (!(lm && c >= cmax - kNearMaxTol)) continue;
         if (n == 0) first = t;
         last = t;
@@ -163,8 +173,7 @@ __global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ 
             coarse_best = t;
         }
         double Kc, Cc, rc;
-        py_refine(cpx, bp, bs, p.P, k, I, den_lin, kPi4 / range * (double)(t - p.tn), Kc, Cc, rc,
-                  nullptr, 0);
+        py_refine(rv, k, I, den_lin, kPi4 / range * (double)(t - p.tn), Kc, Cc, rc, false);
         if (rc > refined_v) {
             refined_v = rc;
             refined_best = t;
@@ -178,8 +187,7 @@ __global__ __launch_bounds__(256) void k_py_topofit(const double2 *__restrict__ 
     else
         sel = refined_best;
     double K, C, coh;
-    py_refine(cpx, bp, bs, p.P, k, I, den_lin, kPi4 / range * (double)(sel - p.tn), K, C, coh, res,
-              p.P);
+    py_refine(rv, k, I, den_lin, kPi4 / range * (double)(sel - p.tn), K, C, coh, rv.res != nullptr);
     Kout[k] = K;
     Cout[k] = C;
     cohout[k] = coh;
@@ -309,9 +317,11 @@ extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int3
     REG_HIP(cs.create());
     REG_HIP(hipMemcpyAsync(dc.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, cs.s));
     REG_HIP(hipMemcpyAsync(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, cs.s));
-    PyFitParams prm{P, (int)I, T, tn, row_invariant ? 1 : 0};
-    hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, cs.s, dc.p, db.p,
-                       dcoh_t.p, dK.p, dC.p, dcoh.p, res_out ? dres.p : nullptr, prm);
+    PyFitParams prm{P, 0, (int)I, T, tn, row_invariant ? 1 : 0};
+    const RowView rv{dc.p, P, 1, db.p, row_invariant ? 1 : P, row_invariant ? 0 : 1,
+                     res_out ? dres.p : nullptr, P, 1};
+    hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, cs.s, rv, dcoh_t.p,
+                       dK.p, dC.p, dcoh.p, prm);
     REG_HIP(hipGetLastError());
     REG_HIP(hipMemcpyAsync(k_out, dK.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
     REG_HIP(hipMemcpyAsync(c_out, dC.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
