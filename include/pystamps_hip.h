@@ -174,6 +174,25 @@ int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int32_t row_inv
                          int64_t n_rows, int64_t n_ifg, double n_trial_wraps, int32_t device,
                          double *k_out, double *c_out, double *coh_out, float *res_out);
 
+/* Allocation-free variant of ps2_hip_topofit_rows.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs and outputs are DEVICE pointers in the same row-major layouts as
+ * ps2_hip_topofit_rows' host arrays (cpx [n_rows][n_ifg] complex128; bperp
+ * [n_rows][n_ifg], or [n_ifg] when row_invariant; k/c/coh [n_rows]; res
+ * [n_rows][n_ifg] complex64 or NULL); results equal ps2_hip_topofit_rows'.
+ * The per-row trial coherences live in a fixed 32 MiB device array of the
+ * library (one per GPU, held until the process ends), so the call makes no
+ * hipMalloc, hipFree or other allocation; it rejects trial counts above
+ * 16384 (2*ceil(8*n_trial_wraps)+1) with PS2_E_ARG.  Work is queued on
+ * `stream` (a hipStream_t; NULL = the null stream) and the call returns after
+ * that stream has completed.  Calls that overlap on one GPU (other host
+ * threads, other streams) share the array and race. */
+int ps2_hip_topofit_rows_noalloc(const double *d_cpx, const double *d_bperp, int32_t row_invariant,
+                                 int64_t n_rows, int64_t n_ifg, double n_trial_wraps,
+                                 int32_t device, void *stream, double *d_k_out, double *d_c_out,
+                                 double *d_coh_out, float *d_res_out);
+
 /* stage2_grid_accumulate (_stage2_grid_accumulate_cpu,
  * pystamps/kernels/accelerated.py:183-200: np.add.at per ifg, rows in index
  * order, complex64): ph_weight [n_rows][n_ifg] complex64, grid_lin [n_rows]

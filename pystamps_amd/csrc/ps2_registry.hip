@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -337,6 +338,95 @@ extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int3
                 res_out[2 * (k * I + i) + 1] = hr[i * P + k].y;
             }
     }
+    return PS2_OK;
+}
+
+// ---- allocation-free topofit ----------------------------------------------
+// ps2_hip_topofit_rows_noalloc keeps the per-row trial coherences of one chunk
+// of rows ([T][chunk] doubles) in this array instead of a hipMalloc'd buffer.
+// Each GPU the library runs on holds its own copy until the process ends.
+// Host code uses the array only through the per-device address that
+// hipGetSymbolAddress returns (noalloc_scratch_address).
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr int kNoAllocRowBlock = 256;   // rows per workgroup of k_py_topofit
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+
+// address of `scratch` on the current device (looked up once per device)
+int noalloc_scratch_address(int device, double **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("topofit_rows_noalloc: device ordinal " + std::to_string(device) + " out of range");
+        return PS2_E_ARG;
+    }
+    if (!g_scratch_addr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("topofit_rows_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return PS2_E_DEVICE;
+        }
+        g_scratch_addr[device] = ptr;
+    }
+    *out = static_cast<double *>(g_scratch_addr[device]);
+    return PS2_OK;
+}
+
+}  // namespace
+
+extern "C" int ps2_hip_topofit_rows_noalloc(const double *d_cpx, const double *d_bperp,
+                                            int32_t row_invariant, int64_t n_rows, int64_t n_ifg,
+                                            double n_trial_wraps, int32_t device, void *stream,
+                                            double *d_k_out, double *d_c_out, double *d_coh_out,
+                                            float *d_res_out) {
+    if (n_rows < 0 || n_ifg <= 0 || n_ifg > (1 << 20) ||
+        (n_rows > 0 && (!d_cpx || !d_bperp || !d_k_out || !d_c_out || !d_coh_out))) {
+        set_error("topofit_rows_noalloc: invalid argument");
+        return PS2_E_ARG;
+    }
+    if (!(n_trial_wraps >= 0.0) || !std::isfinite(n_trial_wraps) || n_trial_wraps > 1.0e6) {
+        set_error("topofit_rows_noalloc: n_trial_wraps must be finite and >= 0");
+        return PS2_E_ARG;
+    }
+    if (n_rows > (((int64_t)1 << 62) / n_ifg)) {
+        set_error("topofit_rows_noalloc: n_rows * n_ifg too large");
+        return PS2_E_ARG;
+    }
+    const int tn = (int)std::ceil(8.0 * n_trial_wraps);
+    const int64_t T = 2 * (int64_t)tn + 1;
+    // rows per launch: whole workgroups whose [T][rows] coherences fit the scratch
+    const int64_t cap_rows = (int64_t)(kNoAllocScratchBytes / sizeof(double)) / T / kNoAllocRowBlock *
+                             kNoAllocRowBlock;
+    if (cap_rows < kNoAllocRowBlock) {
+        set_error("topofit_rows_noalloc: " + std::to_string(T) + " trials need " +
+                  std::to_string(T * kNoAllocRowBlock * (int64_t)sizeof(double)) +
+                  " bytes of scratch per 256 rows; the scratch holds " +
+                  std::to_string(kNoAllocScratchBytes) + " bytes (at most " +
+                  std::to_string(kNoAllocScratchBytes / sizeof(double) / kNoAllocRowBlock) + " trials)");
+        return PS2_E_ARG;
+    }
+    if (n_rows == 0) return PS2_OK;
+    int rc = set_device(device);
+    if (rc) return rc;
+    double *cohs = nullptr;
+    if ((rc = noalloc_scratch_address(device, &cohs))) return rc;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // the caller's row-major layouts: element (k, i) at k * n_ifg + i
+    const RowView rv{reinterpret_cast<const double2 *>(d_cpx), 1, n_ifg, d_bperp, 1,
+                     row_invariant ? 0 : n_ifg, reinterpret_cast<float2 *>(d_res_out), 1, n_ifg};
+    for (int64_t row0 = 0; row0 < n_rows; row0 += cap_rows) {
+        const int64_t rows = std::min(cap_rows, n_rows - row0);
+        PyFitParams prm{rows, row0, (int)n_ifg, (int)T, tn, row_invariant ? 1 : 0};
+        hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((rows + kNoAllocRowBlock - 1) / kNoAllocRowBlock)),
+                           dim3(kNoAllocRowBlock), 0, st, rv, cohs, d_k_out, d_c_out, d_coh_out, prm);
+        REG_HIP(hipGetLastError());
+    }
+    REG_HIP(hipStreamSynchronize(st));
     return PS2_OK;
 }
 
