@@ -52,7 +52,7 @@ class _Out(C.Structure):
         ("ph_weight", C.c_void_p), ("nr", _f64p), ("gamma_change_save", C.c_double),
         ("i_loop", C.c_int64), ("trial_ix", _i32p), ("trial_margin", _f32p),
         ("weighting_last", _f64p), ("k_prev", _f64p), ("trial_margin_min", _f32p),
-        ("decision_margin_min", _f64p),
+        ("decision_margin_min", _f64p), ("refine_margin_min", _f64p),
     ]
 
 
@@ -213,6 +213,7 @@ def run(prep, mode=REF32, threads=0, diagnostics=True):
         o["k_prev"] = np.zeros(P)
         o["trial_margin_min"] = np.zeros(P, np.float32)
         o["decision_margin_min"] = np.zeros(P)
+        o["refine_margin_min"] = np.zeros(P)
     s = _Out()
     s.k_ps, s.c_ps = _p(o["K_ps"], _f64p), _p(o["C_ps"], _f64p)
     s.coh_ps, s.n_opt = _p(o["coh_ps"], _f64p), _p(o["N_opt"], _f64p)
@@ -229,6 +230,7 @@ def run(prep, mode=REF32, threads=0, diagnostics=True):
         s.k_prev = _p(o["k_prev"], _f64p)
         s.trial_margin_min = _p(o["trial_margin_min"], _f32p)
         s.decision_margin_min = _p(o["decision_margin_min"], _f64p)
+        s.refine_margin_min = _p(o["refine_margin_min"], _f64p)
     rc = L.ora_stage2_run(C.byref(inp.s), C.byref(s), mode, threads)
     if rc != 0:
         raise RuntimeError(f"oracle stage 2 failed with code {rc}")

@@ -973,9 +973,9 @@ int ora_fit_rows(const ora_stage2_in *in, const void *ph_patch, double *k_ps, do
                  float *trial_margin, int mode, int threads) {
     if (mode == ORA_MODE_REF64)
         return fit_64(in, (const double *)ph_patch, k_ps, c_ps, coh_ps, n_opt, (double *)ph_res,
-                      trial_ix, trial_margin, threads);
+                      trial_ix, trial_margin, NULL, threads);
     return fit_32(in, (const float *)ph_patch, k_ps, c_ps, coh_ps, n_opt, (float *)ph_res,
-                  trial_ix, trial_margin, threads);
+                  trial_ix, trial_margin, NULL, threads);
 }
 
 void ora_snr_weighting(const float *amp, const void *ph_res, int64_t n_ps, int64_t n_ifg,

@@ -76,6 +76,8 @@ typedef struct {
     double *decision_margin_min; /* [n_ps] smallest distance of coh to a P-square lookup
                                     (n + 1/2)/1000 or histogram bin boundary over the
                                     iterations whose coherence sets weights (not the last) */
+    double *refine_margin_min; /* [n_ps] smallest distance of a residual phase of the
+                                  least-squares step to the +-pi cut of arg() */
 } ora_stage2_out;
 
 /* Whole gamma loop (native_stage2.rs:242-406). Returns 0 or a negative code. */

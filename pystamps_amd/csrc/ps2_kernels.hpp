@@ -1397,7 +1397,7 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
                                     const cplx<R> *__restrict__ offset, double *__restrict__ Kout,
                                     double *__restrict__ Cout, double *__restrict__ cohout,
                                     double *__restrict__ nopt, const FitParams &prm,
-                                    float *max_angle) {
+                                    float *max_angle, float *cut_margin) {
     const int t = tix[k];
     if (t < 0) {
         Kout[k] = __builtin_nan("");
@@ -1418,6 +1418,9 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     const cplx<R> off = offset[k];
     const R ocr = off.re, oci = -off.im;
     double num = 0.0, den = 0.0;
+    // largest |k0 bp| and smallest distance of arg(r conj(offset)) to the
+    // +-pi cut of this pass (the f64 re-decision takes rows near the cut)
+    R amax1 = (R)0, cut = (R)RT<R>::pi4() * (R)4;
     // both passes keep the loads of the next two interferograms in flight
     // while the current one is processed; the loops are unrolled by the
     // period of that register rotation so the copies become renaming (a copy
@@ -1443,9 +1446,13 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
         const double wgt = (double)RT<R>::abs_(v.re, v.im);
         const double wb = wgt * (double)bi;
         den += wb * wb;
-        const cplx<R> r = rotate_neg<R>(v, k0 * bi);
+        const R ang = k0 * bi;
+        const cplx<R> r = rotate_neg<R>(v, ang);
         const R pr = r.re * ocr - r.im * oci, pim = r.re * oci + r.im * ocr;
-        num += wb * (wgt * (double)RT<R>::atan2_(pim, pr));
+        const R a = RT<R>::atan2_(pim, pr);
+        num += wb * (wgt * (double)a);
+        amax1 = fmax(amax1, fabs(ang));
+        cut = fmin(cut, (R)RT<R>::pi4() * (R)4 - fabs(a));
     }
     if (den == 0.0) den = 1.0;
     k0 = (R)((double)k0 + num / den);
@@ -1483,7 +1490,8 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     const double coh = sqrt(mre * mre + mim * mim) / dn;
     cohout[k] = coh;
     nopt[k] = 1.0;
-    *max_angle = (float)amax;
+    *max_angle = (float)fmax(amax, amax1);
+    *cut_margin = (float)cut;
     return coh;
 }
 
@@ -1538,7 +1546,8 @@ __device__ inline void rms_hist_tail(const RmsHistOut &ro, bool mine, int64_t k,
 // then the FP64 values.
 struct RecheckIn {
     const double *bp64;   // [I][P] (or [I] with invariant rows): the caller's f64 bperp
-    int enabled;          // FP32 form with P-square weighting
+    int enabled;          // FP32 form
+    int psquare;          // P-square weighting: also rows near a lookup / bin boundary
 };
 
 // distance of coh to the nearest lookup or histogram boundary
@@ -1662,27 +1671,32 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool row = k < prm.P;
     double coh = 0.0;
-    float amax = 0.0f;
+    float amax = 0.0f, cut = 4.0f;
     if (row) {
         if (prm.rowinv)
             coh = refine_row<R, true>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt,
-                                      prm, &amax);
+                                      prm, &amax, &cut);
         else
             coh = refine_row<R, false>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt,
-                                       prm, &amax);
+                                       prm, &amax, &cut);
     }
     if constexpr (sizeof(R) == 4) {
         if (rc.enabled) {
             // rows near a lookup / bin boundary: bound on |coh_f32 - coh_f64| of
             // this refine, 2 (8 + max |k0 bp|) 2^-24 (per term: psdph product,
-            // rotation, angle rounding <= |k0 bp| ulp; sums in f64)
+            // rotation, angle rounding <= |k0 bp| ulp; sums in f64).  Rows whose
+            // least-squares step has a residual phase within the f32 angle
+            // error of the +-pi cut of arg() (the same bound plus the fitted
+            // atan2's 3.5e-7 rad) could take the other branch in f64.
             __shared__ int n_flag;
             __shared__ int flag_tid[256];
             __shared__ double new_coh[256];
             if (threadIdx.x == 0) n_flag = 0;
             __syncthreads();
             const double bound = 2.0 * (8.0 + (double)amax) * 5.9604644775390625e-08;
-            const bool flag = row && tix[k] >= 0 && decision_distance(coh, ro.mids, ro.hp) < bound;
+            const bool flag = row && tix[k] >= 0 &&
+                              ((double)cut < bound + 4.0e-7 ||
+                               (rc.psquare && decision_distance(coh, ro.mids, ro.hp) < bound));
             if (flag) flag_tid[atomicAdd(&n_flag, 1)] = threadIdx.x;
             __syncthreads();
             const int nf = n_flag;

@@ -830,11 +830,14 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         ro.part = s->rms_part.as<double>();
         ro.hist = s->hist.as<uint32_t>();
     }
-    // FP32 form, P-square weighting: rows near a lookup / bin boundary are
-    // refined again in FP64 inside the refine kernel
+    // FP32 form: rows whose discrete choices (the arg() branch of the
+    // least-squares step; with P-square weighting the lookup and the bin) lie
+    // within the refine's error bound are refined again in FP64 inside the
+    // refine kernel
     RecheckIn rc{};
     rc.bp64 = sizeof(R) == 4 ? s->bp64.as<double>() : nullptr;
-    rc.enabled = sizeof(R) == 4 && s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
+    rc.enabled = sizeof(R) == 4 ? 1 : 0;
+    rc.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
     SlotTimer tm(s, SLOT_REFINE);
     hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
                        s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),

@@ -370,10 +370,12 @@ def test_fit_real_row_fp64_and_fp32(S, ora):
 
 @pytest.fixture(scope="module")
 def loops(S, ora, prep_c1):
+    from conftest import perturb_one_ulp
     return {
         "g64": S.run_stage2_hip(prep_c1, form="fp64"),
         "g32": S.run_stage2_hip(prep_c1, form="fp32"),
         "r64": ora.run(prep_c1, mode=ora.REF64),
+        "r64p": ora.run(perturb_one_ulp(prep_c1), mode=ora.REF64),
         "r32": ora.run(prep_c1, mode=ora.REF32),
     }
 
@@ -393,38 +395,49 @@ def test_loop_fp64_vs_ref64(loops):
     assert np.all(_phase_close(g["ph_res"], r["ph_res"], 1e-4))
 
 
-def test_loop_fp32_vs_ref32(loops):
+def test_loop_fp32_vs_ref32(loops, prep_c1):
+    """The whole loop against the reference's precision map: every pm1 key
+    within the tolerance manifest on every row, except the rows whose trial
+    the reference's own FP32 scan ties (best - second below the FP32 form's
+    re-score bound in some iteration) or whose least-squares step had a
+    residual phase within 1e-5 rad of the arg() cut (the FP32 form takes those
+    choices in FP64); those are counted and must be few."""
     g, r = loops["g32"], loops["r32"]
     assert g["i_loop"] == r["i_loop"]
     assert np.array_equal(g["N_opt"], r["N_opt"])
-    report = {}
+    bound = 2.0 * (16 + 8 + 2 * prep_c1.n_ifg) * 2.0 ** -24
+    tie = (r["trial_margin_min"] <= bound) | (r["refine_margin_min"] <= 1e-5)
+    clear = ~tie
+    print("loop fp32 vs ref32: ref32 trial ties", int(tie.sum()), "of", tie.size)
+    assert tie.mean() < 0.01
+    assert np.array_equal(g["trial_ix"][clear], r["trial_ix"][clear])
+    bad = {}
     for key in ("K_ps", "coh_ps", "coh_ps_save"):
-        ok = _close(g[key], r[key], F64_ATOL, F64_RTOL)
-        report[key] = 1.0 - ok.mean()
-    report["C_ps"] = 1.0 - _phase_close(g["C_ps"], r["C_ps"], F64_ATOL).mean()
-    report["trial_ix"] = float(np.mean(g["trial_ix"] != r["trial_ix"]))
-    for key in ("ph_grid", "ph_weight", "ph_patch"):
+        bad[key] = int((~_close(g[key], r[key], F64_ATOL, F64_RTOL))[clear].sum())
+    bad["C_ps"] = int((~_phase_close(g["C_ps"], r["C_ps"], F64_ATOL))[clear].sum())
+    for key in ("ph_weight", "ph_patch"):
         ok = _close(g[key].real, r[key].real, F32_ATOL, F32_RTOL) & _close(g[key].imag, r[key].imag,
                                                                           F32_ATOL, F32_RTOL)
-        report[key] = 1.0 - ok.mean()
-    report["ph_res"] = 1.0 - _phase_close(g["ph_res"], r["ph_res"], F32_ATOL).mean()
-    print("loop fp32 vs ref32 out-of-tolerance fractions", report)
-    # rows outside tolerance trace back to FP32 near-tie / P-square lookup
-    # decisions of the reference itself; they must stay rare
-    for key, frac in report.items():
-        assert frac < 0.01, (key, frac)
-    np.testing.assert_allclose(g["gamma_change_save"], r["gamma_change_save"], rtol=1e-3)
+        bad[key] = int((~ok)[clear].sum())
+    bad["ph_res"] = int((~_phase_close(g["ph_res"], r["ph_res"], F32_ATOL))[clear].sum())
+    okg = _close(g["ph_grid"].real, r["ph_grid"].real, F32_ATOL, F32_RTOL) & \
+        _close(g["ph_grid"].imag, r["ph_grid"].imag, F32_ATOL, F32_RTOL)
+    bad["ph_grid"] = int((~okg).sum())
+    print("loop fp32 vs ref32 out-of-tolerance elements on untied rows", bad)
+    assert not any(bad.values()), bad
+    np.testing.assert_allclose(g["gamma_change_save"], r["gamma_change_save"], rtol=1e-6)
+    np.testing.assert_allclose(g["Nr"], r["Nr"], rtol=1e-12)
 
 
 def test_loop_fp32_vs_ref64(loops):
+    """The whole loop against the FP64 oracle, judged by the FP64 oracle on
+    one-ulp-perturbed inputs (input-quantisation ties); besides, every
+    coherence within 1e-6 here."""
+    from conftest import floor_check
     g, r = loops["g32"], loops["r64"]
-    assert g["i_loop"] == r["i_loop"]
-    agree = g["trial_ix"] == r["trial_ix"]
-    dcoh = np.abs(g["coh_ps"] - r["coh_ps"])
-    print("loop fp32 vs ref64: trial agreement", agree.mean(), "coh max err", dcoh.max(),
-          "coh err p99.9", np.quantile(dcoh, 0.999), "rescored", g["n_rescored"])
-    assert agree.mean() >= 0.999
-    assert np.quantile(dcoh, 0.999) < 1e-5
+    c = floor_check("C1 loop fp32 vs ref64", g, r, loops["r64p"])
+    assert c["trial_mismatch"] == 0 and c["coh_max"] < 1e-6
+    print("rescored", g["n_rescored"], "rechecked", g["n_rechecked"])
 
 
 def test_session_rerun_is_deterministic(S, prep_small):
