@@ -72,13 +72,15 @@ def kernel_work(prep, name):
         return 143_360.0 * W * I, 8.0 * C * I + part
     if name == "scan":
         return 8.0 * P * I * T, (8.0 + stepb) * P * I + 8.0 * C * I
-    if name == "refine":   # (+ the fused RMS/histogram step: coh_prev read and written)
-        return 2 * 20.0 * P * I, 2 * (8.0 + bpb) * P * I + 8.0 * C * I + 16.0 * P
+    if name == "refine":
+        return 2 * 20.0 * P * I, 2 * (8.0 + bpb) * P * I + 8.0 * C * I
     if name == "fill_grid":
         return 16.0 * P * I, (8.0 + bpb) * P * I + 8.0 * C * I
     if name == "clap_combine":
         return 0.0, part + 8.0 * C * I
     if name == "weights":
+        return 0.0, 24.0 * P
+    if name == "post":   # coh, coh_prev read, coh_prev written, re-decided rows
         return 0.0, 24.0 * P
     if name == "outputs":
         # once per loop: ph_nm + bperp read, ph_weight + ph_patch + ph_res
@@ -103,7 +105,8 @@ def aggregate_roofline(prep, kt, peak_tflops):
 
 KERNEL_SYMBOLS = {"clap_strip": "k_clap_strip", "scan": "k_scan", "refine": "k_refine",
                   "fill_grid": "k_fill_grid", "clap_combine": "k_clap_combine",
-                  "rescore": "k_rescore", "weights": "k_weights", "outputs": "k_final_outputs"}
+                  "rescore": "k_rescore", "weights": "k_weights", "outputs": "k_final_outputs",
+                  "post": "k_post"}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage2_fp32_500k_60ifg_pmc_traffic.md")

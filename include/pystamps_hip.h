@@ -275,11 +275,12 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
 /* Per-kernel device time of the session's runs and single steps so far (the reference prints
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
- * rescore, refine, rms_hist, weights, outputs.  ms[k] is the summed HIP-event
- * time of slot k over launches[k] launches.  The gamma-change RMS and
- * coherence histogram run inside the refine kernel, so slot rms_hist records
- * no launches (it is kept for the ABI); outputs is the once-per-loop kernel
- * that writes the write_pm1 arguments in the caller's layouts.  Timing events are recorded only when
+ * rescore, refine, post, weights, outputs.  ms[k] is the summed HIP-event
+ * time of slot k over launches[k] launches.  post is the kernel after the
+ * refine: the FP64 re-decision of the rows the refine marks (FP32 form) and
+ * the gamma-change RMS partials and coherence histogram (the loop control,
+ * one workgroup, is not timed); outputs is the once-per-loop kernel that
+ * writes the write_pm1 arguments in the caller's layouts.  Timing events are recorded only when
  * ps2_hip_session_set_timing(s, 1) was called before the runs; such runs
  * launch the iterations one by one (no graph), with the same results. */
 #define PS2_NUM_KERNEL_SLOTS 9

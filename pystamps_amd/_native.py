@@ -21,7 +21,7 @@ WEIGHT_PSQUARE = 0
 WEIGHT_SNR = 1
 COH_BINS = 100
 KERNEL_SLOTS = ("fill_grid", "clap_strip", "clap_combine", "scan", "rescore",
-                "refine", "rms_hist", "weights", "outputs")
+                "refine", "post", "weights", "outputs")
 
 _f64p = C.POINTER(C.c_double)
 _f32p = C.POINTER(C.c_float)

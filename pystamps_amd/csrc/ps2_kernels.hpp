@@ -1451,8 +1451,10 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
         const R pr = r.re * ocr - r.im * oci, pim = r.re * oci + r.im * ocr;
         const R a = RT<R>::atan2_(pim, pr);
         num += wb * (wgt * (double)a);
-        amax1 = fmax(amax1, fabs(ang));
-        cut = fmin(cut, (R)RT<R>::pi4() * (R)4 - fabs(a));
+        if constexpr (sizeof(R) == 4) {   // (the FP64 form has no re-decision)
+            amax1 = fmax(amax1, fabs(ang));
+            cut = fmin(cut, (R)RT<R>::pi4() * (R)4 - fabs(a));
+        }
     }
     if (den == 0.0) den = 1.0;
     k0 = (R)((double)k0 + num / den);
@@ -1473,7 +1475,7 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
         const R nb = bat(i2);
         const R ang = k0 * cb;
         const cplx<R> r = rotate_neg<R>(psdph_value<R>(cf, cn), ang);
-        amax = fmax(amax, fabs(ang));
+        if constexpr (sizeof(R) == 4) amax = fmax(amax, fabs(ang));
         cf = mf;
         cn = mn;
         cb = mb;
@@ -1487,7 +1489,7 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
     if (dn == 0.0) dn = 1.0;
     Kout[k] = (double)k0;
     Cout[k] = (double)RT<R>::atan2_((R)mim, (R)mre);
-    const double coh = sqrt(mre * mre + mim * mim) / dn;
+    const double coh = hypot(mre, mim) / dn;
     cohout[k] = coh;
     nopt[k] = 1.0;
     *max_angle = (float)fmax(amax, amax1);
@@ -1496,44 +1498,20 @@ __device__ inline double refine_row(int64_t k, const cplx<R> *__restrict__ phnm,
 }
 
 
-// The loop's gamma-change and histogram step (native_stage2.rs:2045-2057 and
-// hist_with_centers :1538-1622) over the coherences just computed, fused into
-// the refine kernel: per workgroup of 256 rows the sum of (coh - coh_prev)^2
-// in a fixed tree (one partial per workgroup, summed in a fixed order by
-// k_control), coh_prev <- coh, and the bin counts (integer, order-free).
+// Outputs of the loop's gamma-change and histogram step (native_stage2.rs:
+// 2045-2057 and hist_with_centers :1538-1622), taken by k_post over the
+// coherences just computed: per group of 256 rows the sum of
+// (coh - coh_prev)^2 in a fixed tree (one partial per group, summed in a
+// fixed order by k_control), coh_prev <- coh, and the bin counts (integer,
+// order-free).
 struct RmsHistOut {
     double *coh_prev;   // NULL: no RMS/histogram step (single-step use)
-    double *part;       // [gridDim.x]
+    double *part;       // [ceil(P / 256)]
     uint32_t *hist;
     const double *mids;
     HistParams hp;
 };
 
-// The workgroup's share of that step; `mine` marks the threads that own a
-// row (one per row).  Every thread of the workgroup must call it.
-__device__ inline void rms_hist_tail(const RmsHistOut &ro, bool mine, int64_t k, double coh) {
-    __shared__ double red[256];
-    __shared__ uint32_t lh[128];
-    if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
-    __syncthreads();
-    double d2 = 0.0;
-    if (mine) {
-        const double d = coh - ro.coh_prev[k];
-        d2 = d * d;
-        ro.coh_prev[k] = coh;
-        const int bin = hist_bin(coh, ro.mids, ro.hp);
-        if (bin >= 0) atomicAdd(&lh[bin], 1u);
-    }
-    red[threadIdx.x] = d2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) ro.part[blockIdx.x] = red[0];
-    for (int q = threadIdx.x; q < ro.hp.nbins; q += 256)
-        if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
-}
 
 // FP64 re-decision of the P-square inputs (FP32 form).  The loop turns each
 // coherence into two discrete choices: the P-square lookup index
@@ -1565,7 +1543,10 @@ __device__ inline double decision_distance(double coh, const double *__restrict_
     return dist;
 }
 
-// FP64 refine of row k by one wavefront (every lane returns the row's coh)
+// FP64 refine of row k by one wavefront (every lane returns the row's coh).
+// Lane l holds interferograms l, l + 64, ... (up to RW of them, loaded once);
+// beyond 64 RW interferograms the rest are read again in each pass.
+template <int RW>
 __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__restrict__ phnm,
                                              const double *__restrict__ bp64,
                                              const cplx<float> *__restrict__ filt,
@@ -1579,7 +1560,7 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
     const cplx<float> *fl = filt + cell_of[k];
     auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
     // psdph in f64: conj(unit filtered phase) x ph_nm
-    auto val = [&](int i, double &re, double &im) {
+    auto load = [&](int i, double &re, double &im, double &b) {
         const cplx<float> f = fl[(int64_t)i * C], n = nm[(int64_t)i * P];
         double fr = (double)f.re, fi = (double)f.im;
         const double a = hypot(fr, fi);
@@ -1590,10 +1571,32 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
         const double nr = (double)n.re, ni = (double)n.im;
         re = fr * nr + fi * ni;
         im = fr * ni - fi * nr;
+        b = bat(i);
+    };
+    double vr[RW], vi[RW], vb[RW];
+#pragma unroll
+    for (int u = 0; u < RW; ++u) {
+        vr[u] = vi[u] = vb[u] = 0.0;
+        if (lane + 64 * u < I) load(lane + 64 * u, vr[u], vi[u], vb[u]);
+    }
+    // element i of this lane: registers for u < RW, global memory beyond
+    auto elem = [&](int u, int i, double &re, double &im, double &b) {
+        if (u < RW) {
+#pragma unroll
+            for (int w = 0; w < RW; ++w)
+                if (w == u) {
+                    re = vr[w];
+                    im = vi[w];
+                    b = vb[w];
+                }
+        } else {
+            load(i, re, im, b);
+        }
     };
     double mn = INFINITY, mx = -INFINITY;
-    for (int i = lane; i < I; i += 64) {
-        const double b = bat(i);
+    for (int i = lane, u = 0; i < I; i += 64, ++u) {
+        double re, im, b;
+        elem(u, i, re, im, b);
         mn = fmin(mn, b);
         mx = fmax(mx, b);
     }
@@ -1606,20 +1609,19 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
     if (range == 0.0) range = 1.0;
     double k0 = 0.78539816339744830962 / range * (double)(tix[k] - prm.tn);
     double ore = 0.0, oim = 0.0;
-    for (int i = lane; i < I; i += 64) {
-        double re, im, sn, cs;
-        val(i, re, im);
-        sincos(k0 * bat(i), &sn, &cs);
+    for (int i = lane, u = 0; i < I; i += 64, ++u) {
+        double re, im, b, sn, cs;
+        elem(u, i, re, im, b);
+        sincos(k0 * b, &sn, &cs);
         ore += re * cs + im * sn;
         oim += im * cs - re * sn;
     }
     ore = wave_sum_f64(ore);
     oim = wave_sum_f64(oim);
     double num = 0.0, den = 0.0;
-    for (int i = lane; i < I; i += 64) {
-        double re, im, sn, cs;
-        val(i, re, im);
-        const double b = bat(i);
+    for (int i = lane, u = 0; i < I; i += 64, ++u) {
+        double re, im, b, sn, cs;
+        elem(u, i, re, im, b);
         sincos(k0 * b, &sn, &cs);
         const double rr = re * cs + im * sn, ri = im * cs - re * sn;
         const double w = hypot(re, im), wb = w * b;
@@ -1632,10 +1634,10 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
     if (den == 0.0) den = 1.0;
     k0 += num / den;
     double mre = 0.0, mim = 0.0, dn = 0.0;
-    for (int i = lane; i < I; i += 64) {
-        double re, im, sn, cs;
-        val(i, re, im);
-        sincos(k0 * bat(i), &sn, &cs);
+    for (int i = lane, u = 0; i < I; i += 64, ++u) {
+        double re, im, b, sn, cs;
+        elem(u, i, re, im, b);
+        sincos(k0 * b, &sn, &cs);
         const double rr = re * cs + im * sn, ri = im * cs - re * sn;
         mre += rr;
         mim += ri;
@@ -1664,55 +1666,35 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
                                                 double *__restrict__ Kout,
                                                 double *__restrict__ Cout,
                                                 double *__restrict__ cohout,
-                                                double *__restrict__ nopt, FitParams prm, RmsHistOut ro,
-                                                RecheckIn rc, uint32_t *__restrict__ recheck_count,
+                                                double *__restrict__ nopt, FitParams prm,
+                                                const double *__restrict__ mids, HistParams hp,
+                                                RecheckIn rc, uint8_t *__restrict__ flags,
                                                 const int32_t *__restrict__ gate) {
     if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool row = k < prm.P;
-    double coh = 0.0;
+    if (k >= prm.P) return;
+    double coh;
     float amax = 0.0f, cut = 4.0f;
-    if (row) {
-        if (prm.rowinv)
-            coh = refine_row<R, true>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt,
-                                      prm, &amax, &cut);
-        else
-            coh = refine_row<R, false>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt,
-                                       prm, &amax, &cut);
+    if (prm.rowinv)
+        coh = refine_row<R, true>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm,
+                                  &amax, &cut);
+    else
+        coh = refine_row<R, false>(k, phnm, bp, filt, cell_of, tix, rng, offset, Kout, Cout, cohout, nopt, prm,
+                                   &amax, &cut);
+    if (rc.enabled) {
+        // rows near a lookup / bin boundary: bound on |coh_f32 - coh_f64| of
+        // this refine, 2 (8 + max |k0 bp|) 2^-24 (per term: psdph product,
+        // rotation, angle rounding <= |k0 bp| ulp; sums in f64).  Rows whose
+        // least-squares step has a residual phase within the f32 angle error
+        // of the +-pi cut of arg() (the same bound plus the fitted atan2's
+        // 3.5e-7 rad) could take the other branch in f64.  k_post refines the
+        // flagged rows again in FP64.
+        const double bound = 2.0 * (8.0 + (double)amax) * 5.9604644775390625e-08;
+        flags[k] = tix[k] >= 0 && ((double)cut < bound + 4.0e-7 ||
+                                   (rc.psquare && decision_distance(coh, mids, hp) < bound))
+                       ? 1
+                       : 0;
     }
-    if constexpr (sizeof(R) == 4) {
-        if (rc.enabled) {
-            // rows near a lookup / bin boundary: bound on |coh_f32 - coh_f64| of
-            // this refine, 2 (8 + max |k0 bp|) 2^-24 (per term: psdph product,
-            // rotation, angle rounding <= |k0 bp| ulp; sums in f64).  Rows whose
-            // least-squares step has a residual phase within the f32 angle
-            // error of the +-pi cut of arg() (the same bound plus the fitted
-            // atan2's 3.5e-7 rad) could take the other branch in f64.
-            __shared__ int n_flag;
-            __shared__ int flag_tid[256];
-            __shared__ double new_coh[256];
-            if (threadIdx.x == 0) n_flag = 0;
-            __syncthreads();
-            const double bound = 2.0 * (8.0 + (double)amax) * 5.9604644775390625e-08;
-            const bool flag = row && tix[k] >= 0 &&
-                              ((double)cut < bound + 4.0e-7 ||
-                               (rc.psquare && decision_distance(coh, ro.mids, ro.hp) < bound));
-            if (flag) flag_tid[atomicAdd(&n_flag, 1)] = threadIdx.x;
-            __syncthreads();
-            const int nf = n_flag;
-            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            for (int q = wave; q < nf; q += 4) {
-                const int t = flag_tid[q];
-                const double c = refine_row_f64_wave((int64_t)blockIdx.x * blockDim.x + t, phnm, rc.bp64, filt,
-                                                     cell_of, tix, Kout, Cout, cohout, prm, lane);
-                if (lane == 0) new_coh[t] = c;
-            }
-            __syncthreads();
-            if (flag) coh = new_coh[threadIdx.x];
-            if (threadIdx.x == 0 && nf > 0 && recheck_count) atomicAdd(recheck_count, (uint32_t)nf);
-        }
-    }
-    if (ro.coh_prev) rms_hist_tail(ro, row, k, coh);
 }
 
 // The loop's outputs in the caller's layouts (the write_pm1 arguments,
@@ -1986,14 +1968,11 @@ __constant__ double kCtlInterpTaps[79] = {
     -7.92925708931989381e-05,
 };
 
-// one workgroup of CTL_THREADS threads
-constexpr int CTL_THREADS = 1024;
-__global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ ctl,
-                                                        const double *__restrict__ rms_part,
-                                                        uint32_t *__restrict__ hist,
-                                                        uint32_t *__restrict__ rescore_count,
-                                                        double *__restrict__ prand_hi,
-                                                        CtlParams cp) {
+// run by one workgroup of 256 threads (k_control)
+__device__ inline void control_step(LoopCtl *__restrict__ ctl, const double *__restrict__ rms_part,
+                                    uint32_t *__restrict__ hist, uint32_t *__restrict__ rescore_count,
+                                    double *__restrict__ prand_hi, const CtlParams &cp) {
+    constexpr int CTL_THREADS = 256;   // the calling workgroup
     __shared__ double red[CTL_THREADS];
     __shared__ double na[128], nr[128], prand[128], sm[128];
     __shared__ double up[CTL_UP];
@@ -2100,6 +2079,98 @@ __global__ __launch_bounds__(CTL_THREADS) void k_control(LoopCtl *__restrict__ c
         prand_hi[q] = sacc * 10.0;
     }
     if (t == 0) ctl->i_loop = i_loop0 + 1;
+}
+
+// Post-refine step of one gamma iteration, one launch of ceil(P / 256)
+// workgroups of 256 rows:
+//  1. (FP32 form) the rows k_refine flagged are refined again in FP64, one
+//     wavefront per row (refine_row_f64_wave);
+//  2. per workgroup: the gamma-change sum of (coh - coh_prev)^2 in a fixed
+//     tree (one partial per workgroup), coh_prev <- coh, and the coherence
+//     histogram (integer, order-free);
+// and k_control (one workgroup) then takes the RMS over the partials in a
+// fixed order, the stop test, the Nr scaling and the P-square table.  (A
+// last-workgroup-runs-the-control variant needs a device-scope release fence
+// in every workgroup, which writes back the XCD's L2 each time: 134-261 us
+// at 500k x 60 instead of a second launch.)  loop = 0 (the single fit step):
+// step 1 only.  counters: [0] re-score queue, [1] rows refined again;
+// control_step clears them for the next iteration.
+template <typename R>
+__global__ __launch_bounds__(256) void k_post(const cplx<R> *__restrict__ phnm,
+                                              const cplx<R> *__restrict__ filt,
+                                              const int32_t *__restrict__ cell_of,
+                                              const int32_t *__restrict__ tix,
+                                              double *__restrict__ Kout, double *__restrict__ Cout,
+                                              double *__restrict__ cohout, FitParams prm, RecheckIn rc,
+                                              const uint8_t *__restrict__ flags, RmsHistOut ro, int loop,
+                                              uint32_t *__restrict__ counters,
+                                              const int32_t *__restrict__ gate) {
+    if (loop_stopped(gate)) return;
+    // groups of 256 rows, walked grid-stride: one RMS partial per group (the
+    // same fixed tree whatever the grid), the block's histogram counts in LDS
+    // and flushed once (one global atomic per non-empty bin and workgroup)
+    __shared__ uint32_t lh[128];
+    __shared__ double red[256];
+    __shared__ int n_flag;
+    __shared__ int flag_tid[256];
+    __shared__ double new_coh[256];
+    if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
+    uint32_t n_rechecked = 0;
+    const int64_t n_groups = (prm.P + 255) / 256;
+    for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+        const int64_t k = g * 256 + threadIdx.x;
+        const bool row = k < prm.P;
+        double coh = row ? cohout[k] : 0.0;
+        if constexpr (sizeof(R) == 4) {
+            if (rc.enabled) {
+                if (threadIdx.x == 0) n_flag = 0;
+                __syncthreads();
+                const bool flag = row && flags[k] != 0;
+                if (flag) flag_tid[atomicAdd(&n_flag, 1)] = threadIdx.x;
+                __syncthreads();
+                const int nf = n_flag;
+                const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                for (int q = wave; q < nf; q += 4) {
+                    const int t = flag_tid[q];
+                    const double c = refine_row_f64_wave<4>(g * 256 + t, reinterpret_cast<const cplx<float> *>(phnm),
+                                                         rc.bp64, reinterpret_cast<const cplx<float> *>(filt),
+                                                         cell_of, tix, Kout, Cout, cohout, prm, lane);
+                    if (lane == 0) new_coh[t] = c;
+                }
+                __syncthreads();
+                if (flag) coh = new_coh[threadIdx.x];
+                n_rechecked += (uint32_t)nf;
+            }
+        }
+        if (loop) {
+            double d2 = 0.0;
+            if (row) {
+                const double d = coh - ro.coh_prev[k];
+                d2 = d * d;
+                ro.coh_prev[k] = coh;
+                const int bin = hist_bin(coh, ro.mids, ro.hp);
+                if (bin >= 0) atomicAdd(&lh[bin], 1u);
+            }
+            red[threadIdx.x] = d2;
+            __syncthreads();
+            for (int o = 128; o > 0; o >>= 1) {
+                if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) ro.part[g] = red[0];
+        }
+    }
+    if (threadIdx.x == 0 && n_rechecked > 0) atomicAdd(&counters[1], n_rechecked);
+    if (!loop) return;
+    __syncthreads();
+    for (int q = threadIdx.x; q < ro.hp.nbins; q += 256)
+        if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
+}
+
+__global__ __launch_bounds__(256) void k_control(LoopCtl *__restrict__ ctl, const double *__restrict__ rms_part,
+                                                 uint32_t *__restrict__ hist, uint32_t *__restrict__ counters,
+                                                 double *__restrict__ prand_hi, CtlParams cp) {
+    control_step(ctl, rms_part, hist, counters, prand_hi, cp);
 }
 
 }  // namespace ps2

@@ -292,8 +292,10 @@ struct ps2_session {
     // at the end of every loop (and by the fit step)
     DBuf o_weight, o_patch, o_res, o_vec, o_tix, o_grid;
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
+    DBuf flags;   // FP32 form: rows k_refine marks for the FP64 re-decision
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
-    int n_rms_blocks = 0;
+    int n_rms_blocks = 0;   // 256-row groups of k_post, one RMS partial each
+    int n_cu = 256;         // compute units of the device
     // device-resident loop: control block, gate of the loop kernels (set
     // while a loop is being launched), one iteration captured as a graph
     DBuf ctl;
@@ -518,10 +520,12 @@ int alloc_state(ps2_session *s) {
     int rc;
     if ((rc = s->grid.alloc(sizeof(cplx<R>) * C * I))) return rc;
     if ((rc = s->filt.alloc(sizeof(cplx<R>) * C * I))) return rc;
+    PS2_HIP(hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+    s->n_cu = std::max(1, s->n_cu);
     if (s->windows) {
         // strip slots: CUs x CLAP wavefronts resident per CU (register bound)
-        int n_cu = 0, per_cu = 0;
-        PS2_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+        const int n_cu = s->n_cu;
+        int per_cu = 0;
         PS2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(&k_clap_strip<R, CLAP_TEAMS, true>), 32 * CLAP_TEAMS, 0));
         s->segs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS), (int)I,
@@ -544,13 +548,15 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_list.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_count.alloc(sizeof(uint32_t) * 4))) return rc;
+    if (sizeof(R) == 4)
+        if ((rc = s->flags.alloc(std::max<int64_t>(P, 1)))) return rc;
     if ((rc = s->hist.alloc(sizeof(uint32_t) * 128))) return rc;
     for (DBuf *b : {&s->K, &s->Kprev, &s->W, &s->Wprev, &s->Cc, &s->coh, &s->cohprev, &s->nopt})
         if ((rc = b->alloc(sizeof(double) * P))) return rc;
     if ((rc = s->prand_hi.alloc(sizeof(double) * 2048))) return rc;
     if ((rc = s->ctl.alloc(sizeof(LoopCtl)))) return rc;
     if ((rc = s->h_ctl.alloc(sizeof(LoopCtl)))) return rc;
-    s->n_rms_blocks = (int)blocks_for(P, 256);   // one partial per refine workgroup
+    s->n_rms_blocks = (int)blocks_for(P, 256);   // one partial per 256-row group of k_post
     if ((rc = s->rms_part.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
     {   // control-kernel constants: gausswin(7, 2.5) taps and their sum (native_stage2.rs:1685-1699)
         CtlParams &cp = s->ctl_params;
@@ -586,7 +592,7 @@ int alloc_state(ps2_session *s) {
 // --- per-kernel timing ---------------------------------------------------------
 
 enum Slot { SLOT_FILL = 0, SLOT_CLAP_STRIP, SLOT_CLAP_COMBINE, SLOT_SCAN, SLOT_RESCORE, SLOT_REFINE,
-            SLOT_RMS, SLOT_WEIGHTS, SLOT_OUTPUTS };
+            SLOT_POST, SLOT_WEIGHTS, SLOT_OUTPUTS };
 
 struct SlotTimer {
     ps2_session *s;
@@ -759,7 +765,7 @@ template <typename R>
 int launch_fit(ps2_session *s, bool in_loop = false) {
     const FitParams fp = fit_params(s);
     const unsigned nb = blocks_for(s->P, 256);
-    if (sizeof(R) == 4 && !in_loop) {   // (inside the loop k_control clears it)
+    if (sizeof(R) == 4 && !in_loop) {   // (inside the loop the control step clears it)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4,
                            s->gate);
         PS2_LAUNCHED();
@@ -816,8 +822,10 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
                            s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp, s->gate);
         PS2_LAUNCHED();
     }
-    // inside the loop the refine kernel also runs the gamma-change RMS partials,
-    // coh_prev <- coh and the coherence histogram (k_control reads them)
+    // the refine kernel marks the rows whose discrete choices lie within its
+    // error bound (FP32 form); k_post refines them again in FP64 and, inside
+    // the loop, computes the gamma-change RMS partials, coh_prev <- coh and
+    // the coherence histogram (k_control then takes the loop decisions)
     RmsHistOut ro{};
     ro.mids = s->mids.as<double>();
     ro.hp.P = s->P;
@@ -830,21 +838,27 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         ro.part = s->rms_part.as<double>();
         ro.hist = s->hist.as<uint32_t>();
     }
-    // FP32 form: rows whose discrete choices (the arg() branch of the
-    // least-squares step; with P-square weighting the lookup and the bin) lie
-    // within the refine's error bound are refined again in FP64 inside the
-    // refine kernel
     RecheckIn rc{};
     rc.bp64 = sizeof(R) == 4 ? s->bp64.as<double>() : nullptr;
     rc.enabled = sizeof(R) == 4 ? 1 : 0;
     rc.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
-    SlotTimer tm(s, SLOT_REFINE);
-    hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                       s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                       s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
-                       s->Cc.as<double>(),
-                       s->coh.as<double>(), s->nopt.as<double>(), fp, ro, rc,
-                       sizeof(R) == 4 ? s->rescore_count.as<uint32_t>() + 1 : nullptr, s->gate);
+    {
+        SlotTimer tm(s, SLOT_REFINE);
+        hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
+                           s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
+                           s->Cc.as<double>(), s->coh.as<double>(), s->nopt.as<double>(), fp, ro.mids, ro.hp, rc,
+                           s->flags.as<uint8_t>(), s->gate);
+        PS2_LAUNCHED();
+    }
+    if (!in_loop && !rc.enabled) return PS2_OK;
+    SlotTimer tm(s, SLOT_POST);
+    // grid-stride over the 256-row groups: 2 workgroups per CU at most
+    hipLaunchKernelGGL(k_post<R>, dim3(std::min<unsigned>(nb, 2 * (unsigned)s->n_cu)), dim3(256), 0, s->stream,
+                       s->phnm.as<cplx<R>>(),
+                       s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->K.as<double>(),
+                       s->Cc.as<double>(), s->coh.as<double>(), fp, rc, s->flags.as<uint8_t>(), ro,
+                       in_loop ? 1 : 0, s->rescore_count.as<uint32_t>(), s->gate);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -867,9 +881,8 @@ int launch_iteration(ps2_session *s) {
     if ((rc = launch_fit<R>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
     // gamma-change RMS, stop test, Nr scaling and the P-square table
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(CTL_THREADS), 0, s->stream, s->ctl.as<LoopCtl>(),
-                       s->rms_part.as<double>(), s->hist.as<uint32_t>(),
-                       sizeof(R) == 4 ? s->rescore_count.as<uint32_t>() : nullptr,
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s->stream, s->ctl.as<LoopCtl>(),
+                       s->rms_part.as<double>(), s->hist.as<uint32_t>(), s->rescore_count.as<uint32_t>(),
                        s->prand_hi.as<double>(), s->ctl_params);
     PS2_LAUNCHED();
     SlotTimer tm(s, SLOT_WEIGHTS);
