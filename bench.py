@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import tempfile
 import math
 import os
 import sys
@@ -208,6 +209,8 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--pmc-dir", default="", help="rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE outputs "
                     "of this workload, for the roofline traffic field")
+    ap.add_argument("--autotune-cache", default=os.path.join(tempfile.gettempdir(), "pystamps_hip_tiles.txt"),
+                    help="FP32 tile cache (measured on a miss); 'none' keeps the built-in tile model")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain-copy kernel that calibrates FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -241,10 +244,14 @@ def main():
     log(f"[bench] prep (incl. GPU Nr, {args.nr_rows} rows) {prep_s:.2f}s grid {prep.n_i}x{prep.n_j} "
         f"trials {len(prep.trial_values)}")
 
+    cache = None if args.autotune_cache == "none" else args.autotune_cache
+
     def time_form(form, steps, warmup):
         t0 = time.perf_counter()
-        s = Stage2Session(prep, form=form)
+        s = Stage2Session(prep, form=form, autotune_cache=cache if form == "fp32" else None)
         setup = time.perf_counter() - t0
+        tiles = s.tiles()
+        log(f"[bench] {form} tiles {tiles}")
         for _ in range(warmup):
             s.run()
         # timed loops: the default launch path (one iteration captured as a
@@ -274,7 +281,7 @@ def main():
             if n:
                 log(f"[bench]   {name:13s} {ms / n * 1e3:9.1f} us/launch x {n}")
         return {"wall": wall, "iters": iters, "dev": dev_s, "kt": kt, "out": out, "setup": setup,
-                "download": download}
+                "download": download, "tiles": tiles}
 
     r32 = time_form("fp32", args.steps, args.warmup)
     e2e = prep_s + r32["setup"] + r32["wall"] / args.steps + r32["download"]
@@ -334,7 +341,8 @@ def main():
         "config": {"workload": cfg.name, "n_ps": P, "n_ifg": I, "grid": [prep.n_i, prep.n_j],
                    "clap_windows_per_ifg": clap_windows(prep.n_i) * clap_windows(prep.n_j),
                    "n_trial": len(prep.trial_values), "iterations_per_step": r32["iters"] // args.steps,
-                   "form": "fp32 bulk + fp64 near-tie re-score"},
+                   "form": "fp32 bulk + fp64 near-tie re-score and P-square re-decision",
+                   "tiles": r32["tiles"]},
         "roofline": roof,
         "rescored_rows_per_loop": r32["out"]["n_rescored"],
         "rechecked_rows_per_loop": r32["out"]["n_rechecked"],

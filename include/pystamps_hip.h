@@ -108,7 +108,26 @@ typedef struct {
                                  are re-scored in FP64; <= 0 selects the derived bound */
     int32_t verbose;          /* 1: per-iteration phase timers on stderr */
     int32_t reserved;
+    int32_t autotune;         /* FP32 form, 1: tiles for the patch's shape (n_ifg, n_trial, patch
+                                 size, window grid) from autotune_cache, measured on the session's
+                                 data and appended on a miss; 0: the built-in tile model */
+    const char *autotune_cache; /* path of that text cache (created if absent; lines are appended
+                                   under an advisory lock); NULL: measure, keep nothing */
 } ps2_run_opts;
+
+/* tile sources (ps2_tiles.source) */
+#define PS2_TILES_MODEL 0
+#define PS2_TILES_CACHE 1
+#define PS2_TILES_MEASURED 2
+
+/* Tiles a session runs with. */
+typedef struct {
+    int32_t fill_cells;              /* cells per fill workgroup (FP32 form: 64 or 128) */
+    int32_t scan_step_table;         /* FP32 scan: 1 unit steps from the table, 0 computed */
+    int32_t clap_segments;           /* CLAP strip segments per window row (0: no windows) */
+    int32_t clap_windows_per_segment;
+    int32_t source;                  /* PS2_TILES_* */
+} ps2_tiles;
 
 /* ABI version (PS2_HIP_ABI_VERSION). */
 int ps2_hip_abi_version(void);
@@ -285,6 +304,9 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
  * launch the iterations one by one (no graph), with the same results. */
 #define PS2_NUM_KERNEL_SLOTS 9
 int ps2_hip_session_set_timing(ps2_session *s, int32_t enable);
+
+/* The tiles of a session (the model's, or the autotuner's: ps2_run_opts). */
+int ps2_hip_session_tiles(ps2_session *s, ps2_tiles *out);
 int ps2_hip_session_kernel_times(ps2_session *s, double *ms, int64_t *launches);
 
 /* Single steps on a session, for state injection.  Real-typed arrays are float

@@ -65,8 +65,19 @@ class DeviceOutputs(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [
         ("form", C.c_int32), ("device", C.c_int32), ("rescore_bound", C.c_double),
-        ("verbose", C.c_int32), ("reserved", C.c_int32),
+        ("verbose", C.c_int32), ("reserved", C.c_int32), ("autotune", C.c_int32),
+        ("autotune_cache", C.c_char_p),
     ]
+
+
+class Tiles(C.Structure):
+    _fields_ = [
+        ("fill_cells", C.c_int32), ("scan_step_table", C.c_int32), ("clap_segments", C.c_int32),
+        ("clap_windows_per_segment", C.c_int32), ("source", C.c_int32),
+    ]
+
+
+TILE_SOURCES = {0: "model", 1: "cache", 2: "measured"}
 
 
 _lib = None

@@ -191,12 +191,12 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
 }
 
 // Outputs in the caller's row order [P][I] from sorted ifg-major planes [I][P]:
-// a workgroup takes FINAL_TP sorted pixels x FINAL_TF interferograms, reads
-// the planes along the pixel axis, and writes each pixel's run of
+// a workgroup takes TP sorted pixels x TF interferograms (TP x TF = 2048 or
+// 4096), reads the planes along the pixel axis, and writes each pixel's run of
 // interferograms as one contiguous row segment (order[k] scatters whole rows,
-// not single elements).  Tile shape of k_final_outputs.
-constexpr int FINAL_TP = 32;
-constexpr int FINAL_TF = 64;
+// not single elements).  TF covers the whole row where I <= 256, so no row is
+// split between workgroups (a split leaves partial cache lines in the middle
+// of rows: 2.7 TB/s at 4M x 120 with 64-wide tiles).
 
 // ---------------------------------------------------------------------------
 // CLAP (native_stage2.rs:742-1184).
@@ -1702,8 +1702,8 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
 //  - ph_weight of the last fill (fill_phase_weight :629-657, from the K and
 //    weighting that fill used), ph_patch (the unit filtered cell value,
 //    extract_patch_phase :1186-1205) and ph_res = arg(residual) of valid rows
-//    (:714-716), [P][I] row order.  A pixel tile is FINAL_TP sorted pixels x
-//    FINAL_TF interferograms: the inputs are read once along the planes
+//    (:714-716), [P][I] row order.  A pixel tile is TP sorted pixels x TF
+//    interferograms: the inputs are read once along the planes
 //    (coalesced over pixels), each output goes out through one LDS tile as
 //    whole row segments (coalesced over interferograms);
 //  - K, C, coh, N_opt, coh_ps_save and the trial index in row order (the
@@ -1718,8 +1718,8 @@ struct FinalOut {
     double *vec;           // [5][P]: K, C, coh, N_opt, coh_ps_save
     int32_t *tix;          // [P]
     float2 *grid;          // [C][I]
-    int64_t n_pix_blocks;  // pixel tiles: ceil(P / FINAL_TP) x ceil(I / FINAL_TF)
-    int n_ifg_tiles;       // ceil(I / FINAL_TF)
+    int64_t n_pix_blocks;  // pixel tiles: ceil(P / TP) x ceil(I / TF)
+    int n_ifg_tiles;       // ceil(I / TF)
     int n_grid_ifg_tiles;  // ceil(I / 32)
 };
 
@@ -1729,7 +1729,7 @@ struct FinalIn {
     const int32_t *tix;
 };
 
-template <typename R>
+template <typename R, int TP, int TF>
 __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict__ phnm,
                                                        const R *__restrict__ bp, int rowinv,
                                                        const cplx<R> *__restrict__ filt,
@@ -1737,7 +1737,9 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
                                                        const int32_t *__restrict__ cell_of,
                                                        const int64_t *__restrict__ order, FinalIn in,
                                                        FinalOut o, int64_t P, int I, int64_t C) {
-    __shared__ float2 tile[FINAL_TP][FINAL_TF + 1];
+    static_assert(TP * TF % 256 == 0 && 256 % TP == 0 && 256 % TF == 0 || TF == 256, "tile shape");
+    static_assert(TP * (TF + 1) >= 32 * 33, "the grid transpose shares the tile");
+    __shared__ float2 tile[TP][TF + 1];
     const int64_t b = blockIdx.x;
     if (b >= o.n_pix_blocks) {   // ph_grid transpose tile
         float2(*t)[33] = reinterpret_cast<float2(*)[33]>(&tile[0][0]);
@@ -1761,9 +1763,9 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
         }
         return;
     }
-    const int64_t k0 = (b / o.n_ifg_tiles) * FINAL_TP;
-    const int f0 = (int)(b % o.n_ifg_tiles) * FINAL_TF;
-    const int np = (int)min<int64_t>(FINAL_TP, P - k0), nf = min(FINAL_TF, I - f0);
+    const int64_t k0 = (b / o.n_ifg_tiles) * TP;
+    const int f0 = (int)(b % o.n_ifg_tiles) * TF;
+    const int np = (int)min<int64_t>(TP, P - k0), nf = min(TF, I - f0);
     if (f0 == 0 && (int)threadIdx.x < np) {
         const int64_t k = k0 + threadIdx.x, r = order[k];
         if (o.vec) {
@@ -1778,8 +1780,9 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
     // read side: a thread keeps one pixel of the tile (kl) and takes the
     // interferograms fl0 + 8 q; indices are clamped into the tile so that every
     // load of the thread is issued at once (out-of-tile values are not stored)
-    constexpr int PER = FINAL_TP * FINAL_TF / 256;
-    const int kl = threadIdx.x % FINAL_TP, fl0 = threadIdx.x / FINAL_TP;
+    constexpr int PER = TP * TF / 256;
+    constexpr int FSTEP = 256 / TP;   // interferograms between a thread's reads
+    const int kl = threadIdx.x % TP, fl0 = threadIdx.x / TP;
     const int64_t k = k0 + min(kl, np - 1);
     const int32_t cell = cell_of[k];
     const R kres = (R)in.K[k];
@@ -1789,7 +1792,7 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
     R bv[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int f = f0 + min(fl0 + 8 * q, nf - 1);
+        const int f = f0 + min(fl0 + FSTEP * q, nf - 1);
         const int64_t g = (int64_t)f * P + k;
         nm[q] = phnm[g];
         bv[q] = rowinv ? bp[f] : bp[g];
@@ -1811,21 +1814,22 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
             rv[q] = (float)RT<R>::atan2_(r.im, r.re);
         }
     }
-    // write side: thread t stores column t % 64 of rows t / 64 + 4 q
-    const int fw = threadIdx.x % FINAL_TF, kw0 = threadIdx.x / FINAL_TF;
+    // write side: thread t stores column t % TF of rows t / TF + KSTEP q
+    constexpr int KSTEP = TF >= 256 ? 1 : 256 / TF;
+    const int fw = threadIdx.x % TF, kw0 = threadIdx.x / TF;
     int64_t orow[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) orow[q] = order[k0 + min(kw0 + 4 * q, np - 1)] * I + f0 + fw;
+    for (int q = 0; q < PER; ++q) orow[q] = order[k0 + min(kw0 + KSTEP * q, np - 1)] * I + f0 + fw;
     // one output at a time through the tile, written as row segments
     for (int which = 0; which < 3; ++which) {
         if ((which == 0 && !o.weight) || (which == 1 && !o.patch) || (which == 2 && !o.res)) continue;
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            tile[kl][fl0 + 8 * q] = which == 0 ? wv[q] : which == 1 ? pv[q] : make_float2(rv[q], 0.0f);
+            tile[kl][fl0 + FSTEP * q] = which == 0 ? wv[q] : which == 1 ? pv[q] : make_float2(rv[q], 0.0f);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const int kr = kw0 + 4 * q;
+            const int kr = kw0 + KSTEP * q;
             if (kr < np && fw < nf) {
                 const float2 v = tile[kr][fw];
                 if (which == 0)

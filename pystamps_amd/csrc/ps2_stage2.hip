@@ -7,8 +7,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/file.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <atomic>
+#include <cerrno>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -279,6 +284,12 @@ struct ps2_session {
     int nwi = 0, nwj = 0;
     bool windows = false;
     ClapSegs segs{1, 1, 4};   // set with the device state
+    // tiles (model rules, or the autotuner's choice): fill workgroup cells,
+    // FP32 scan unit-step source, CLAP segmentation above
+    int fill_cb = 128;
+    int scan_table = 1;
+    int tile_source = PS2_TILES_MODEL;
+    bool autotune = false;
     double alpha = 1.0, beta = 0.3, nr_max_nz_ix = 1.0, conv = 1e-4;
     int64_t lct = 31, max_iter = 25;
     int weighting = PS2_WEIGHT_PSQUARE;
@@ -296,6 +307,7 @@ struct ps2_session {
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;   // 256-row groups of k_post, one RMS partial each
     int n_cu = 256;         // compute units of the device
+    int clap_slots = 1;     // CLAP strip wavefronts resident on the device
     // device-resident loop: control block, gate of the loop kernels (set
     // while a loop is being launched), one iteration captured as a graph
     DBuf ctl;
@@ -342,8 +354,11 @@ struct ps2_session {
 // choice where the scan is bound by HBM bandwidth: 8-trial chunks (fewer than
 // 8 positive trials) over large patches with per-pixel bperp.  Invariant
 // bperp rows keep a one-row table.
-static inline bool scan_step_table(const ps2_session *s) {
+static inline bool scan_step_table_rule(const ps2_session *s) {
     return s->form == PS2_FORM_FP32 && (s->rowinv || s->tn >= 8 || s->P < 65536);
+}
+static inline bool scan_step_table(const ps2_session *s) {
+    return s->form == PS2_FORM_FP32 && (s->scan_table != 0 || s->rowinv || s->tn >= 8);
 }
 
 // Host -> device copy on the session's stream, complete on return (the
@@ -494,24 +509,31 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
 // segments; segments write their partial sums into nplanes = 1 + ceil(3 / wseg)
 // planes in turn (2 from 3 windows per segment on), so no two overlapping
 // segments share a plane.
-static ClapSegs clap_segments(int nwi, int nwj, int ngroups, int I, int slots) {
+// Segmentations ranked by the model's cost (cheapest first; ties keep the
+// longer segments).  The model's choice is the first; the autotuner times the
+// first few.
+static std::vector<ClapSegs> clap_segment_candidates(int nwi, int nwj, int ngroups, int I, int slots) {
     const int64_t rows = std::max<int64_t>(1, (int64_t)nwi * ngroups);
     const double seg_cost = 3e-5 * (double)I * (double)nwi;
     const int64_t xslots = std::max<int64_t>(1, slots / PS2_NUM_XCD);
     const int nw = std::max(1, nwj);
-    ClapSegs best{1, nw, 1 + (3 + nw - 1) / nw};
-    double best_cost = -1.0;
+    std::vector<std::pair<double, ClapSegs>> c;
     for (int nseg = 1; nseg <= nw; ++nseg) {
         const int wseg = (nw + nseg - 1) / nseg;
         if ((nw + wseg - 1) / wseg != nseg) continue;   // same segments as a smaller nseg
         const int64_t per_xcd = (rows * nseg + PS2_NUM_XCD - 1) / PS2_NUM_XCD;
         const double cost = (double)((per_xcd + xslots - 1) / xslots) * (wseg + 0.25) + seg_cost * nseg;
-        if (best_cost < 0.0 || cost < best_cost) {   // ties keep the longer segments
-            best_cost = cost;
-            best = {nseg, wseg, 1 + (3 + wseg - 1) / wseg};
-        }
+        c.push_back({cost, ClapSegs{nseg, wseg, 1 + (3 + wseg - 1) / wseg}});
     }
-    return best;
+    std::stable_sort(c.begin(), c.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    std::vector<ClapSegs> out;
+    for (const auto &x : c) out.push_back(x.second);
+    if (out.empty()) out.push_back(ClapSegs{1, nw, 1 + (3 + nw - 1) / nw});
+    return out;
+}
+
+static ClapSegs clap_segments(int nwi, int nwj, int ngroups, int I, int slots) {
+    return clap_segment_candidates(nwi, nwj, ngroups, I, slots).front();
 }
 
 template <typename R>
@@ -528,16 +550,20 @@ int alloc_state(ps2_session *s) {
         int per_cu = 0;
         PS2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(&k_clap_strip<R, CLAP_TEAMS, true>), 32 * CLAP_TEAMS, 0));
-        s->segs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS), (int)I,
-                                std::max(1, n_cu) * std::max(1, per_cu));
-        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * s->segs.nplanes * CLAP_WIN * s->nj * I)))
+        s->clap_slots = std::max(1, n_cu) * std::max(1, per_cu);
+        s->segs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS), (int)I, s->clap_slots);
+        // the autotuner tries segmentations with up to 4 planes
+        const int planes = s->autotune ? 4 : s->segs.nplanes;
+        if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * planes * CLAP_WIN * s->nj * I)))
             return rc;
     }
+    s->fill_cb = (double)P / (double)std::max<int64_t>(C, 1) < 12.0 ? 128 : 64;
+    s->scan_table = scan_step_table_rule(s) ? 1 : 0;
     if ((rc = s->rng.alloc(sizeof(R) * P))) return rc;
     // the FP32 unit-step table (one row with invariant bperp rows), unless
     // the scan computes the steps itself; the chunked scan (48 or more
     // positive trials) always reads it
-    const bool table = sizeof(R) == 4 && (scan_step_table(s) || s->tn >= 48);
+    const bool table = sizeof(R) == 4 && (scan_step_table(s) || s->tn >= 48 || s->autotune);
     if (table)
         if ((rc = s->wt.alloc(sizeof(cplx<R>) * (s->rowinv ? I : P * I)))) return rc;
     hipLaunchKernelGGL(k_trial_setup<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
@@ -638,6 +664,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *K
         // ranges take several chunks: LDS beyond 16 KB per workgroup costs more
         // in occupancy than the extra passes).
         const double density = (double)s->P / (double)std::max<int64_t>(s->C, 1);
+        // the cells per workgroup come from the session's tiles
         auto launch = [&](auto cb_tag) {
             constexpr int CB = decltype(cb_tag)::value;
             const double mean = density * CB;
@@ -656,14 +683,10 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *K
             else
                 go(k_fill_grid_staged<R, CB, 1024>);
         };
-#ifdef PS2_FILL_CB
-        launch(std::integral_constant<int, PS2_FILL_CB>{});
-#else
-        if (density < 12.0)
+        if (s->fill_cb == 128)
             launch(std::integral_constant<int, 128>{});
         else
             launch(std::integral_constant<int, 64>{});
-#endif
     } else {
         hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
                            s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
@@ -922,15 +945,187 @@ int launch_outputs(ps2_session *s, bool loop) {
     fo.vec = s->o_vec.as<double>();
     fo.tix = s->o_tix.as<int32_t>();
     fo.grid = loop ? s->o_grid.as<float2>() : nullptr;
-    fo.n_ifg_tiles = (int)blocks_for(I, FINAL_TF);
-    fo.n_pix_blocks = (int64_t)blocks_for(P, FINAL_TP) * fo.n_ifg_tiles;
-    fo.n_grid_ifg_tiles = (int)blocks_for(I, 32);
-    const int64_t n_grid_blocks = loop ? (int64_t)blocks_for(C, 32) * fo.n_grid_ifg_tiles : 0;
-    hipLaunchKernelGGL(k_final_outputs<R>, dim3((unsigned)(fo.n_pix_blocks + n_grid_blocks)), dim3(256), 0,
-                       s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->filt.as<cplx<R>>(),
-                       s->grid.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->d_order.as<int64_t>(), fi, fo, P,
-                       (int)I, C);
+    auto go = [&](auto tp_tag, auto tf_tag) {
+        constexpr int TP = decltype(tp_tag)::value, TF = decltype(tf_tag)::value;
+        fo.n_ifg_tiles = (int)blocks_for(I, TF);
+        fo.n_pix_blocks = (int64_t)blocks_for(P, TP) * fo.n_ifg_tiles;
+        fo.n_grid_ifg_tiles = (int)blocks_for(I, 32);
+        const int64_t n_grid_blocks = loop ? (int64_t)blocks_for(C, 32) * fo.n_grid_ifg_tiles : 0;
+        hipLaunchKernelGGL((k_final_outputs<R, TP, TF>), dim3((unsigned)(fo.n_pix_blocks + n_grid_blocks)),
+                           dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv,
+                           s->filt.as<cplx<R>>(), s->grid.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                           s->d_order.as<int64_t>(), fi, fo, P, (int)I, C);
+    };
+    // the tile spans the whole row where it can (at most 256 interferograms)
+    if (I <= 64)
+        go(std::integral_constant<int, 32>{}, std::integral_constant<int, 64>{});
+    else if (I <= 128)
+        go(std::integral_constant<int, 16>{}, std::integral_constant<int, 128>{});
+    else
+        go(std::integral_constant<int, 16>{}, std::integral_constant<int, 256>{});
     PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+// --- autotuner ------------------------------------------------------------------
+// Tiles measured on the session's own data, keyed on the patch's shape and
+// kept in a text cache (one line per shape), FP32 form.  Candidates: fill
+// workgroups of 64 or 128 cells (per-cell sums in the same order: bit-identical
+// grids), the scan's unit steps from the table or computed from bperp (the same
+// bits), and the CLAP strip segmentations the model ranks first (sums grouped
+// differently: equal to f32 rounding).  The tuned tiles apply to every later
+// run of the session; a cache hit applies them without measuring.
+
+static std::string tile_key(const ps2_session *s) {
+    char b[256];
+    const int pb = (int)std::lround(2.0 * std::log2((double)std::max<int64_t>(s->P, 1)));
+    std::snprintf(b, sizeof b, "ps2tiles1 cu=%d form=%d I=%lld T=%d rowinv=%d P2=%d win=%dx%d", s->n_cu, s->form,
+                  (long long)s->I, s->T, s->rowinv, pb, s->nwi, s->nwj);
+    return b;
+}
+
+struct TileLine {
+    int fill_cb = 0, scan_table = -1, nseg = 0;
+};
+
+// cache lookup / append under an advisory lock of the file
+static int tile_cache(const char *path, const std::string &key, TileLine *found, const TileLine *add) {
+    const int fd = ::open(path, O_RDWR | O_CREAT | O_APPEND, 0644);
+    if (fd < 0) {
+        set_error(std::string("autotune cache ") + path + ": " + std::strerror(errno));
+        return PS2_E_ARG;
+    }
+    ::flock(fd, add ? LOCK_EX : LOCK_SH);
+    int rc = PS2_OK;
+    if (found) {
+        FILE *f = ::fdopen(::dup(fd), "r");
+        if (f) {
+            ::rewind(f);
+            char line[512];
+            while (std::fgets(line, sizeof line, f)) {
+                const size_t n = key.size();
+                if (std::strncmp(line, key.c_str(), n) != 0 || line[n] != ' ') continue;
+                TileLine t;
+                if (std::sscanf(line + n, " fill_cells=%d scan_step_table=%d clap_segments=%d", &t.fill_cb,
+                                &t.scan_table, &t.nseg) == 3) {
+                    *found = t;
+                    break;
+                }
+            }
+            std::fclose(f);
+        }
+    }
+    if (add) {
+        char b[96];
+        const int len = std::snprintf(b, sizeof b, " fill_cells=%d scan_step_table=%d clap_segments=%d\n", add->fill_cb,
+                                      add->scan_table, add->nseg);
+        const std::string line = key + std::string(b, (size_t)len);
+        if (::write(fd, line.data(), line.size()) != (ssize_t)line.size()) {
+            set_error(std::string("autotune cache ") + path + ": write failed");
+            rc = PS2_E_ARG;
+        }
+    }
+    ::flock(fd, LOCK_UN);
+    ::close(fd);
+    return rc;
+}
+
+template <typename R>
+int apply_segments(ps2_session *s, int nseg) {
+    const int ngroups = (int)((s->I + CLAP_TEAMS - 1) / CLAP_TEAMS);
+    for (const ClapSegs &c : clap_segment_candidates(s->nwi, s->nwj, ngroups, (int)s->I, s->clap_slots))
+        if (c.nseg == nseg) {
+            s->segs = c;
+            return PS2_OK;
+        }
+    return arg_error("autotune cache: no CLAP segmentation with " + std::to_string(nseg) + " segments");
+}
+
+template <typename R>
+int autotune_tiles(ps2_session *s, const char *path) {
+    const std::string key = tile_key(s);
+    TileLine hit;
+    int rc;
+    if (path && (rc = tile_cache(path, key, &hit, nullptr))) return rc;
+    const bool scan_choice = !s->rowinv && s->tn < 8;   // the one scan that can compute its steps
+    if (hit.fill_cb) {
+        s->fill_cb = hit.fill_cb == 64 ? 64 : 128;
+        if (scan_choice) s->scan_table = hit.scan_table ? 1 : 0;
+        if (s->windows && (rc = apply_segments<R>(s, hit.nseg))) return rc;
+        s->tile_source = PS2_TILES_CACHE;
+    } else {
+        // measured on this session's inputs from the initial state (K = 0,
+        // weighting 1/D_A): average of 3 launches after one warm-up
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        PS2_HIP(hipEventCreate(&e0));
+        PS2_HIP(hipEventCreate(&e1));
+        auto timed = [&](auto &&launch, float *ms) -> int {
+            int r;
+            if ((r = launch())) return r;
+            PS2_HIP(hipEventRecord(e0, s->stream));
+            for (int q = 0; q < 3 && !r; ++q) r = launch();
+            if (r) return r;
+            PS2_HIP(hipEventRecord(e1, s->stream));
+            PS2_HIP(hipEventSynchronize(e1));
+            PS2_HIP(hipEventElapsedTime(ms, e0, e1));
+            return PS2_OK;
+        };
+        auto done = [&](int r) {
+            (void)hipStreamSynchronize(s->stream);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return r;
+        };
+        PS2_HIP(hipMemsetAsync(s->K.p, 0, sizeof(double) * s->P, s->stream));
+        PS2_HIP(hipMemcpyAsync(s->W.p, s->W0.p, sizeof(double) * s->P, hipMemcpyDeviceToDevice, s->stream));
+        float best = -1.0f;
+        int best_cb = s->fill_cb;
+        for (int cb : {64, 128}) {
+            s->fill_cb = cb;
+            float ms = 0.0f;
+            if ((rc = timed([&] { return launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>()); }, &ms)))
+                return done(rc);
+            if (best < 0.0f || ms < best) best = ms, best_cb = cb;
+        }
+        s->fill_cb = best_cb;
+        if (s->windows) {
+            const int ngroups = (int)((s->I + CLAP_TEAMS - 1) / CLAP_TEAMS);
+            std::vector<ClapSegs> cand = clap_segment_candidates(s->nwi, s->nwj, ngroups, (int)s->I, s->clap_slots);
+            if (cand.size() > 4) cand.resize(4);
+            best = -1.0f;
+            ClapSegs best_seg = cand.front();
+            for (const ClapSegs &c : cand) {
+                s->segs = c;
+                float ms = 0.0f;
+                if ((rc = timed([&] { return launch_clap<R>(s, true); }, &ms))) return done(rc);
+                if (best < 0.0f || ms < best) best = ms, best_seg = c;
+            }
+            s->segs = best_seg;
+            if ((rc = launch_clap<R>(s, true))) return done(rc);   // the filtered grid the scan reads
+        }
+        if (scan_choice) {
+            best = -1.0f;
+            int best_t = s->scan_table;
+            for (int tab : {1, 0}) {
+                s->scan_table = tab;
+                float ms = 0.0f;
+                if ((rc = timed([&] { return launch_fit<R>(s, false); }, &ms))) return done(rc);
+                if (best < 0.0f || ms < best) best = ms, best_t = tab;
+            }
+            s->scan_table = best_t;
+        }
+        if ((rc = done(PS2_OK))) return rc;
+        s->tile_source = PS2_TILES_MEASURED;
+        if (path) {
+            TileLine t;
+            t.fill_cb = s->fill_cb;
+            t.scan_table = s->scan_table;
+            t.nseg = s->windows ? s->segs.nseg : 0;
+            if ((rc = tile_cache(path, key, nullptr, &t))) return rc;
+        }
+    }
+    // the step table stays only where the scan reads it
+    if (!(scan_step_table(s) || s->tn >= 48)) (void)s->wt.alloc(0);
     return PS2_OK;
 }
 
@@ -1090,6 +1285,7 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
     if (s->form != PS2_FORM_FP32 && s->form != PS2_FORM_FP64) return arg_error("unknown form");
     s->device = opts ? opts->device : 0;
     s->verbose = opts ? opts->verbose : 0;
+    s->autotune = opts && opts->autotune && s->form == PS2_FORM_FP32;
     {
         std::lock_guard<std::mutex> lk(g_init_mu);
         int n = 0;
@@ -1140,6 +1336,7 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
     } else {
         if ((rc = upload_inputs<float>(s.get(), in))) return rc;
         if ((rc = alloc_state<float>(s.get()))) return rc;
+        if (s->autotune && (rc = autotune_tiles<float>(s.get(), opts->autotune_cache))) return rc;
     }
     *out = s.release();
     return PS2_OK;
@@ -1261,6 +1458,17 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out) {
     for (int b = 0; b < PS2_COH_BINS; ++b) out->nr[b] = s->nr[b];
     out->gamma_change_save = s->gamma_change_save;
     out->i_loop = s->i_loop;
+    return PS2_OK;
+}
+
+int ps2_hip_session_tiles(ps2_session *s, ps2_tiles *out) {
+    if (!s || !out) return arg_error("session or output is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->fill_cells = s->form == PS2_FORM_FP32 ? s->fill_cb : 1;
+    out->scan_step_table = scan_step_table(s) ? 1 : 0;
+    out->clap_segments = s->windows ? s->segs.nseg : 0;
+    out->clap_windows_per_segment = s->windows ? s->segs.wseg : 0;
+    out->source = s->tile_source;
     return PS2_OK;
 }
 

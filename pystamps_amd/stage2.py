@@ -71,7 +71,8 @@ class _InHolder:
         self.s = s
 
 
-def _opts(form: str, device: int, rescore_bound: float, verbose: bool) -> _native.RunOpts:
+def _opts(form: str, device: int, rescore_bound: float, verbose: bool,
+          autotune_cache: Optional[str] = None, autotune: bool = False) -> _native.RunOpts:
     if form not in FORMS:
         raise Stage2Error(f"unknown stage-2 form '{form}' (use fp32 or fp64)")
     o = _native.RunOpts()
@@ -79,6 +80,8 @@ def _opts(form: str, device: int, rescore_bound: float, verbose: bool) -> _nativ
     o.device = int(device)
     o.rescore_bound = float(rescore_bound)
     o.verbose = int(bool(verbose))
+    o.autotune = int(bool(autotune or autotune_cache))
+    o.autotune_cache = None if autotune_cache is None else str(autotune_cache).encode()
     return o
 
 
@@ -131,13 +134,14 @@ class _OutHolder:
 
 
 def run_stage2_hip(prep: Stage2Prepared, form: str = "fp32", device: int = 0,
-                   rescore_bound: float = 0.0, verbose: bool = False) -> dict:
+                   rescore_bound: float = 0.0, verbose: bool = False,
+                   autotune_cache: Optional[str] = None) -> dict:
     """Whole gamma loop on the GPU; returns the pm1.mat content."""
     L = _native.lib()
     inp = _InHolder(prep)
     out = _OutHolder(prep)
     _check(L.ps2_hip_run(C.byref(inp.s), C.byref(out.s),
-                         C.byref(_opts(form, device, rescore_bound, verbose))))
+                         C.byref(_opts(form, device, rescore_bound, verbose, autotune_cache))))
     return out.finish(prep)
 
 
@@ -164,7 +168,7 @@ def run_sessions(sessions) -> tuple[list[int], list[float]]:
 
 
 def run_stage2_hip_batch(preps, form: str = "fp32", device: int = 0,
-                         rescore_bound: float = 0.0) -> list[dict]:
+                         rescore_bound: float = 0.0, autotune_cache: Optional[str] = None) -> list[dict]:
     """Several patches in one batched call (ps2_hip_run_batch)."""
     L = _native.lib()
     ins = [_InHolder(p) for p in preps]
@@ -172,7 +176,8 @@ def run_stage2_hip_batch(preps, form: str = "fp32", device: int = 0,
     n = len(preps)
     arr_in = (_native.PatchIn * n)(*[x.s for x in ins])
     arr_out = (_native.PatchOut * n)(*[x.s for x in outs])
-    _check(L.ps2_hip_run_batch(n, arr_in, arr_out, C.byref(_opts(form, device, rescore_bound, False))))
+    _check(L.ps2_hip_run_batch(n, arr_in, arr_out,
+                               C.byref(_opts(form, device, rescore_bound, False, autotune_cache))))
     res = []
     for k, p in enumerate(preps):
         outs[k].s = arr_out[k]
@@ -184,12 +189,15 @@ class Stage2Session:
     """Device-resident patch: repeated loop runs and single steps."""
 
     def __init__(self, prep: Stage2Prepared, form: str = "fp32", device: int = 0,
-                 rescore_bound: float = 0.0, verbose: bool = False):
+                 rescore_bound: float = 0.0, verbose: bool = False,
+                 autotune_cache: Optional[str] = None, autotune: bool = False):
+        """autotune / autotune_cache: FP32 form tiles measured for the patch's
+        shape (kept in the cache file when one is named); else the model's."""
         self.L = _native.lib()
         self.prep = prep
         self.form = form
         self._in = _InHolder(prep)
-        self._opts = _opts(form, device, rescore_bound, verbose)
+        self._opts = _opts(form, device, rescore_bound, verbose, autotune_cache, autotune)
         self.h = C.c_void_p()
         _check(self.L.ps2_hip_session_create(C.byref(self._in.s), C.byref(self._opts), C.byref(self.h)))
 
@@ -206,6 +214,13 @@ class Stage2Session:
         secs = C.c_double()
         _check(self.L.ps2_hip_session_run(self.h, C.byref(i_loop), C.byref(secs)))
         return int(i_loop.value), float(secs.value)
+
+    def tiles(self) -> dict:
+        t = _native.Tiles()
+        _check(self.L.ps2_hip_session_tiles(self.h, C.byref(t)))
+        return {"fill_cells": t.fill_cells, "scan_step_table": bool(t.scan_step_table),
+                "clap_segments": t.clap_segments, "clap_windows_per_segment": t.clap_windows_per_segment,
+                "source": _native.TILE_SOURCES.get(t.source, str(t.source))}
 
     def set_timing(self, enable: bool = True):
         _check(self.L.ps2_hip_session_set_timing(self.h, int(bool(enable))))
