@@ -174,6 +174,9 @@ extern "C" {
     pub fn ps2_hip_topofit_rows(cpx: *const f64, bperp: *const f64, row_invariant: i32, n_rows: i64,
                                 n_ifg: i64, n_trial_wraps: f64, device: i32, k_out: *mut f64,
                                 c_out: *mut f64, coh_out: *mut f64, res_out: *mut f32) -> c_int;
+    pub fn ps2_hip_topofit_coh_row_invariant(cpx: *const f64, bperp_vec: *const f64, n_rows: i64,
+                                             n_ifg: i64, n_trial_wraps: f64, device: i32,
+                                             coh_out: *mut f64) -> c_int;
     pub fn ps2_hip_topofit_rows_noalloc(d_cpx: *const f64, d_bperp: *const f64, row_invariant: i32,
                                         n_rows: i64, n_ifg: i64, n_trial_wraps: f64, device: i32,
                                         stream: *mut c_void, d_k_out: *mut f64, d_c_out: *mut f64,

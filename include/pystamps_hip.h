@@ -193,6 +193,20 @@ int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int32_t row_inv
                          int64_t n_rows, int64_t n_ifg, double n_trial_wraps, int32_t device,
                          double *k_out, double *c_out, double *coh_out, float *res_out);
 
+/* Coherence only of n_rows rows against one bperp vector: the registry
+ * kernel stage2_topofit_coh_row_invariant, whose python and native providers
+ * both run _ps_topofit_batch_row_invariant_coh
+ * (pystamps/kernels/accelerated.py:368-389, pystamps/pipeline/ported.py:
+ * 2555-2616); it takes the place of the native module's
+ * ps_topofit_coh_row_invariant (src/lib.rs:1326-1378).  The trial grid spans the range of the whole
+ * bperp vector, zero elements included; a row with exactly one near-maximum
+ * local peak on that grid is refined once from it, every other row takes
+ * ps2_hip_topofit_rows' coherence.  FP64.  cpx [n_rows][n_ifg] complex128,
+ * bperp_vec [n_ifg], coh_out [n_rows]. */
+int ps2_hip_topofit_coh_row_invariant(const double *cpx, const double *bperp_vec, int64_t n_rows,
+                                      int64_t n_ifg, double n_trial_wraps, int32_t device,
+                                      double *coh_out);
+
 /* Allocation-free variant of ps2_hip_topofit_rows.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -132,6 +132,9 @@ def _bind(L):
     L.ora_py_select_candidate.restype = C.c_int64
     L.ora_py_topofit_rows.argtypes = [_f64p, _f64p, C.c_int, C.c_int64, C.c_int64, C.c_double,
                                       _f64p, _f64p, _f64p, _f64p]
+    L.ora_py_topofit_coh_row_invariant.argtypes = [_f64p, _f64p, C.c_int64, C.c_int64, C.c_double,
+                                                   _f64p, C.POINTER(C.c_int8)]
+    L.ora_py_topofit_coh_row_invariant.restype = None
     L.ora_py_grid_accumulate.argtypes = [_f32p, _i64p, C.c_int64, C.c_int64, C.c_int64, _f32p]
     L.ora_py_grid_accumulate.restype = C.c_int
     L.ora_topofit_phase_row_coh.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64]
@@ -451,6 +454,20 @@ def py_topofit_rows(cpxphase, bperp, n_trial_wraps):
                               float(n_trial_wraps), _p(K, _f64p), _p(Cc, _f64p), _p(coh, _f64p),
                               res.ctypes.data_as(_f64p))
     return K, Cc, coh, res
+
+
+def py_topofit_coh_row_invariant(cpxphase, bperp_vec, n_trial_wraps):
+    """stage2_topofit_coh_row_invariant in FP64: (coh, single) where single
+    marks the rows refined from the whole-vector trial grid."""
+    cpx = np.ascontiguousarray(cpxphase, dtype=np.complex128)
+    P, I = cpx.shape
+    bp = _c(np.asarray(bperp_vec, dtype=np.float64).reshape(-1), np.float64)
+    coh = np.zeros(P)
+    single = np.zeros(P, np.int8)
+    lib().ora_py_topofit_coh_row_invariant(cpx.ctypes.data_as(_f64p), _p(bp, _f64p), P, I,
+                                           float(n_trial_wraps), _p(coh, _f64p),
+                                           single.ctypes.data_as(C.POINTER(C.c_int8)))
+    return coh, single.astype(bool)
 
 
 def py_grid_accumulate(ph_weight, grid_lin, n_i, n_j):

@@ -8,6 +8,7 @@
  *   _ps_topofit_select_candidate       :2451-2477
  *   _ps_topofit_refine_candidate       :2480-2513
  *   _stage2_trial_values               :2350-2352
+ *   _ps_topofit_batch_row_invariant_coh :2555-2616
  *   _stage2_grid_accumulate_cpu        pystamps/kernels/accelerated.py:183-200
  * Sums run in index order (the FP64 GPU provider does the same); numpy's
  * pairwise 1-D sums round differently only at the 1e-16 level.
@@ -207,6 +208,95 @@ void ora_py_topofit_rows(const double *cpx, const double *bperp, int row_invaria
     for (int64_t p = 0; p < P; ++p)
         topofit_single(cpx + 2 * p * I, row_invariant ? bperp : bperp + p * I, I, tm, nt, &K[p],
                        &C[p], &coh[p], res + 2 * p * I, scratch);
+    free(scratch);
+    free(tm);
+}
+
+/* _ps_topofit_batch_row_invariant_coh (ported.py:2555-2616): coherence only,
+ * one bperp vector for every row.  The trial grid spans the range of the
+ * whole bperp vector (zero elements included, :2561-2564); a row with exactly
+ * one near-maximum local peak on that grid (:2580-2589) is refined once from
+ * it (:2591-2608, zero elements add nothing to any sum); every other row takes
+ * _ps_topofit_single's coherence (:2611-2614), whose grid spans the non-zero
+ * elements only.  single_out (may be NULL) receives 1 for rows of the first
+ * kind. */
+void ora_py_topofit_coh_row_invariant(const double *cpx, const double *bperp, int64_t P,
+                                      int64_t I, double n_trial_wraps, double *coh,
+                                      int8_t *single_out) {
+    const int64_t tn = (int64_t)ceil(8.0 * n_trial_wraps);
+    const int64_t nt = 2 * tn + 1;
+    double *tm = (double *)malloc(sizeof(double) * (size_t)nt);
+    for (int64_t t = 0; t < nt; ++t) tm[t] = (double)(t - tn);
+    double mn = bperp[0], mx = bperp[0];
+    for (int64_t i = 1; i < I; ++i) {
+        mn = bperp[i] < mn ? bperp[i] : mn;
+        mx = bperp[i] > mx ? bperp[i] : mx;
+    }
+    double range = mx - mn;
+    if (range == 0.0) range = 1.0;
+    const size_t scratch_n = (size_t)(11 * I + 4 * nt + 16);
+    double *scratch = (double *)malloc(sizeof(double) * scratch_n + sizeof(int64_t) * (size_t)(nt + I + 8));
+    double *coh_t = (double *)malloc(sizeof(double) * (size_t)nt);
+    double *w = (double *)malloc(sizeof(double) * (size_t)(2 * I + 2));
+    double *wb = w + I + 1, *res = (double *)malloc(sizeof(double) * (size_t)(2 * I + 2));
+    double *cr = (double *)malloc(sizeof(double) * (size_t)(2 * I + 2)), *ci = cr + I + 1;
+    double *tmp = (double *)malloc(sizeof(double) * 3 * (size_t)I + 8);
+    for (int64_t p = 0; p < P; ++p) {
+        const double *row = cpx + 2 * p * I;
+        double denom = 0.0, den_lin = 0.0;
+        for (int64_t i = 0; i < I; ++i) {
+            cr[i] = row[2 * i];
+            ci[i] = row[2 * i + 1];
+            w[i] = hypot(cr[i], ci[i]);
+            wb[i] = w[i] * bperp[i];
+            denom += w[i];
+            den_lin += wb[i] * wb[i];
+        }
+        if (denom == 0.0) denom = 1.0;
+        if (den_lin == 0.0) den_lin = 1.0;
+        double cmax = -INFINITY;
+        for (int64_t t = 0; t < nt; ++t) {
+            double sr = 0.0, si = 0.0;
+            for (int64_t i = 0; i < I; ++i) {
+                double er, ei;
+                cexp_neg((bperp[i] / range * (PYT_PI / 4.0)) * tm[t], &er, &ei);
+                sr += er * cr[i] - ei * ci[i];
+                si += er * ci[i] + ei * cr[i];
+            }
+            coh_t[t] = hypot(sr, si) / denom;
+            cmax = coh_t[t] > cmax ? coh_t[t] : cmax;
+        }
+        int64_t count = 0, only = 0;
+        for (int64_t t = 0; t < nt; ++t) {
+            int lm;
+            if (nt == 1)
+                lm = 1;
+            else if (t == 0)
+                lm = coh_t[0] >= coh_t[1];
+            else if (t == nt - 1)
+                lm = coh_t[nt - 1] >= coh_t[nt - 2];
+            else
+                lm = coh_t[t] >= coh_t[t - 1] && coh_t[t] >= coh_t[t + 1];
+            if (lm && coh_t[t] >= cmax - PYT_NEAR_MAX_TOL) {
+                if (count == 0) only = t;
+                ++count;
+            }
+        }
+        if (single_out) single_out[p] = (int8_t)(count == 1);
+        if (count == 1) {
+            double K, Cc;
+            refine_candidate(cr, ci, bperp, w, wb, I, den_lin, (PYT_PI / 4.0) / range * tm[only],
+                             &K, &Cc, &coh[p], res);
+        } else {
+            double K, Cc;
+            topofit_single(row, bperp, I, tm, nt, &K, &Cc, &coh[p], tmp, scratch);
+        }
+    }
+    free(tmp);
+    free(cr);
+    free(res);
+    free(w);
+    free(coh_t);
     free(scratch);
     free(tm);
 }

@@ -111,6 +111,8 @@ def lib() -> C.CDLL:
         C.c_int32, _f64p, _f64p]
     L.ps2_hip_topofit_rows.argtypes = [_f64p, _f64p, C.c_int32, C.c_int64, C.c_int64, C.c_double,
                                        C.c_int32, _f64p, _f64p, _f64p, C.POINTER(C.c_float)]
+    L.ps2_hip_topofit_coh_row_invariant.argtypes = [_f64p, _f64p, C.c_int64, C.c_int64, C.c_double,
+                                                    C.c_int32, _f64p]
     L.ps2_hip_grid_accumulate.argtypes = [C.POINTER(C.c_float), _i64p, C.c_int64, C.c_int64,
                                           C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_float)]
     L.ps2_hip_histogram_with_centers.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, _f64p]

@@ -30,6 +30,7 @@ struct PyFitParams {
     int64_t P;      // rows in this launch
     int64_t row0;   // first row of this launch
     int I, T, tn, rowinv;
+    int cohrule;    // _ps_topofit_batch_row_invariant_coh's rule (shared bperp only)
 };
 
 // Element (row k, interferogram i) of the fit's arrays, for either layout:
@@ -107,6 +108,68 @@ __global__ __launch_bounds__(256) void k_py_topofit(RowView rv, double *__restri
     if (local >= p.P) return;
     const int64_t k = p.row0 + local;
     const int I = p.I, T = p.T;
+    if (p.cohrule) {
+        // ported.py:2561-2608: trial grid over the whole bperp vector's range
+        // (zero elements included); a row with exactly one near-maximum local
+        // peak on it is refined from that peak, every other row continues
+        // below with _ps_topofit_single's fit (:2611-2614)
+        double fmn = INFINITY, fmx = -INFINITY, dn = 0.0, dl = 0.0;
+        for (int i = 0; i < I; ++i) {
+            const double b = rv.b(k, i);
+            fmn = fmin(fmn, b);
+            fmx = fmax(fmx, b);
+            const double2 v = rv.v(k, i);
+            if (v.x == 0.0 && v.y == 0.0) continue;
+            const double w = hypot(v.x, v.y);
+            dn += w;
+            dl += (w * b) * (w * b);
+        }
+        double fr = fmx - fmn;
+        if (fr == 0.0) fr = 1.0;
+        if (dn == 0.0) dn = 1.0;
+        if (dl == 0.0) dl = 1.0;
+        double cm = -INFINITY;
+        for (int t = 0; t < T; ++t) {
+            const double tm = (double)(t - p.tn);
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < I; ++i) {
+                const double2 v = rv.v(k, i);
+                if (v.x == 0.0 && v.y == 0.0) continue;
+                double rr, ri;
+                rot(v.x, v.y, (rv.b(k, i) / fr * kPi4) * tm, rr, ri);
+                sr += rr;
+                si += ri;
+            }
+            const double c = hypot(sr, si) / dn;
+            cohs[(int64_t)t * p.P + local] = c;
+            cm = fmax(cm, c);
+        }
+        int count = 0, only = 0;
+        for (int t = 0; t < T; ++t) {
+            const double c = cohs[(int64_t)t * p.P + local];
+            bool lm;
+            if (T == 1)
+                lm = true;
+            else if (t == 0)
+                lm = c >= cohs[(int64_t)1 * p.P + local];
+            else if (t == T - 1)
+                lm = c >= cohs[(int64_t)(T - 2) * p.P + local];
+            else
+                lm = c >= cohs[(int64_t)(t - 1) * p.P + local] && c >= cohs[(int64_t)(t + 1) * p.P + local];
+            if (lm && c >= cm - kNearMaxTol) {
+                if (count == 0) only = t;
+                ++count;
+            }
+        }
+        if (count == 1) {
+            double K, C, coh;
+            py_refine(rv, k, I, dl, kPi4 / fr * (double)(only - p.tn), K, C, coh, false);
+            Kout[k] = K;
+            Cout[k] = C;
+            cohout[k] = coh;
+            return;
+        }
+    }
     double mn = INFINITY, mx = -INFINITY, denom = 0.0, den_lin = 0.0;
     int m = 0;
     for (int i = 0; i < I; ++i) {
@@ -272,21 +335,28 @@ int set_device(int32_t device) {
 
 using namespace ps2;
 
-extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int32_t row_invariant,
-                                    int64_t n_rows, int64_t n_ifg, double n_trial_wraps,
-                                    int32_t device, double *k_out, double *c_out,
-                                    double *coh_out, float *res_out) {
-    if (n_rows < 0 || n_ifg <= 0 || (n_rows > 0 && (!cpx || !bperp || !k_out || !c_out || !coh_out))) {
-        set_error("topofit_rows: invalid argument");
+namespace {
+
+// ps2_hip_topofit_rows and ps2_hip_topofit_coh_row_invariant: host arrays in,
+// host arrays out, one kernel launch on a stream of the call's own.  k_out and
+// c_out may be null only under the coherence rule.
+int topofit_rows_host(const char *who, const double *cpx, const double *bperp, int32_t row_invariant,
+                      int cohrule, int64_t n_rows, int64_t n_ifg, double n_trial_wraps,
+                      int32_t device, double *k_out, double *c_out, double *coh_out,
+                      float *res_out) {
+    const std::string name(who);
+    if (n_rows < 0 || n_ifg <= 0 ||
+        (n_rows > 0 && (!cpx || !bperp || !coh_out || (!cohrule && (!k_out || !c_out))))) {
+        set_error(name + ": invalid argument");
         return PS2_E_ARG;
     }
     if (!(n_trial_wraps >= 0.0) || !std::isfinite(n_trial_wraps) || n_trial_wraps > 1.0e6) {
-        set_error("topofit_rows: n_trial_wraps must be finite and >= 0");
+        set_error(name + ": n_trial_wraps must be finite and >= 0");
         return PS2_E_ARG;
     }
     if (n_rows == 0) return PS2_OK;
     if (n_rows * n_ifg >= ((int64_t)1 << 31) / 2) {
-        set_error("topofit_rows: n_rows * n_ifg too large for one call");
+        set_error(name + ": n_rows * n_ifg too large for one call");
         return PS2_E_ARG;
     }
     int rc = set_device(device);
@@ -318,14 +388,14 @@ extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int3
     REG_HIP(cs.create());
     REG_HIP(hipMemcpyAsync(dc.p, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, cs.s));
     REG_HIP(hipMemcpyAsync(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, cs.s));
-    PyFitParams prm{P, 0, (int)I, T, tn, row_invariant ? 1 : 0};
+    PyFitParams prm{P, 0, (int)I, T, tn, row_invariant ? 1 : 0, cohrule};
     const RowView rv{dc.p, P, 1, db.p, row_invariant ? 1 : P, row_invariant ? 0 : 1,
                      res_out ? dres.p : nullptr, P, 1};
     hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, cs.s, rv, dcoh_t.p,
                        dK.p, dC.p, dcoh.p, prm);
     REG_HIP(hipGetLastError());
-    REG_HIP(hipMemcpyAsync(k_out, dK.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
-    REG_HIP(hipMemcpyAsync(c_out, dC.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
+    if (k_out) REG_HIP(hipMemcpyAsync(k_out, dK.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
+    if (c_out) REG_HIP(hipMemcpyAsync(c_out, dC.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
     REG_HIP(hipMemcpyAsync(coh_out, dcoh.p, sizeof(double) * P, hipMemcpyDeviceToHost, cs.s));
     REG_HIP(hipStreamSynchronize(cs.s));
     if (res_out) {
@@ -339,6 +409,24 @@ extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int3
             }
     }
     return PS2_OK;
+}
+
+}  // namespace
+
+extern "C" int ps2_hip_topofit_rows(const double *cpx, const double *bperp, int32_t row_invariant,
+                                    int64_t n_rows, int64_t n_ifg, double n_trial_wraps,
+                                    int32_t device, double *k_out, double *c_out,
+                                    double *coh_out, float *res_out) {
+    return topofit_rows_host("topofit_rows", cpx, bperp, row_invariant, 0, n_rows, n_ifg,
+                             n_trial_wraps, device, k_out, c_out, coh_out, res_out);
+}
+
+extern "C" int ps2_hip_topofit_coh_row_invariant(const double *cpx, const double *bperp_vec,
+                                                 int64_t n_rows, int64_t n_ifg,
+                                                 double n_trial_wraps, int32_t device,
+                                                 double *coh_out) {
+    return topofit_rows_host("topofit_coh_row_invariant", cpx, bperp_vec, 1, 1, n_rows, n_ifg,
+                             n_trial_wraps, device, nullptr, nullptr, coh_out, nullptr);
 }
 
 // ---- allocation-free topofit ----------------------------------------------
@@ -421,7 +509,7 @@ extern "C" int ps2_hip_topofit_rows_noalloc(const double *d_cpx, const double *d
                      row_invariant ? 0 : n_ifg, reinterpret_cast<float2 *>(d_res_out), 1, n_ifg};
     for (int64_t row0 = 0; row0 < n_rows; row0 += cap_rows) {
         const int64_t rows = std::min(cap_rows, n_rows - row0);
-        PyFitParams prm{rows, row0, (int)n_ifg, (int)T, tn, row_invariant ? 1 : 0};
+        PyFitParams prm{rows, row0, (int)n_ifg, (int)T, tn, row_invariant ? 1 : 0, 0};
         hipLaunchKernelGGL(k_py_topofit, dim3((unsigned)((rows + kNoAllocRowBlock - 1) / kNoAllocRowBlock)),
                            dim3(kNoAllocRowBlock), 0, st, rv, cohs, d_k_out, d_c_out, d_coh_out, prm);
         REG_HIP(hipGetLastError());

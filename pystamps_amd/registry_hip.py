@@ -51,21 +51,28 @@ def available() -> bool:
         return False
 
 
+def _shared_bperp(bperp, P, I):
+    """The one bperp vector of a row-invariant call (1-D, or the first row of
+    a [1|n_row][n_ifg] matrix)."""
+    bp = np.asarray(bperp, dtype=np.float64)
+    if bp.ndim == 2:
+        if bp.shape[0] not in (1, P):
+            raise BackendError(f"Row-invariant stage-2 bperp has incompatible shape {bp.shape} "
+                               f"for n_row={P}")
+        bp = bp.reshape(-1) if bp.shape[0] == 1 else bp[0, :]
+    bp = np.ascontiguousarray(bp.reshape(-1))
+    if bp.size != I:
+        raise BackendError(f"bperp has {bp.size} values for {I} interferograms")
+    return bp
+
+
 def _topofit(cpxphase, bperp, n_trial_wraps, row_invariant: bool, device: int = 0):
     cpx = np.ascontiguousarray(cpxphase, dtype=np.complex128)
     if cpx.ndim != 2:
         raise BackendError("stage-2 topofit expects a 2-D cpxphase")
     P, I = cpx.shape
     if row_invariant:
-        bp = np.asarray(bperp, dtype=np.float64)
-        if bp.ndim == 2:
-            if bp.shape[0] not in (1, P):
-                raise BackendError(f"Row-invariant stage-2 bperp has incompatible shape {bp.shape} "
-                                   f"for n_row={P}")
-            bp = bp.reshape(-1) if bp.shape[0] == 1 else bp[0, :]
-        bp = np.ascontiguousarray(bp.reshape(-1))
-        if bp.size != I:
-            raise BackendError(f"bperp has {bp.size} values for {I} interferograms")
+        bp = _shared_bperp(bperp, P, I)
     else:
         bp = np.ascontiguousarray(bperp, dtype=np.float64)
         if bp.shape != (P, I):
@@ -91,9 +98,23 @@ def stage2_topofit_row_invariant(cpxphase, bperp, n_trial_wraps, threads: int = 
     return _topofit(cpxphase, bperp, n_trial_wraps, row_invariant=True)
 
 
-def stage2_topofit_coh_row_invariant(cpxphase, bperp, n_trial_wraps, threads: int = 0):
-    """Coherence only of stage2_topofit_row_invariant."""
-    return _topofit(cpxphase, bperp, n_trial_wraps, row_invariant=True)[2]
+def stage2_topofit_coh_row_invariant(cpxphase, bperp, n_trial_wraps, threads: int = 0, device: int = 0):
+    """Coherence per row against one bperp vector, by the rule of
+    _ps_topofit_batch_row_invariant_coh (pystamps/pipeline/ported.py:2555-2616):
+    rows with one near-maximum peak on the trial grid of the whole vector's
+    range are refined from it; the others take stage2_topofit_row_invariant's
+    coherence."""
+    cpx = np.ascontiguousarray(cpxphase, dtype=np.complex128)
+    if cpx.ndim != 2:
+        raise BackendError("stage-2 topofit expects a 2-D cpxphase")
+    P, I = cpx.shape
+    bp = _shared_bperp(bperp, P, I)
+    coh = np.zeros(P)
+    if P:
+        _check(_native.lib().ps2_hip_topofit_coh_row_invariant(
+            cpx.ctypes.data_as(_f64p), bp.ctypes.data_as(_f64p), P, I, float(n_trial_wraps),
+            int(device), coh.ctypes.data_as(_f64p)))
+    return coh
 
 
 def stage2_grid_accumulate(ph_weight, grid_lin, n_i, n_j, threads: int = 0):

@@ -111,6 +111,55 @@ def test_real_row_takes_the_near_max_refinement_path(ora):
     np.testing.assert_allclose(np.abs(res[0]), np.abs(cpx), atol=1e-6)   # a rotation (complex64 out)
 
 
+def _coh_rule_rows(seed=8, P=400, I=24):
+    """Rows against one bperp vector, symmetric about 0; a third of them have
+    zero elements at the vector's extremes (the columns that set its range),
+    some random holes.  The last eighth are real rows cos(7 s b) (s the trial
+    step of the whole vector), whose trial coherence is symmetric in the trial,
+    so its peaks at the grid's two ends tie; half of those miss both extremes."""
+    rng = np.random.default_rng(seed)
+    u = np.sort(rng.uniform(5, 220, I // 2))
+    bp = np.concatenate([-u[::-1], np.zeros(I % 2), u])
+    k = rng.uniform(-0.008, 0.008, P)
+    cpx = np.exp(1j * (k[:, None] * bp + rng.uniform(0.0, 1.2, P)[:, None] * rng.standard_normal((P, I))))
+    ext = [int(np.argmin(bp)), int(np.argmax(bp))]
+    cpx[: P // 3, ext[0]] = 0.0
+    cpx[P // 6: P // 2, ext[1]] = 0.0
+    cpx[rng.random((P, I)) < 0.02] = 0.0
+    tie = P - P // 8
+    step = (np.pi / 4) / (bp.max() - bp.min())
+    cpx[tie:] = np.cos(7 * step * bp) * rng.uniform(0.5, 2.0, (P - tie, 1))
+    cpx[tie::2, ext] = 0.0
+    return cpx, bp, ext
+
+
+def test_coh_row_invariant_rule_pins(ora):
+    """_ps_topofit_batch_row_invariant_coh (ported.py:2555-2616).  Rows whose
+    whole-vector trial grid has one near-maximum peak equal the subset fit of
+    the same row with a 1e-200 phasor put in every zero element (which makes
+    the subset's range the whole vector's and changes no sum); the other rows
+    equal the subset fit of the row as it is; rows without zeros equal it
+    exactly either way."""
+    cpx, bp, ext = _coh_rule_rows()
+    coh, single = ora.py_topofit_coh_row_invariant(cpx, bp, 0.7)
+    _, _, coh_sub, _ = ora.py_topofit_rows(cpx, bp, 0.7)
+    filled = np.where(cpx == 0, 1e-200 * np.exp(1j * 0.3), cpx)
+    _, _, coh_fill, _ = ora.py_topofit_rows(filled, bp, 0.7)
+    assert single.sum() > 200 and (~single).sum() > 0
+    np.testing.assert_allclose(coh[single], coh_fill[single], rtol=0, atol=1e-13)
+    np.testing.assert_array_equal(coh[~single], coh_sub[~single])
+    whole = np.all(cpx != 0, axis=1)
+    np.testing.assert_array_equal(coh[whole], coh_sub[whole])
+    # the rule is seen: rows missing an extreme element differ from the subset fit
+    gap = np.abs(coh - coh_sub)[single & ~np.all(cpx[:, ext] != 0, axis=1)]
+    assert gap.size > 50 and np.count_nonzero(gap > 1e-6) > gap.size // 4
+    # all-zero rows: NaN on a grid of several trials, 0 on a single-trial grid
+    z = np.zeros((2, 6))
+    assert np.all(np.isnan(ora.py_topofit_coh_row_invariant(z, bp[:6], 0.7)[0]))
+    c0, s0 = ora.py_topofit_coh_row_invariant(z, bp[:6], 0.0)
+    assert np.array_equal(c0, [0.0, 0.0]) and s0.all()
+
+
 def test_py_grid_accumulate_matches_numpy_add_at(ora):
     rng = np.random.default_rng(3)
     P, I, ni, nj = 700, 9, 6, 7
@@ -189,8 +238,23 @@ def test_gpu_topofit_matches_oracle(ora):
     Kr, Cr, cr, rr = registry_hip.stage2_topofit_row_invariant(cpx, np.tile(sym, (P, 1)), 0.7)
     Ko2, Co2, co2, ro2 = ora.py_topofit_rows(cpx, sym, 0.7)
     np.testing.assert_allclose(Kr, Ko2, rtol=0, atol=1e-9, equal_nan=True)
-    c_only = registry_hip.stage2_topofit_coh_row_invariant(cpx, sym, 0.7)
-    np.testing.assert_array_equal(c_only, cr)
+    np.testing.assert_allclose(cr, co2, rtol=0, atol=1e-9, equal_nan=True)
+
+
+@pytest.mark.gpu
+def test_gpu_topofit_coh_row_invariant_matches_oracle(ora):
+    from pystamps_amd import registry_hip
+    cpx, bp, ext = _coh_rule_rows(seed=9, P=3000, I=33)
+    cpx[5] = 0.0
+    got = registry_hip.stage2_topofit_coh_row_invariant(cpx, bp, 0.7)
+    want, single = ora.py_topofit_coh_row_invariant(cpx, bp, 0.7)
+    assert single.sum() > 1000 and np.isnan(got[5])
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9, equal_nan=True)
+    # a [n_row][n_ifg] bperp matrix of equal rows is the same call
+    got2 = registry_hip.stage2_topofit_coh_row_invariant(cpx, np.tile(bp, (cpx.shape[0], 1)), 0.7)
+    np.testing.assert_array_equal(got2, got)
+    z = registry_hip.stage2_topofit_coh_row_invariant(np.zeros((3, 6)), bp[:6], 0.0)
+    assert np.array_equal(z, np.zeros(3))
 
 
 @pytest.mark.gpu

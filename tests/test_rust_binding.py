@@ -162,7 +162,7 @@ def test_repr_c_offsets_equal_gcc_offsetof(tmp_path):
 
 def test_extern_functions_mirror_the_header():
     h, r = header_functions(), rust_functions()
-    assert len(h) == 25
+    assert len(h) == 26
     assert set(h) == set(r), (set(h) ^ set(r))
     for name, (ret, params) in h.items():
         assert r[name][1] == params, (name, params, r[name][1])
