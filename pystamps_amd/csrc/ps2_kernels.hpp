@@ -18,6 +18,8 @@
 
 #include "ps2_device.hpp"
 
+#include <type_traits>
+
 namespace ps2 {
 
 #ifndef PS2_CLAP_MEDIAN_HIST
@@ -36,6 +38,40 @@ constexpr int CLAP_INC = 6;    // n_inc = n_win / 4
 
 // gfx950 dispatches workgroup b to XCD b % 8 (8 XCDs, each with its own L2)
 #define PS2_NUM_XCD 8
+
+// Kernel launches of the gamma loop take their arguments as one struct A.
+// Launched for one patch (template flag M = false) a kernel takes A itself and
+// uses its own workgroup index.  Launched once for up to PS2_PACK patches
+// (M = true) it takes a MultiPack: workgroups range[2j] .. range[2j] +
+// range[2j+1] - 1 run patch j with a[j] and local index b - range[2j].  The
+// ranges start at multiples of 8, so a local index keeps the XCD of the global
+// one; workgroups of the padding between ranges return at once.  The pack is
+// a kernel argument, like A, so the pointers in it are known to address
+// global memory (loaded from a table in memory they would be generic pointers
+// and every access a flat one).
+#define PS2_PACK 8
+template <typename A>
+struct MultiPack {
+    A a[PS2_PACK];
+    uint32_t range[2 * PS2_PACK];   // (first workgroup, workgroups), ascending
+    int32_t n;
+};
+template <bool M, typename A>
+using KArg = typename std::conditional<M, MultiPack<A>, A>::type;
+
+template <typename A>
+__device__ __forceinline__ bool multi_locate(const MultiPack<A> &p, A &a, uint32_t &bid, uint32_t &nblk) {
+    const uint32_t b = blockIdx.x;
+    int j = p.n - 1;
+    while (j > 0 && p.range[2 * j] > b) --j;
+    const uint32_t loc = b - p.range[2 * j];
+    const uint32_t cnt = p.range[2 * j + 1];
+    if (loc >= cnt) return false;
+    a = p.a[j];
+    bid = loc;
+    nblk = cnt;
+    return true;
+}
 
 // ---------------------------------------------------------------------------
 // fill_phase_weight value (native_stage2.rs:648-652):
@@ -57,17 +93,23 @@ __device__ inline cplx<R> weight_value(cplx<R> s, R b, double K, double W) {
 // With Kprev/Wprev non-null, the K and weighting this fill used are kept
 // there (for the ph_weight output of the last fill).
 template <typename R>
-__global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ phnm,
-                                                   const R *__restrict__ bp, int bp_rowinv,
-                                                   const double *__restrict__ K,
-                                                   const double *__restrict__ W,
-                                                   const int32_t *__restrict__ cell_start,
-                                                   cplx<R> *__restrict__ grid, int64_t C, int I,
-                                                   int64_t P, double *__restrict__ Kprev,
-                                                   double *__restrict__ Wprev,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void fill_grid_body(
+    const cplx<R> *__restrict__ phnm,
+    const R *__restrict__ bp,
+    int bp_rowinv,
+    const double *__restrict__ K,
+    const double *__restrict__ W,
+    const int32_t *__restrict__ cell_start,
+    cplx<R> *__restrict__ grid,
+    int64_t C,
+    int I,
+    int64_t P,
+    double *__restrict__ Kprev,
+    double *__restrict__ Wprev,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gid = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (gid >= C * I) return;
     const int f = (int)(gid / C);
     const int64_t c = gid - (int64_t)f * C;
@@ -89,6 +131,35 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
     grid[gid] = {sre, sim};
 }
 
+template <typename R>
+struct FillArgs {
+    const cplx<R> * phnm;
+    const R * bp;
+    int bp_rowinv;
+    const double * K;
+    const double * W;
+    const int32_t * cell_start;
+    cplx<R> * grid;
+    int64_t C;
+    int I;
+    int64_t P;
+    double * Kprev;
+    double * Wprev;
+    const int32_t * gate;
+};
+
+template <bool M, typename R>
+__global__ __launch_bounds__(256) void k_fill_grid(KArg<M, FillArgs<R>> p) {
+    if constexpr (M) {
+        FillArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        fill_grid_body<R>(a.phnm, a.bp, a.bp_rowinv, a.K, a.W, a.cell_start, a.grid, a.C, a.I, a.P, a.Kprev, a.Wprev, a.gate, bid_, nblk_);
+    } else {
+        fill_grid_body<R>(p.phnm, p.bp, p.bp_rowinv, p.K, p.W, p.cell_start, p.grid, p.C, p.I, p.P, p.Kprev, p.Wprev, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
 // The same step with the phase weights computed cooperatively: a 256-thread
 // workgroup covers CB consecutive cells and FG = 256 / CB interferogram
 // planes; the weights of the cells' pixels (a contiguous sorted-pixel range)
@@ -98,16 +169,22 @@ __global__ __launch_bounds__(256) void k_fill_grid(const cplx<R> *__restrict__ p
 // count of CB cells: LDS per workgroup bounds occupancy.  Kprev/Wprev as in
 // k_fill_grid.
 template <typename R, int CB, int CHUNK>
-__global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restrict__ phnm,
-                                                          const R *__restrict__ bp, int bp_rowinv,
-                                                          const double *__restrict__ K,
-                                                          const double *__restrict__ W,
-                                                          const int32_t *__restrict__ cell_start,
-                                                          cplx<R> *__restrict__ grid, int64_t C,
-                                                          int I, int64_t P, int npg,
-                                                          double *__restrict__ Kprev,
-                                                          double *__restrict__ Wprev,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void fill_grid_staged_body(
+    const cplx<R> *__restrict__ phnm,
+    const R *__restrict__ bp,
+    int bp_rowinv,
+    const double *__restrict__ K,
+    const double *__restrict__ W,
+    const int32_t *__restrict__ cell_start,
+    cplx<R> *__restrict__ grid,
+    int64_t C,
+    int I,
+    int64_t P,
+    int npg,
+    double *__restrict__ Kprev,
+    double *__restrict__ Wprev,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
     constexpr int FG = 256 / CB;
     __shared__ cplx<R> vals[FG][CHUNK];
@@ -115,8 +192,8 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
     // XCD-aware order: cell group cg belongs to XCD cg % 8, and its npg
     // plane groups are consecutive workgroups there, so the group's K, W and
     // cell_start reads are shared through that XCD's L2
-    const int xcd = blockIdx.x % PS2_NUM_XCD;
-    const int loc = blockIdx.x / PS2_NUM_XCD;
+    const int xcd = bid_ % PS2_NUM_XCD;
+    const int loc = bid_ / PS2_NUM_XCD;
     const int cgl = loc / npg;
     const int pg = loc - cgl * npg;
     const int64_t c0 = ((int64_t)cgl * PS2_NUM_XCD + xcd) * CB;
@@ -188,6 +265,36 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(const cplx<R> *__restr
         __syncthreads();
     }
     if (mine) grid[(int64_t)f * C + c] = {sre, sim};
+}
+
+template <typename R>
+struct FillStagedArgs {
+    const cplx<R> * phnm;
+    const R * bp;
+    int bp_rowinv;
+    const double * K;
+    const double * W;
+    const int32_t * cell_start;
+    cplx<R> * grid;
+    int64_t C;
+    int I;
+    int64_t P;
+    int npg;
+    double * Kprev;
+    double * Wprev;
+    const int32_t * gate;
+};
+
+template <bool M, typename R, int CB, int CHUNK>
+__global__ __launch_bounds__(256) void k_fill_grid_staged(KArg<M, FillStagedArgs<R>> p) {
+    if constexpr (M) {
+        FillStagedArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        fill_grid_staged_body<R, CB, CHUNK>(a.phnm, a.bp, a.bp_rowinv, a.K, a.W, a.cell_start, a.grid, a.C, a.I, a.P, a.npg, a.Kprev, a.Wprev, a.gate, bid_, nblk_);
+    } else {
+        fill_grid_staged_body<R, CB, CHUNK>(p.phnm, p.bp, p.bp_rowinv, p.K, p.W, p.cell_start, p.grid, p.C, p.I, p.P, p.npg, p.Kprev, p.Wprev, p.gate, blockIdx.x, gridDim.x);
+    }
 }
 
 // Outputs in the caller's row order [P][I] from sorted ifg-major planes [I][P]:
@@ -426,11 +533,13 @@ __device__ inline R team_median(const typename RT<R>::key_t (&keys)[32], bool hi
 }
 
 template <typename R, int TEAMS, bool ALPHA_ONE>
-__global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(const cplx<R> *__restrict__ grid,
-                                                           cplx<R> *__restrict__ part,
-                                                           const R *__restrict__ lp,
-                                                           ClapParams<R> prm,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void clap_strip_body(
+    const cplx<R> *__restrict__ grid,
+    cplx<R> *__restrict__ part,
+    const R *__restrict__ lp,
+    ClapParams<R> prm,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
     const int team = threadIdx.x >> 5, lane = threadIdx.x & 31;
     const bool hi_team = (threadIdx.x & 63) >= 32;
@@ -438,8 +547,8 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
     // pair g, segment seg) belongs to XCD u % 8, which walks its strips' window
     // rows top to bottom, so the 4 window rows that read a grid row run close
     // together on the XCD whose L2 holds it.
-    const int xcd = blockIdx.x % PS2_NUM_XCD;
-    const int loc = blockIdx.x / PS2_NUM_XCD;
+    const int xcd = bid_ % PS2_NUM_XCD;
+    const int loc = bid_ / PS2_NUM_XCD;
     const int ix = loc / prm.upx;
     const int u = (loc - ix * prm.upx) * PS2_NUM_XCD + xcd;
     if (u >= prm.units) return;
@@ -663,6 +772,27 @@ __global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8
     }
 }
 
+template <typename R>
+struct StripArgs {
+    const cplx<R> * grid;
+    cplx<R> * part;
+    const R * lp;
+    ClapParams<R> prm;
+    const int32_t * gate;
+};
+
+template <bool M, typename R, int TEAMS, bool ALPHA_ONE>
+__global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(KArg<M, StripArgs<R>> p) {
+    if constexpr (M) {
+        StripArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        clap_strip_body<R, TEAMS, ALPHA_ONE>(a.grid, a.part, a.lp, a.prm, a.gate, bid_, nblk_);
+    } else {
+        clap_strip_body<R, TEAMS, ALPHA_ONE>(p.grid, p.part, p.lp, p.prm, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
 // unit phase of a complex value with the reference's f32/f64 arithmetic
 // (normalize_complex_unit_magnitude, native_stage2.rs:1198-1205)
 template <typename R>
@@ -689,16 +819,24 @@ __device__ inline cplx<R> unit_phase(cplx<R> v) {
 // Grid: tiles x I, the interferogram slowest.
 constexpr int COMB_TR = 4 * CLAP_INC, COMB_TC = 32;
 template <typename R, bool UNIT, int MAXSEG>
-__global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict__ part,
-                                                      cplx<R> *__restrict__ filt, int ni, int nj,
-                                                      int I, int nwi, int nseg, int wseg, int nplanes,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void clap_combine_body(
+    const cplx<R> *__restrict__ part,
+    cplx<R> *__restrict__ filt,
+    int ni,
+    int nj,
+    int I,
+    int nwi,
+    int nseg,
+    int wseg,
+    int nplanes,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
     static_assert(COMB_TR * COMB_TC % 256 == 0, "whole tile per pass");
     __shared__ cplx<R> tile[COMB_TC][COMB_TR + 1];   // [column][row]
     const int tj_n = (nj + COMB_TC - 1) / COMB_TC;
     const int tiles = ((ni + COMB_TR - 1) / COMB_TR) * tj_n;
-    const int f = (int)(blockIdx.x / tiles), tl = (int)(blockIdx.x - (unsigned)f * tiles);
+    const int f = (int)(bid_ / tiles), tl = (int)(bid_ - (unsigned)f * tiles);
     const int r0 = (tl / tj_n) * COMB_TR, c0 = (tl % tj_n) * COMB_TC;
     const int64_t C = (int64_t)ni * nj;
     const cplx<R> *pf = part + (int64_t)f * nwi * nplanes * nj * CLAP_WIN;
@@ -768,6 +906,32 @@ __global__ __launch_bounds__(256) void k_clap_combine(const cplx<R> *__restrict_
         const int rl = e / COMB_TC, cl = e - rl * COMB_TC;
         const int r = r0 + rl, c = c0 + cl;
         if (r < ni && c < nj) filt[(int64_t)f * C + (int64_t)r * nj + c] = tile[cl][rl];
+    }
+}
+
+template <typename R>
+struct CombineArgs {
+    const cplx<R> * part;
+    cplx<R> * filt;
+    int ni;
+    int nj;
+    int I;
+    int nwi;
+    int nseg;
+    int wseg;
+    int nplanes;
+    const int32_t * gate;
+};
+
+template <bool M, typename R, bool UNIT, int MAXSEG>
+__global__ __launch_bounds__(256) void k_clap_combine(KArg<M, CombineArgs<R>> p) {
+    if constexpr (M) {
+        CombineArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        clap_combine_body<R, UNIT, MAXSEG>(a.part, a.filt, a.ni, a.nj, a.I, a.nwi, a.nseg, a.wseg, a.nplanes, a.gate, bid_, nblk_);
+    } else {
+        clap_combine_body<R, UNIT, MAXSEG>(p.part, p.filt, p.ni, p.nj, p.I, p.nwi, p.nseg, p.wseg, p.nplanes, p.gate, blockIdx.x, gridDim.x);
     }
 }
 
@@ -937,20 +1101,22 @@ __device__ inline float div_from_rcp(float e, float d, float rcp) {
 // NQ (single-chunk path only): tn known at compile time (NQ = tn > 0), so the
 // trial loop has no per-trial test of a runtime count; 0 reads tn at run time.
 template <typename R, int TC, bool SINGLE = false, int NQ = 0>
-__global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
-                                              const R *__restrict__ bp,
-                                              const cplx<R> *__restrict__ wt,
-                                              const cplx<R> *__restrict__ filt,
-                                              const int32_t *__restrict__ cell_of,
-                                              int32_t *__restrict__ tix,
-                                              const R *__restrict__ rng,
-                                              cplx<R> *__restrict__ offset,
-                                              int32_t *__restrict__ rescore_list,
-                                              uint32_t *__restrict__ rescore_count,
-                                              FitParams prm,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void scan_body(
+    const cplx<R> *__restrict__ phnm,
+    const R *__restrict__ bp,
+    const cplx<R> *__restrict__ wt,
+    const cplx<R> *__restrict__ filt,
+    const int32_t *__restrict__ cell_of,
+    int32_t *__restrict__ tix,
+    const R *__restrict__ rng,
+    cplx<R> *__restrict__ offset,
+    int32_t *__restrict__ rescore_list,
+    uint32_t *__restrict__ rescore_count,
+    FitParams prm,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= prm.P) return;
     const int I = prm.I;
     const int64_t P = prm.P;
@@ -1182,6 +1348,34 @@ __global__ __launch_bounds__(256) void k_scan(const cplx<R> *__restrict__ phnm,
     }
 }
 
+template <typename R>
+struct ScanArgs {
+    const cplx<R> * phnm;
+    const R * bp;
+    const cplx<R> * wt;
+    const cplx<R> * filt;
+    const int32_t * cell_of;
+    int32_t * tix;
+    const R * rng;
+    cplx<R> * offset;
+    int32_t * rescore_list;
+    uint32_t * rescore_count;
+    FitParams prm;
+    const int32_t * gate;
+};
+
+template <bool M, typename R, int TC, bool SINGLE = false, int NQ = 0>
+__global__ __launch_bounds__(256) void k_scan(KArg<M, ScanArgs<R>> p) {
+    if constexpr (M) {
+        ScanArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        scan_body<R, TC, SINGLE, NQ>(a.phnm, a.bp, a.wt, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.rescore_list, a.rescore_count, a.prm, a.gate, bid_, nblk_);
+    } else {
+        scan_body<R, TC, SINGLE, NQ>(p.phnm, p.bp, p.wt, p.filt, p.cell_of, p.tix, p.rng, p.offset, p.rescore_list, p.rescore_count, p.prm, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
 // FP64 re-score of near-tie rows (FP32 form): one 64-lane wavefront per
 // queued row, lanes over interferograms, wave sums per trial; first argmax
 // with strict > over trials in ascending order, as ref64's topofit_row
@@ -1207,22 +1401,25 @@ __device__ inline void wave_sum4_f64(double &a, double &b, double &c, double &d)
 }
 
 template <typename RF>
-__global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phnm,
-                                                const double *__restrict__ bp64,
-                                                const cplx<RF> *__restrict__ filt,
-                                                const int32_t *__restrict__ cell_of,
-                                                const int32_t *__restrict__ rescore_list,
-                                                const uint32_t *__restrict__ rescore_count,
-                                                int32_t *__restrict__ tix,
-                                                cplx<RF> *__restrict__ offset, FitParams prm,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void rescore_body(
+    const cplx<RF> *__restrict__ phnm,
+    const double *__restrict__ bp64,
+    const cplx<RF> *__restrict__ filt,
+    const int32_t *__restrict__ cell_of,
+    const int32_t *__restrict__ rescore_list,
+    const uint32_t *__restrict__ rescore_count,
+    int32_t *__restrict__ tix,
+    cplx<RF> *__restrict__ offset,
+    FitParams prm,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
     // lane l holds interferograms l, l + 64, ... (up to RESCORE_PER_LANE of
     // them) for every trial; larger I re-reads the rest per trial
     constexpr int RESCORE_PER_LANE = 4;
     const uint32_t n = *rescore_count;
     const int lane = threadIdx.x;
-    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+    for (uint32_t q = bid_; q < n; q += nblk_) {
         const int64_t k = rescore_list[q];
         const int I = prm.I;
         const int64_t P = prm.P, C = prm.C;
@@ -1360,6 +1557,32 @@ __global__ __launch_bounds__(64) void k_rescore(const cplx<RF> *__restrict__ phn
             tix[k] = besti;
             offset[k] = {(RF)bsr, (RF)bsi};
         }
+    }
+}
+
+template <typename RF>
+struct RescoreArgs {
+    const cplx<RF> * phnm;
+    const double * bp64;
+    const cplx<RF> * filt;
+    const int32_t * cell_of;
+    const int32_t * rescore_list;
+    const uint32_t * rescore_count;
+    int32_t * tix;
+    cplx<RF> * offset;
+    FitParams prm;
+    const int32_t * gate;
+};
+
+template <bool M, typename RF>
+__global__ __launch_bounds__(64) void k_rescore(KArg<M, RescoreArgs<RF>> p) {
+    if constexpr (M) {
+        RescoreArgs<RF> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        rescore_body<RF>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.rescore_count, a.tix, a.offset, a.prm, a.gate, bid_, nblk_);
+    } else {
+        rescore_body<RF>(p.phnm, p.bp64, p.filt, p.cell_of, p.rescore_list, p.rescore_count, p.tix, p.offset, p.prm, p.gate, blockIdx.x, gridDim.x);
     }
 }
 
@@ -1656,22 +1879,27 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm,
-                                                const R *__restrict__ bp,
-                                                const cplx<R> *__restrict__ filt,
-                                                const int32_t *__restrict__ cell_of,
-                                                const int32_t *__restrict__ tix,
-                                                const R *__restrict__ rng,
-                                                const cplx<R> *__restrict__ offset,
-                                                double *__restrict__ Kout,
-                                                double *__restrict__ Cout,
-                                                double *__restrict__ cohout,
-                                                double *__restrict__ nopt, FitParams prm,
-                                                const double *__restrict__ mids, HistParams hp,
-                                                RecheckIn rc, uint8_t *__restrict__ flags,
-                                                const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void refine_body(
+    const cplx<R> *__restrict__ phnm,
+    const R *__restrict__ bp,
+    const cplx<R> *__restrict__ filt,
+    const int32_t *__restrict__ cell_of,
+    const int32_t *__restrict__ tix,
+    const R *__restrict__ rng,
+    const cplx<R> *__restrict__ offset,
+    double *__restrict__ Kout,
+    double *__restrict__ Cout,
+    double *__restrict__ cohout,
+    double *__restrict__ nopt,
+    FitParams prm,
+    const double *__restrict__ mids,
+    HistParams hp,
+    RecheckIn rc,
+    uint8_t *__restrict__ flags,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= prm.P) return;
     double coh;
     float amax = 0.0f, cut = 4.0f;
@@ -1694,6 +1922,39 @@ __global__ __launch_bounds__(256) void k_refine(const cplx<R> *__restrict__ phnm
                                    (rc.psquare && decision_distance(coh, mids, hp) < bound))
                        ? 1
                        : 0;
+    }
+}
+
+template <typename R>
+struct RefineArgs {
+    const cplx<R> * phnm;
+    const R * bp;
+    const cplx<R> * filt;
+    const int32_t * cell_of;
+    const int32_t * tix;
+    const R * rng;
+    const cplx<R> * offset;
+    double * Kout;
+    double * Cout;
+    double * cohout;
+    double * nopt;
+    FitParams prm;
+    const double * mids;
+    HistParams hp;
+    RecheckIn rc;
+    uint8_t * flags;
+    const int32_t * gate;
+};
+
+template <bool M, typename R>
+__global__ __launch_bounds__(256) void k_refine(KArg<M, RefineArgs<R>> p) {
+    if constexpr (M) {
+        RefineArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        refine_body<R>(a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.Kout, a.Cout, a.cohout, a.nopt, a.prm, a.mids, a.hp, a.rc, a.flags, a.gate, bid_, nblk_);
+    } else {
+        refine_body<R>(p.phnm, p.bp, p.filt, p.cell_of, p.tix, p.rng, p.offset, p.Kout, p.Cout, p.cohout, p.nopt, p.prm, p.mids, p.hp, p.rc, p.flags, p.gate, blockIdx.x, gridDim.x);
     }
 }
 
@@ -1729,25 +1990,32 @@ struct FinalIn {
     const int32_t *tix;
 };
 
-template <typename R, int TP, int TF>
-__global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict__ phnm,
-                                                       const R *__restrict__ bp, int rowinv,
-                                                       const cplx<R> *__restrict__ filt,
-                                                       const cplx<R> *__restrict__ grid,
-                                                       const int32_t *__restrict__ cell_of,
-                                                       const int64_t *__restrict__ order, FinalIn in,
-                                                       FinalOut o, int64_t P, int I, int64_t C) {
-    static_assert(TP * TF % 256 == 0 && 256 % TP == 0 && 256 % TF == 0 || TF == 256, "tile shape");
+template <typename R, int TP, int TF, int NT>
+__device__ __forceinline__ void final_outputs_body(
+    const cplx<R> *__restrict__ phnm,
+    const R *__restrict__ bp,
+    int rowinv,
+    const cplx<R> *__restrict__ filt,
+    const cplx<R> *__restrict__ grid,
+    const int32_t *__restrict__ cell_of,
+    const int64_t *__restrict__ order,
+    FinalIn in,
+    FinalOut o,
+    int64_t P,
+    int I,
+    int64_t C,
+    const uint32_t bid_, const uint32_t nblk_) {
+    static_assert(TP * TF % NT == 0 && NT % TP == 0 && NT % TF == 0 && NT % 32 == 0, "tile shape");
     static_assert(TP * (TF + 1) >= 32 * 33, "the grid transpose shares the tile");
     __shared__ float2 tile[TP][TF + 1];
-    const int64_t b = blockIdx.x;
+    const int64_t b = bid_;
     if (b >= o.n_pix_blocks) {   // ph_grid transpose tile
         float2(*t)[33] = reinterpret_cast<float2(*)[33]>(&tile[0][0]);
         const int64_t gb = b - o.n_pix_blocks;
         const int64_t c0 = (gb / o.n_grid_ifg_tiles) * 32;
         const int f0 = (int)(gb % o.n_grid_ifg_tiles) * 32;
         const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-        for (int r = ty; r < 32; r += 8) {
+        for (int r = ty; r < 32; r += NT / 32) {
             const int f = f0 + r;
             const int64_t c = c0 + tx;
             if (f < I && c < C) {
@@ -1756,7 +2024,7 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
             }
         }
         __syncthreads();
-        for (int r = ty; r < 32; r += 8) {
+        for (int r = ty; r < 32; r += NT / 32) {
             const int64_t c = c0 + r;
             const int f = f0 + tx;
             if (f < I && c < C) o.grid[c * I + f] = t[tx][r];
@@ -1780,8 +2048,8 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
     // read side: a thread keeps one pixel of the tile (kl) and takes the
     // interferograms fl0 + 8 q; indices are clamped into the tile so that every
     // load of the thread is issued at once (out-of-tile values are not stored)
-    constexpr int PER = TP * TF / 256;
-    constexpr int FSTEP = 256 / TP;   // interferograms between a thread's reads
+    constexpr int PER = TP * TF / NT;
+    constexpr int FSTEP = NT / TP;   // interferograms between a thread's reads
     const int kl = threadIdx.x % TP, fl0 = threadIdx.x / TP;
     const int64_t k = k0 + min(kl, np - 1);
     const int32_t cell = cell_of[k];
@@ -1815,7 +2083,7 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
         }
     }
     // write side: thread t stores column t % TF of rows t / TF + KSTEP q
-    constexpr int KSTEP = TF >= 256 ? 1 : 256 / TF;
+    constexpr int KSTEP = NT / TF;
     const int fw = threadIdx.x % TF, kw0 = threadIdx.x / TF;
     int64_t orow[PER];
 #pragma unroll
@@ -1844,21 +2112,74 @@ __global__ __launch_bounds__(256) void k_final_outputs(const cplx<R> *__restrict
     }
 }
 
-__global__ void k_zero_u32(uint32_t *p, int64_t n, const int32_t *__restrict__ gate) {
+template <typename R>
+struct OutputsArgs {
+    const cplx<R> * phnm;
+    const R * bp;
+    int rowinv;
+    const cplx<R> * filt;
+    const cplx<R> * grid;
+    const int32_t * cell_of;
+    const int64_t * order;
+    FinalIn in;
+    FinalOut o;
+    int64_t P;
+    int I;
+    int64_t C;
+};
+
+template <bool M, typename R, int TP, int TF, int NT>
+__global__ __launch_bounds__(NT) void k_final_outputs(KArg<M, OutputsArgs<R>> p) {
+    if constexpr (M) {
+        OutputsArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        final_outputs_body<R, TP, TF, NT>(a.phnm, a.bp, a.rowinv, a.filt, a.grid, a.cell_of, a.order, a.in, a.o, a.P, a.I, a.C, bid_, nblk_);
+    } else {
+        final_outputs_body<R, TP, TF, NT>(p.phnm, p.bp, p.rowinv, p.filt, p.grid, p.cell_of, p.order, p.in, p.o, p.P, p.I, p.C, blockIdx.x, gridDim.x);
+    }
+}
+
+__device__ __forceinline__ void zero_u32_body(
+    uint32_t * p,
+    int64_t n,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (q < n) p[q] = 0u;
+}
+
+struct ZeroArgs {
+    uint32_t * p;
+    int64_t n;
+    const int32_t * gate;
+};
+
+template <bool M>
+__global__ void k_zero_u32(KArg<M, ZeroArgs> p) {
+    if constexpr (M) {
+        ZeroArgs a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        zero_u32_body(a.p, a.n, a.gate, bid_, nblk_);
+    } else {
+        zero_u32_body(p.p, p.n, p.gate, blockIdx.x, gridDim.x);
+    }
 }
 
 // psquare weighting lookup (native_stage2.rs:1674-1682):
 // w = (1 - prand_hi[round_half_away(1000 coh)])^2
-__global__ __launch_bounds__(256) void k_weights_psquare(const double *__restrict__ coh,
-                                                         const double *__restrict__ prand_hi,
-                                                         int nhi, double *__restrict__ W,
-                                                         int64_t P,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void weights_psquare_body(
+    const double *__restrict__ coh,
+    const double *__restrict__ prand_hi,
+    int nhi,
+    double *__restrict__ W,
+    int64_t P,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= P) return;
     const double v = coh[k] * 1000.0;
     double r = v >= 0.0 ? floor(v + 0.5) : ceil(v - 0.5);
@@ -1870,20 +2191,46 @@ __global__ __launch_bounds__(256) void k_weights_psquare(const double *__restric
     W[k] = (1.0 - p) * (1.0 - p);
 }
 
+struct PsquareArgs {
+    const double * coh;
+    const double * prand_hi;
+    int nhi;
+    double * W;
+    int64_t P;
+    const int32_t * gate;
+};
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_weights_psquare(KArg<M, PsquareArgs> p) {
+    if constexpr (M) {
+        PsquareArgs a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        weights_psquare_body(a.coh, a.prand_hi, a.nhi, a.W, a.P, a.gate, bid_, nblk_);
+    } else {
+        weights_psquare_body(p.coh, p.prand_hi, p.nhi, p.W, p.P, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
 // snr_weighting (native_stage2.rs:1875-1897) from the current residuals.
 template <typename R>
-__global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__ phnm,
-                                                     const float *__restrict__ amp,
-                                                     const R *__restrict__ bp,
-                                                     const cplx<R> *__restrict__ filt,
-                                                     const int32_t *__restrict__ cell_of,
-                                                     const double *__restrict__ K,
-                                                     const double *__restrict__ nopt,
-                                                     double *__restrict__ W, int64_t P, int I,
-                                                     int rowinv, int64_t C,
-    const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void weights_snr_body(
+    const cplx<R> *__restrict__ phnm,
+    const float *__restrict__ amp,
+    const R *__restrict__ bp,
+    const cplx<R> *__restrict__ filt,
+    const int32_t *__restrict__ cell_of,
+    const double *__restrict__ K,
+    const double *__restrict__ nopt,
+    double *__restrict__ W,
+    int64_t P,
+    int I,
+    int rowinv,
+    int64_t C,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= P) return;
     double g = 0.0, a2 = 0.0;
     const bool valid = nopt[k] == 1.0;
@@ -1907,6 +2254,35 @@ __global__ __launch_bounds__(256) void k_weights_snr(const cplx<R> *__restrict__
     W[k] = sigma != 0.0 ? g / sigma : 0.0;
 }
 
+template <typename R>
+struct SnrArgs {
+    const cplx<R> * phnm;
+    const float * amp;
+    const R * bp;
+    const cplx<R> * filt;
+    const int32_t * cell_of;
+    const double * K;
+    const double * nopt;
+    double * W;
+    int64_t P;
+    int I;
+    int rowinv;
+    int64_t C;
+    const int32_t * gate;
+};
+
+template <bool M, typename R>
+__global__ __launch_bounds__(256) void k_weights_snr(KArg<M, SnrArgs<R>> p) {
+    if constexpr (M) {
+        SnrArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        weights_snr_body<R>(a.phnm, a.amp, a.bp, a.filt, a.cell_of, a.K, a.nopt, a.W, a.P, a.I, a.rowinv, a.C, a.gate, bid_, nblk_);
+    } else {
+        weights_snr_body<R>(p.phnm, p.amp, p.bp, p.filt, p.cell_of, p.K, p.nopt, p.W, p.P, p.I, p.rowinv, p.C, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // Device-resident loop control.  One control kernel per iteration takes the
@@ -1928,6 +2304,52 @@ struct LoopCtl {
     double gcs;           // gamma_change_save
     double nr[CTL_BINS];  // Nr, scaled each P-square update
 };
+
+// The loop's starting state as a kernel (the merged launches of several
+// patches start with it; a single patch's loop sets the same state with
+// memsets and copies): K = 0, coh_prev = 0, W = W0, the first iteration's
+// histogram and re-score counters zero, and the control block with i_loop 1
+// and Nr = the base histogram nr0.  Not gated.
+struct PrepareArgs {
+    double *K, *cohprev, *W;
+    const double *W0;
+    uint32_t *hist, *counters;
+    LoopCtl *ctl;
+    const double *nr0;
+    int64_t P;
+};
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_loop_prepare(KArg<M, PrepareArgs> p) {
+    PrepareArgs a;
+    uint32_t bid, nblk;
+    if constexpr (M) {
+        if (!multi_locate(p, a, bid, nblk)) return;
+    } else {
+        a = p;
+        bid = blockIdx.x;
+        nblk = gridDim.x;
+    }
+    (void)nblk;
+    const int64_t k = (int64_t)bid * 256 + threadIdx.x;
+    if (k < a.P) {
+        a.K[k] = 0.0;
+        a.cohprev[k] = 0.0;
+        a.W[k] = a.W0[k];
+    }
+    if (bid == 0) {
+        for (int w = threadIdx.x; w < CTL_HIST_WORDS; w += 256) a.hist[w] = 0u;
+        if (threadIdx.x < 4) a.counters[threadIdx.x] = 0u;
+        for (int b = threadIdx.x; b < CTL_BINS; b += 256) a.ctl->nr[b] = a.nr0[b];
+        if (threadIdx.x == 0) {
+            a.ctl->stopped = 0;
+            a.ctl->i_loop = 1;
+            a.ctl->rescored = 0;
+            a.ctl->rechecked = 0;
+            a.ctl->gcs = 0.0;
+        }
+    }
+}
 
 struct CtlParams {
     int64_t P;
@@ -2100,15 +2522,22 @@ __device__ inline void control_step(LoopCtl *__restrict__ ctl, const double *__r
 // step 1 only.  counters: [0] re-score queue, [1] rows refined again;
 // control_step clears them for the next iteration.
 template <typename R>
-__global__ __launch_bounds__(256) void k_post(const cplx<R> *__restrict__ phnm,
-                                              const cplx<R> *__restrict__ filt,
-                                              const int32_t *__restrict__ cell_of,
-                                              const int32_t *__restrict__ tix,
-                                              double *__restrict__ Kout, double *__restrict__ Cout,
-                                              double *__restrict__ cohout, FitParams prm, RecheckIn rc,
-                                              const uint8_t *__restrict__ flags, RmsHistOut ro, int loop,
-                                              uint32_t *__restrict__ counters,
-                                              const int32_t *__restrict__ gate) {
+__device__ __forceinline__ void post_body(
+    const cplx<R> *__restrict__ phnm,
+    const cplx<R> *__restrict__ filt,
+    const int32_t *__restrict__ cell_of,
+    const int32_t *__restrict__ tix,
+    double *__restrict__ Kout,
+    double *__restrict__ Cout,
+    double *__restrict__ cohout,
+    FitParams prm,
+    RecheckIn rc,
+    const uint8_t *__restrict__ flags,
+    RmsHistOut ro,
+    int loop,
+    uint32_t *__restrict__ counters,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
     // groups of 256 rows, walked grid-stride: one RMS partial per group (the
     // same fixed tree whatever the grid), the block's histogram counts in LDS
@@ -2121,7 +2550,7 @@ __global__ __launch_bounds__(256) void k_post(const cplx<R> *__restrict__ phnm,
     if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
     uint32_t n_rechecked = 0;
     const int64_t n_groups = (prm.P + 255) / 256;
-    for (int64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+    for (int64_t g = bid_; g < n_groups; g += nblk_) {
         const int64_t k = g * 256 + threadIdx.x;
         const bool row = k < prm.P;
         double coh = row ? cohout[k] : 0.0;
@@ -2171,10 +2600,66 @@ __global__ __launch_bounds__(256) void k_post(const cplx<R> *__restrict__ phnm,
         if (lh[q]) atomicAdd(&ro.hist[q], lh[q]);
 }
 
-__global__ __launch_bounds__(256) void k_control(LoopCtl *__restrict__ ctl, const double *__restrict__ rms_part,
-                                                 uint32_t *__restrict__ hist, uint32_t *__restrict__ counters,
-                                                 double *__restrict__ prand_hi, CtlParams cp) {
+template <typename R>
+struct PostArgs {
+    const cplx<R> * phnm;
+    const cplx<R> * filt;
+    const int32_t * cell_of;
+    const int32_t * tix;
+    double * Kout;
+    double * Cout;
+    double * cohout;
+    FitParams prm;
+    RecheckIn rc;
+    const uint8_t * flags;
+    RmsHistOut ro;
+    int loop;
+    uint32_t * counters;
+    const int32_t * gate;
+};
+
+template <bool M, typename R>
+__global__ __launch_bounds__(256) void k_post(KArg<M, PostArgs<R>> p) {
+    if constexpr (M) {
+        PostArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        post_body<R>(a.phnm, a.filt, a.cell_of, a.tix, a.Kout, a.Cout, a.cohout, a.prm, a.rc, a.flags, a.ro, a.loop, a.counters, a.gate, bid_, nblk_);
+    } else {
+        post_body<R>(p.phnm, p.filt, p.cell_of, p.tix, p.Kout, p.Cout, p.cohout, p.prm, p.rc, p.flags, p.ro, p.loop, p.counters, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
+__device__ __forceinline__ void control_body(
+    LoopCtl *__restrict__ ctl,
+    const double *__restrict__ rms_part,
+    uint32_t *__restrict__ hist,
+    uint32_t *__restrict__ counters,
+    double *__restrict__ prand_hi,
+    CtlParams cp,
+    const uint32_t bid_, const uint32_t nblk_) {
     control_step(ctl, rms_part, hist, counters, prand_hi, cp);
+}
+
+struct ControlArgs {
+    LoopCtl * ctl;
+    const double * rms_part;
+    uint32_t * hist;
+    uint32_t * counters;
+    double * prand_hi;
+    CtlParams cp;
+};
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_control(KArg<M, ControlArgs> p) {
+    if constexpr (M) {
+        ControlArgs a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        control_body(a.ctl, a.rms_part, a.hist, a.counters, a.prand_hi, a.cp, bid_, nblk_);
+    } else {
+        control_body(p.ctl, p.rms_part, p.hist, p.counters, p.prand_hi, p.cp, blockIdx.x, gridDim.x);
+    }
 }
 
 }  // namespace ps2

@@ -16,6 +16,7 @@
 #include <cerrno>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -271,6 +272,30 @@ struct ClapSegs {
 
 constexpr int CLAP_TEAMS = 2;   // teams (interferograms) per CLAP wavefront
 
+// The loop kernels of one gamma iteration in launch order, then the outputs.
+// A recorded launch (below) carries its stage, so that the launches of
+// several patches can be merged stage by stage.
+enum Stage {
+    ST_PREPARE, ST_FILL, ST_CLAP_A, ST_CLAP_B, ST_SCAN, ST_RESCORE, ST_REFINE, ST_POST, ST_CONTROL,
+    ST_WEIGHTS, ST_OUTPUTS, ST_COUNT
+};
+
+// A loop-kernel launch kept for later: the kernel (its multi-patch
+// instantiation), grid and workgroup size, the argument struct's bytes and
+// the layout of that kernel's MultiPack (its size and where the ranges and
+// the count sit; the structs are at multiples of the struct size).
+struct LaunchRec {
+    const void *fn;
+    uint32_t blocks, threads;
+    int stage;
+    std::vector<char> args;
+    size_t pack_size, range_off, n_off;
+};
+
+struct ps2_session;
+// drops the merged-launch graphs that include session s (its destructor)
+void batch_forget(const ps2_session *s);
+
 struct ps2_session {
     std::mutex mu;
     int form = PS2_FORM_FP32;
@@ -310,9 +335,12 @@ struct ps2_session {
     int clap_slots = 1;     // CLAP strip wavefronts resident on the device
     // device-resident loop: control block, gate of the loop kernels (set
     // while a loop is being launched), one iteration captured as a graph
-    DBuf ctl;
+    DBuf ctl, nr0;   // control block; the base Nr histogram (its starting Nr)
     HostPinned h_ctl;
     const int32_t *gate = nullptr;
+    // while set, the launch functions instantiated for several patches record
+    // their launches here instead of launching them
+    std::vector<LaunchRec> *rec = nullptr;
     CtlParams ctl_params{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -333,6 +361,7 @@ struct ps2_session {
     int64_t n_rescored = 0, n_rechecked = 0;
 
     ~ps2_session() {
+        batch_forget(this);
         if (stream) {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
@@ -348,6 +377,21 @@ struct ps2_session {
     }
     size_t rsz() const { return form == PS2_FORM_FP64 ? sizeof(double) : sizeof(float); }
 };
+
+// Launch of a loop kernel k with argument struct a: for one patch (M false)
+// on the session's stream now; for several (M true) recorded in s->rec.
+template <bool M, typename A>
+void kgo(ps2_session *s, void (*k)(KArg<M, A>), dim3 g, dim3 b, const A &a, int stage) {
+    if constexpr (M) {
+        static_assert(sizeof(MultiPack<A>) <= 4096, "kernel argument pack too large");
+        LaunchRec r{reinterpret_cast<const void *>(k), g.x, b.x, stage, {}, sizeof(MultiPack<A>),
+                    offsetof(MultiPack<A>, range), offsetof(MultiPack<A>, n)};
+        r.args.assign(reinterpret_cast<const char *>(&a), reinterpret_cast<const char *>(&a) + sizeof(A));
+        s->rec->push_back(std::move(r));
+    } else {
+        hipLaunchKernelGGL(k, g, b, 0, s->stream, a);
+    }
+}
 
 // FP32 trial scan: unit steps e^{-i phi} from a per-(pixel, ifg) table, or
 // computed in the scan from the pixel's bperp.  Computing them is the better
@@ -549,7 +593,7 @@ int alloc_state(ps2_session *s) {
         const int n_cu = s->n_cu;
         int per_cu = 0;
         PS2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_clap_strip<R, CLAP_TEAMS, true>), 32 * CLAP_TEAMS, 0));
+            &per_cu, reinterpret_cast<const void *>(&k_clap_strip<false, R, CLAP_TEAMS, true>), 32 * CLAP_TEAMS, 0));
         s->clap_slots = std::max(1, n_cu) * std::max(1, per_cu);
         s->segs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS), (int)I, s->clap_slots);
         // the autotuner tries segmentations with up to 4 planes
@@ -582,6 +626,10 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->prand_hi.alloc(sizeof(double) * 2048))) return rc;
     if ((rc = s->ctl.alloc(sizeof(LoopCtl)))) return rc;
     if ((rc = s->h_ctl.alloc(sizeof(LoopCtl)))) return rc;
+    if ((rc = s->nr0.alloc(sizeof(double) * CTL_BINS))) return rc;
+    PS2_HIP(hipMemcpyAsync(s->nr0.p, s->nr_base.data(), sizeof(double) * CTL_BINS, hipMemcpyHostToDevice,
+                           s->stream));
+    PS2_HIP(hipStreamSynchronize(s->stream));
     s->n_rms_blocks = (int)blocks_for(P, 256);   // one partial per 256-row group of k_post
     if ((rc = s->rms_part.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
     {   // control-kernel constants: gausswin(7, 2.5) taps and their sum (native_stage2.rs:1685-1699)
@@ -649,7 +697,7 @@ void collect_timers(ps2_session *s) {
 // --- kernel launch helpers ---------------------------------------------------
 
 
-template <typename R>
+template <typename R, bool M = false>
 int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *Kprev = nullptr,
                      double *Wprev = nullptr) {
     const int64_t n = s->C * s->I;
@@ -671,27 +719,26 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *K
             const int64_t ncg = blocks_for(s->C, CB);
             const int npg = (int)blocks_for(s->I, 256 / CB);
             const dim3 grid((unsigned)(blocks_for(ncg, PS2_NUM_XCD) * PS2_NUM_XCD * npg));
-            auto go = [&](auto kern) {
-                hipLaunchKernelGGL(kern, grid, dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
-                                   s->rowinv, K, W, s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C,
-                                   (int)s->I, s->P, npg, Kprev, Wprev, s->gate);
-            };
+            const FillStagedArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
+                                      s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I,
+                                      s->P, npg, Kprev, Wprev, s->gate};
+            auto go = [&](auto kern) { kgo<M>(s, kern, grid, dim3(256), a, ST_FILL); };
             if (mean <= 256.0)
-                go(k_fill_grid_staged<R, CB, 256>);
+                go(k_fill_grid_staged<M, R, CB, 256>);
             else if (mean <= 512.0)
-                go(k_fill_grid_staged<R, CB, 512>);
+                go(k_fill_grid_staged<M, R, CB, 512>);
             else
-                go(k_fill_grid_staged<R, CB, 1024>);
+                go(k_fill_grid_staged<M, R, CB, 1024>);
         };
         if (s->fill_cb == 128)
             launch(std::integral_constant<int, 128>{});
         else
             launch(std::integral_constant<int, 64>{});
     } else {
-        hipLaunchKernelGGL(k_fill_grid<R>, dim3(blocks_for(n, 256)), dim3(256), 0, s->stream,
-                           s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
-                           s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P,
-                           Kprev, Wprev, s->gate);
+        const FillArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
+                            s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P,
+                            Kprev, Wprev, s->gate};
+        kgo<M>(s, k_fill_grid<M, R>, dim3(blocks_for(n, 256)), dim3(256), a, ST_FILL);
     }
     PS2_LAUNCHED();
     return PS2_OK;
@@ -723,13 +770,13 @@ ClapParams<R> clap_params(ps2_session *s, int teams) {
     return p;
 }
 
-template <typename R>
+template <typename R, bool M = false>
 int launch_clap(ps2_session *s, bool unit) {
     const int64_t n = s->C * s->I;
     if (!s->windows) {   // empty CLAP: zero filtered grid (a kernel: this may be graph-captured)
         const int64_t words = (int64_t)(sizeof(cplx<R>) / 4) * n;
-        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(words, 256)), dim3(256), 0, s->stream,
-                           s->filt.as<uint32_t>(), words, s->gate);
+        kgo<M>(s, k_zero_u32<M>, dim3(blocks_for(words, 256)), dim3(256),
+               ZeroArgs{s->filt.as<uint32_t>(), words, s->gate}, ST_CLAP_A);
         PS2_LAUNCHED();
         return PS2_OK;
     }
@@ -738,25 +785,24 @@ int launch_clap(ps2_session *s, bool unit) {
     {
         SlotTimer tm(s, SLOT_CLAP_STRIP);
         const dim3 g((unsigned)s->nwi * prm.upx * PS2_NUM_XCD);
+        const StripArgs<R> a{s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm, s->gate};
         if (prm.alpha_is_one)
-            hipLaunchKernelGGL((k_clap_strip<R, TEAMS, true>), g, dim3(32 * TEAMS), 0, s->stream,
-                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm, s->gate);
+            kgo<M>(s, k_clap_strip<M, R, TEAMS, true>, g, dim3(32 * TEAMS), a, ST_CLAP_A);
         else
-            hipLaunchKernelGGL((k_clap_strip<R, TEAMS, false>), g, dim3(32 * TEAMS), 0, s->stream,
-                               s->grid.as<cplx<R>>(), s->part.as<cplx<R>>(), s->lp.as<R>(), prm, s->gate);
+            kgo<M>(s, k_clap_strip<M, R, TEAMS, false>, g, dim3(32 * TEAMS), a, ST_CLAP_A);
         PS2_LAUNCHED();
     }
     SlotTimer tm(s, SLOT_CLAP_COMBINE);
+    const CombineArgs<R> ca{s->part.as<cplx<R>>(), s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj, (int)s->I,
+                            s->nwi, prm.nseg, prm.wseg, prm.nplanes, s->gate};
     auto combine = [&](auto kern) {
         const dim3 g(blocks_for(s->ni, COMB_TR) * blocks_for(s->nj, COMB_TC) * (unsigned)s->I);
-        hipLaunchKernelGGL(kern, g, dim3(256), 0, s->stream, s->part.as<cplx<R>>(),
-                           s->filt.as<cplx<R>>(), (int)s->ni, (int)s->nj, (int)s->I, s->nwi, prm.nseg,
-                           prm.wseg, prm.nplanes, s->gate);
+        kgo<M>(s, kern, g, dim3(256), ca, ST_CLAP_B);
     };
     if (prm.nplanes <= 2)
-        unit ? combine(k_clap_combine<R, true, 2>) : combine(k_clap_combine<R, false, 2>);
+        unit ? combine(k_clap_combine<M, R, true, 2>) : combine(k_clap_combine<M, R, false, 2>);
     else
-        unit ? combine(k_clap_combine<R, true, 4>) : combine(k_clap_combine<R, false, 4>);
+        unit ? combine(k_clap_combine<M, R, true, 4>) : combine(k_clap_combine<M, R, false, 4>);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -784,23 +830,22 @@ int scan_chunk(const ps2_session *s) {
     return 48;
 }
 
-template <typename R>
+template <typename R, bool M = false>
 int launch_fit(ps2_session *s, bool in_loop = false) {
     const FitParams fp = fit_params(s);
     const unsigned nb = blocks_for(s->P, 256);
     if (sizeof(R) == 4 && !in_loop) {   // (inside the loop the control step clears it)
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->stream, s->rescore_count.as<uint32_t>(), 4,
-                           s->gate);
+        kgo<M>(s, k_zero_u32<M>, dim3(1), dim3(64), ZeroArgs{s->rescore_count.as<uint32_t>(), 4, s->gate},
+               ST_SCAN);
         PS2_LAUNCHED();
     }
     {
     SlotTimer tm(s, SLOT_SCAN);
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(),
-                           s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(),
-                           s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(), fp, s->gate);
-    };
+    const ScanArgs<R> sa{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(),
+                         s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
+                         s->offset.as<cplx<R>>(), s->rescore_list.as<int32_t>(),
+                         s->rescore_count.as<uint32_t>(), fp, s->gate};
+    auto go = [&](auto kern) { kgo<M>(s, kern, dim3(nb), dim3(256), sa, ST_SCAN); };
     // trials per chunk: the FP32 form scans all trials in one pass up to 48
     // positive trials (larger grids take 48-trial chunks)
     const int tc = scan_chunk(s);
@@ -810,30 +855,30 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         if (tc == 8) {
             // the trial count as a compile-time constant (no per-trial test)
             switch (s->tn) {
-            case 1: go(k_scan<R, 8, true, 1>); break;
-            case 2: go(k_scan<R, 8, true, 2>); break;
-            case 3: go(k_scan<R, 8, true, 3>); break;
-            case 4: go(k_scan<R, 8, true, 4>); break;
-            case 5: go(k_scan<R, 8, true, 5>); break;
-            case 6: go(k_scan<R, 8, true, 6>); break;
-            case 7: go(k_scan<R, 8, true, 7>); break;
-            default: go(k_scan<R, 8, true>); break;
+            case 1: go(k_scan<M, R, 8, true, 1>); break;
+            case 2: go(k_scan<M, R, 8, true, 2>); break;
+            case 3: go(k_scan<M, R, 8, true, 3>); break;
+            case 4: go(k_scan<M, R, 8, true, 4>); break;
+            case 5: go(k_scan<M, R, 8, true, 5>); break;
+            case 6: go(k_scan<M, R, 8, true, 6>); break;
+            case 7: go(k_scan<M, R, 8, true, 7>); break;
+            default: go(k_scan<M, R, 8, true>); break;
             }
         } else if (tc == 16)
-            go(k_scan<R, 16, true>);
+            go(k_scan<M, R, 16, true>);
         else if (tc == 24)
-            go(k_scan<R, 24, true>);
+            go(k_scan<M, R, 24, true>);
         else if (tc == 32)
-            go(k_scan<R, 32, true>);
+            go(k_scan<M, R, 32, true>);
         else if (s->tn < 48)
-            go(k_scan<R, 48, true>);
+            go(k_scan<M, R, 48, true>);
         else
-            go(k_scan<R, 48>);
+            go(k_scan<M, R, 48>);
     } else {
         if (tc == 8)
-            go(k_scan<R, 8>);
+            go(k_scan<M, R, 8>);
         else
-            go(k_scan<R, 16>);
+            go(k_scan<M, R, 16>);
     }
     PS2_LAUNCHED();
     }
@@ -842,10 +887,11 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         // one wavefront per queued row (grid-stride); the queue holds ~0.2-1% of
         // the rows, and launching many more workgroups than that costs more
         // than it saves (16,384 of them: 12 us at 20,000 rows for ~20 rows)
-        hipLaunchKernelGGL(k_rescore<R>, dim3(std::min<int64_t>(s->P, 4096)), dim3(64), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp64.as<double>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->rescore_list.as<int32_t>(), s->rescore_count.as<uint32_t>(),
-                           s->tix.as<int32_t>(), s->offset.as<cplx<R>>(), fp, s->gate);
+        const RescoreArgs<R> a{s->phnm.as<cplx<R>>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
+                               s->cell_of.as<int32_t>(), s->rescore_list.as<int32_t>(),
+                               s->rescore_count.as<uint32_t>(), s->tix.as<int32_t>(), s->offset.as<cplx<R>>(),
+                               fp, s->gate};
+        kgo<M>(s, k_rescore<M, R>, dim3((unsigned)std::min<int64_t>(s->P, 4096)), dim3(64), a, ST_RESCORE);
         PS2_LAUNCHED();
     }
     // the refine kernel marks the rows whose discrete choices lie within its
@@ -870,21 +916,22 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     rc.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
     {
         SlotTimer tm(s, SLOT_REFINE);
-        hipLaunchKernelGGL(k_refine<R>, dim3(nb), dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(),
-                           s->bp.as<R>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->tix.as<int32_t>(), s->rng.as<R>(), s->offset.as<cplx<R>>(), s->K.as<double>(),
-                           s->Cc.as<double>(), s->coh.as<double>(), s->nopt.as<double>(), fp, ro.mids, ro.hp, rc,
-                           s->flags.as<uint8_t>(), s->gate);
+        const RefineArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
+                              s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
+                              s->offset.as<cplx<R>>(), s->K.as<double>(), s->Cc.as<double>(),
+                              s->coh.as<double>(), s->nopt.as<double>(), fp, ro.mids, ro.hp, rc,
+                              s->flags.as<uint8_t>(), s->gate};
+        kgo<M>(s, k_refine<M, R>, dim3(nb), dim3(256), a, ST_REFINE);
         PS2_LAUNCHED();
     }
     if (!in_loop && !rc.enabled) return PS2_OK;
     SlotTimer tm(s, SLOT_POST);
     // grid-stride over the 256-row groups: 2 workgroups per CU at most
-    hipLaunchKernelGGL(k_post<R>, dim3(std::min<unsigned>(nb, 2 * (unsigned)s->n_cu)), dim3(256), 0, s->stream,
-                       s->phnm.as<cplx<R>>(),
-                       s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->K.as<double>(),
-                       s->Cc.as<double>(), s->coh.as<double>(), fp, rc, s->flags.as<uint8_t>(), ro,
-                       in_loop ? 1 : 0, s->rescore_count.as<uint32_t>(), s->gate);
+    const PostArgs<R> pa{s->phnm.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
+                         s->tix.as<int32_t>(), s->K.as<double>(), s->Cc.as<double>(), s->coh.as<double>(), fp,
+                         rc, s->flags.as<uint8_t>(), ro, in_loop ? 1 : 0, s->rescore_count.as<uint32_t>(),
+                         s->gate};
+    kgo<M>(s, k_post<M, R>, dim3(std::min<unsigned>(nb, 2 * (unsigned)s->n_cu)), dim3(256), pa, ST_POST);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -892,35 +939,37 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
 // One gamma iteration (native_stage2.rs:261-390): every launch is gated by
 // the control block, so a whole loop can be launched without the host seeing
 // the stop decision.
-template <typename R>
+template <typename R, bool M = false>
 int launch_iteration(ps2_session *s) {
     const int64_t P = s->P;
     int rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
     // the fill also keeps the K and weighting it used (for the ph_weight output)
-    if ((rc = launch_fill_grid<R>(s, s->K.as<double>(), s->W.as<double>(), s->Kprev.as<double>(),
-                                  s->Wprev.as<double>())))
+    if ((rc = launch_fill_grid<R, M>(s, s->K.as<double>(), s->W.as<double>(), s->Kprev.as<double>(),
+                                     s->Wprev.as<double>())))
         return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
-    if ((rc = launch_clap<R>(s, true))) return rc;
+    if ((rc = launch_clap<R, M>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[4], s->stream));
-    if ((rc = launch_fit<R>(s, true))) return rc;
+    if ((rc = launch_fit<R, M>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
     // gamma-change RMS, stop test, Nr scaling and the P-square table
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s->stream, s->ctl.as<LoopCtl>(),
-                       s->rms_part.as<double>(), s->hist.as<uint32_t>(), s->rescore_count.as<uint32_t>(),
-                       s->prand_hi.as<double>(), s->ctl_params);
+    kgo<M>(s, k_control<M>, dim3(1), dim3(256),
+           ControlArgs{s->ctl.as<LoopCtl>(), s->rms_part.as<double>(), s->hist.as<uint32_t>(),
+                       s->rescore_count.as<uint32_t>(), s->prand_hi.as<double>(), s->ctl_params},
+           ST_CONTROL);
     PS2_LAUNCHED();
     SlotTimer tm(s, SLOT_WEIGHTS);
     if (s->weighting == PS2_WEIGHT_PSQUARE)
-        hipLaunchKernelGGL(k_weights_psquare, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
-                           s->coh.as<double>(), s->prand_hi.as<double>(), CTL_HI, s->W.as<double>(), P,
-                           s->gate);
+        kgo<M>(s, k_weights_psquare<M>, dim3(blocks_for(P, 256)), dim3(256),
+               PsquareArgs{s->coh.as<double>(), s->prand_hi.as<double>(), CTL_HI, s->W.as<double>(), P, s->gate},
+               ST_WEIGHTS);
     else
-        hipLaunchKernelGGL(k_weights_snr<R>, dim3(blocks_for(P, 256)), dim3(256), 0, s->stream,
-                           s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
-                           s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(),
-                           s->W.as<double>(), P, (int)s->I, s->rowinv, s->C, s->gate);
+        kgo<M>(s, k_weights_snr<M, R>, dim3(blocks_for(P, 256)), dim3(256),
+               SnrArgs<R>{s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
+                          s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(), s->W.as<double>(),
+                          P, (int)s->I, s->rowinv, s->C, s->gate},
+               ST_WEIGHTS);
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -928,7 +977,7 @@ int launch_iteration(ps2_session *s) {
 // The outputs in the caller's layouts (k_final_outputs).  loop: every
 // write_pm1 argument of the loop; otherwise (the fit step) ph_patch, ph_res,
 // K/C/coh/N_opt and the trial index.
-template <typename R>
+template <typename R, bool M = false>
 int launch_outputs(ps2_session *s, bool loop) {
     const int64_t P = s->P, I = s->I, C = s->C;
     SlotTimer tm(s, SLOT_OUTPUTS);
@@ -948,24 +997,31 @@ int launch_outputs(ps2_session *s, bool loop) {
     fo.vec = s->o_vec.as<double>();
     fo.tix = s->o_tix.as<int32_t>();
     fo.grid = loop ? s->o_grid.as<float2>() : nullptr;
-    auto go = [&](auto tp_tag, auto tf_tag) {
+    auto go = [&](auto tp_tag, auto tf_tag, auto nt_tag) {
         constexpr int TP = decltype(tp_tag)::value, TF = decltype(tf_tag)::value;
+        constexpr int NT = decltype(nt_tag)::value;
         fo.n_ifg_tiles = (int)blocks_for(I, TF);
         fo.n_pix_blocks = (int64_t)blocks_for(P, TP) * fo.n_ifg_tiles;
         fo.n_grid_ifg_tiles = (int)blocks_for(I, 32);
         const int64_t n_grid_blocks = loop ? (int64_t)blocks_for(C, 32) * fo.n_grid_ifg_tiles : 0;
-        hipLaunchKernelGGL((k_final_outputs<R, TP, TF>), dim3((unsigned)(fo.n_pix_blocks + n_grid_blocks)),
-                           dim3(256), 0, s->stream, s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv,
-                           s->filt.as<cplx<R>>(), s->grid.as<cplx<R>>(), s->cell_of.as<int32_t>(),
-                           s->d_order.as<int64_t>(), fi, fo, P, (int)I, C);
+        const OutputsArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, s->filt.as<cplx<R>>(),
+                               s->grid.as<cplx<R>>(), s->cell_of.as<int32_t>(), s->d_order.as<int64_t>(), fi,
+                               fo, P, (int)I, C};
+        kgo<M>(s, k_final_outputs<M, R, TP, TF, NT>, dim3((unsigned)(fo.n_pix_blocks + n_grid_blocks)), dim3(NT),
+               a, ST_OUTPUTS);
     };
-    // the tile spans the whole row where it can (at most 256 interferograms)
+    // the tile spans the whole row where it can (at most 256 interferograms).
+    // The stores are row segments at the caller's row positions: for pixels
+    // listed in random spatial order these are scattered 0.25-1 KB writes,
+    // which bound the kernel (wider pixel tiles or 512/1024 threads measured
+    // no faster at 500k x 60, 4M x 120 or 1M x 200).
+    using std::integral_constant;
     if (I <= 64)
-        go(std::integral_constant<int, 32>{}, std::integral_constant<int, 64>{});
+        go(integral_constant<int, 32>{}, integral_constant<int, 64>{}, integral_constant<int, 256>{});
     else if (I <= 128)
-        go(std::integral_constant<int, 16>{}, std::integral_constant<int, 128>{});
+        go(integral_constant<int, 16>{}, integral_constant<int, 128>{}, integral_constant<int, 256>{});
     else
-        go(std::integral_constant<int, 16>{}, std::integral_constant<int, 256>{});
+        go(integral_constant<int, 16>{}, integral_constant<int, 256>{}, integral_constant<int, 256>{});
     PS2_LAUNCHED();
     return PS2_OK;
 }
@@ -1141,23 +1197,54 @@ int autotune_tiles(ps2_session *s, const char *path) {
 // are launched one by one and the host reads the control block after each.
 constexpr int64_t LOOP_GRAPH_MAX = 64;
 
-template <typename R>
-int run_loop(ps2_session *s) {
+// The loop's starting state, on stream st (not captured): K = 0,
+// coh_prev = 0, the first iteration's histogram and re-score counter (k_control
+// clears them for every later one), the initial weighting, the control block.
+int loop_prepare(ps2_session *s, hipStream_t st) {
     const int64_t P = s->P;
-    int rc;
     LoopCtl *hc = s->h_ctl.as<LoopCtl>();
-    PS2_HIP(hipEventRecord(s->ev[0], s->stream));
-    PS2_HIP(hipMemsetAsync(s->K.p, 0, sizeof(double) * P, s->stream));
-    PS2_HIP(hipMemsetAsync(s->cohprev.p, 0, sizeof(double) * P, s->stream));
-    // the first iteration's histogram and re-score counter (k_control clears
-    // them for every later one)
-    PS2_HIP(hipMemsetAsync(s->hist.p, 0, sizeof(uint32_t) * CTL_HIST_WORDS, s->stream));
-    PS2_HIP(hipMemsetAsync(s->rescore_count.p, 0, sizeof(uint32_t) * 4, s->stream));
-    PS2_HIP(hipMemcpyAsync(s->W.p, s->W0.p, sizeof(double) * P, hipMemcpyDeviceToDevice, s->stream));
+    PS2_HIP(hipMemsetAsync(s->K.p, 0, sizeof(double) * P, st));
+    PS2_HIP(hipMemsetAsync(s->cohprev.p, 0, sizeof(double) * P, st));
+    PS2_HIP(hipMemsetAsync(s->hist.p, 0, sizeof(uint32_t) * CTL_HIST_WORDS, st));
+    PS2_HIP(hipMemsetAsync(s->rescore_count.p, 0, sizeof(uint32_t) * 4, st));
+    PS2_HIP(hipMemcpyAsync(s->W.p, s->W0.p, sizeof(double) * P, hipMemcpyDeviceToDevice, st));
     std::memset(hc, 0, sizeof(LoopCtl));
     hc->i_loop = 1;
     for (int b = 0; b < CTL_BINS; ++b) hc->nr[b] = s->nr_base[b];
-    PS2_HIP(hipMemcpyAsync(s->ctl.p, hc, sizeof(LoopCtl), hipMemcpyHostToDevice, s->stream));
+    PS2_HIP(hipMemcpyAsync(s->ctl.p, hc, sizeof(LoopCtl), hipMemcpyHostToDevice, st));
+    return PS2_OK;
+}
+
+// The same starting state as one kernel (recorded for the merged launches).
+template <bool M>
+int launch_prepare(ps2_session *s) {
+    const PrepareArgs a{s->K.as<double>(), s->cohprev.as<double>(), s->W.as<double>(), s->W0.as<double>(),
+                        s->hist.as<uint32_t>(), s->rescore_count.as<uint32_t>(), s->ctl.as<LoopCtl>(),
+                        s->nr0.as<double>(), s->P};
+    kgo<M>(s, k_loop_prepare<M>, dim3(blocks_for(s->P, 256)), dim3(256), a, ST_PREPARE);
+    PS2_LAUNCHED();
+    return PS2_OK;
+}
+
+// The loop's results from the control block read back into h_ctl.
+void loop_finish(ps2_session *s, double seconds) {
+    const LoopCtl *hc = s->h_ctl.as<LoopCtl>();
+    s->loop_seconds = seconds;
+    s->i_loop = hc->i_loop;
+    s->gamma_change_save = hc->gcs;
+    s->nr.assign(hc->nr, hc->nr + CTL_BINS);
+    s->n_rescored = (int64_t)hc->rescored;
+    s->n_rechecked = (int64_t)hc->rechecked;
+    s->ran = true;
+    s->loop_outputs = true;
+}
+
+template <typename R>
+int run_loop(ps2_session *s) {
+    int rc;
+    LoopCtl *hc = s->h_ctl.as<LoopCtl>();
+    PS2_HIP(hipEventRecord(s->ev[0], s->stream));
+    if ((rc = loop_prepare(s, s->stream))) return rc;
     s->gate = &s->ctl.as<LoopCtl>()->stopped;
     const int64_t max_iter = s->ctl_params.max_iter;
     bool stepwise = s->timing || s->verbose || s->no_graph;
@@ -1233,14 +1320,7 @@ int run_loop(ps2_session *s) {
     collect_timers(s);
     float ms = 0.0f;
     PS2_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
-    s->loop_seconds = (double)ms * 1e-3;
-    s->i_loop = hc->i_loop;
-    s->gamma_change_save = hc->gcs;
-    s->nr.assign(hc->nr, hc->nr + CTL_BINS);
-    s->n_rescored = (int64_t)hc->rescored;
-    s->n_rechecked = (int64_t)hc->rechecked;
-    s->ran = true;
-    s->loop_outputs = true;
+    loop_finish(s, (double)ms * 1e-3);
     return PS2_OK;
 }
 
@@ -1275,6 +1355,309 @@ int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs) {
     out->n_rescored = s->n_rescored;
     out->n_rechecked = s->n_rechecked;
     out->loop_seconds = s->loop_seconds;
+    return PS2_OK;
+}
+
+// --- several patches per launch ---------------------------------------------
+// ps2_hip_sessions_run runs the loops of several sessions of one device as
+// merged launches: every loop kernel is launched once per gamma iteration for
+// a chain of patches (MultiPack: each patch keeps its own workgroups,
+// arguments, gate and convergence; up to PS2_PACK patches per launch).  A
+// chain holds patches whose per-iteration data fit together in a part of the
+// device's 256 MB last-level cache (kChainBytes), so that each kernel of an
+// iteration finds the previous kernel's planes there; a larger patch is a
+// chain of its own.  Each chain is a sequence of merged launches on its own
+// stream, and the chains overlap (the memory-bound kernels of one with the
+// ALU-bound CLAP of another); all chains, all iterations and the outputs are
+// captured in one HIP graph (forked from and joined to the first session's
+// stream).  The launches are recorded from the same launch
+// functions as a single patch's (launch_iteration<R, true>), so every patch
+// runs exactly the kernels and arguments of its own loop: results are
+// bit-identical to the single runs.  The graph is kept for the next call with
+// the same sessions.
+
+// batch_build / sessions_run_merged: the sessions cannot run merged (the
+// caller runs them concurrently instead)
+constexpr int kNotMerged = 1;
+constexpr double kChainBytes = 128e6;
+
+struct MergedLaunch {
+    const void *fn;
+    uint32_t blocks, threads;
+    std::vector<char> pack;   // the kernel's MultiPack argument
+};
+
+struct Chain {
+    hipStream_t stream;
+    // the starting state, one iteration's launches in stage order, the outputs
+    std::vector<MergedLaunch> prepare, iter, outputs;
+};
+
+struct BatchGraph {
+    std::vector<ps2_session *> ss;
+    int device = 0;
+    std::vector<Chain> chains;
+    std::vector<hipEvent_t> events;   // fork, then one join per further chain
+    int64_t max_iter = 0, per_graph = 0;
+    bool outputs_in_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchGraph() {
+        (void)hipSetDevice(device);
+        AllocGuard g(capture_mutex());
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : events) (void)hipEventDestroy(e);
+    }
+};
+
+std::mutex g_batch_mu;
+// most recent last; a run holds its own reference while it uses one
+std::vector<std::shared_ptr<BatchGraph>> g_batches;
+constexpr size_t kBatchGraphsKept = 4;
+
+void batch_forget_impl(const ps2_session *s) {
+    std::vector<std::shared_ptr<BatchGraph>> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_batch_mu);
+        for (auto it = g_batches.begin(); it != g_batches.end();) {
+            if (std::find((*it)->ss.begin(), (*it)->ss.end(), s) != (*it)->ss.end()) {
+                drop.push_back(std::move(*it));
+                it = g_batches.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    // the graphs are released here, outside the list's lock
+}
+
+hipError_t launch_merged(const MergedLaunch &m, hipStream_t st) {
+    void *kargs[1] = {const_cast<char *>(m.pack.data())};
+    return hipLaunchKernel(m.fn, dim3(m.blocks), dim3(m.threads), kargs, 0, st);
+}
+
+// bytes one gamma iteration reads and writes per patch (sorted pixel planes,
+// bperp and unit steps; grid, filtered grid and CLAP partials)
+double loop_bytes(const ps2_session *s) {
+    const double r = (double)s->rsz();
+    return (double)s->I * ((double)s->P * (3.0 * r + 8.0) + (double)s->C * 6.0 * r);
+}
+
+// Records every session's iteration and outputs launches and merges them, per
+// chain, stage by stage: one launch per (stage, kernel) with the chain's
+// sessions in order, each session's workgroups starting at a multiple of 8.
+int batch_build(const std::vector<ps2_session *> &ss, BatchGraph &bg) {
+    const size_t n = ss.size();
+    std::vector<std::vector<LaunchRec>> recs(n);
+    int rc = PS2_OK;
+    for (size_t q = 0; q < n && !rc; ++q) {
+        ps2_session *s = ss[q];
+        s->gate = &s->ctl.as<LoopCtl>()->stopped;
+        s->rec = &recs[q];
+        rc = launch_prepare<true>(s);
+        if (!rc && s->form == PS2_FORM_FP64) {
+            rc = launch_iteration<double, true>(s);
+            if (!rc) rc = launch_outputs<double, true>(s, true);
+        } else if (!rc) {
+            rc = launch_iteration<float, true>(s);
+            if (!rc) rc = launch_outputs<float, true>(s, true);
+        }
+        s->rec = nullptr;
+        s->gate = nullptr;
+        int last = -1;
+        for (const auto &r : recs[q]) {
+            if (r.stage <= last) return kNotMerged;
+            last = r.stage;
+        }
+    }
+    if (rc) return rc;
+    // chains: first fit of the patches, largest first, into chains of at
+    // most kChainBytes per iteration (a larger patch alone)
+    std::vector<size_t> by_size(n);
+    for (size_t q = 0; q < n; ++q) by_size[q] = q;
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](size_t x, size_t y) { return loop_bytes(ss[x]) > loop_bytes(ss[y]); });
+    std::vector<double> load;
+    std::vector<int> chain_of(n, 0);
+    for (size_t q : by_size) {
+        size_t c = 0;
+        while (c < load.size() && load[c] + loop_bytes(ss[q]) > kChainBytes) ++c;
+        if (c == load.size()) load.push_back(0.0);
+        chain_of[q] = (int)c;
+        load[c] += loop_bytes(ss[q]);
+    }
+    const size_t nch = load.size();
+    // groups of one (chain, stage, kernel), sessions in order
+    struct Group {
+        int chain, stage;
+        const void *fn;
+        uint32_t threads;
+        std::vector<const LaunchRec *> r;
+    };
+    std::vector<Group> groups;
+    for (size_t c = 0; c < nch; ++c)
+        for (int st = 0; st < ST_COUNT; ++st)
+            for (size_t q = 0; q < n; ++q) {
+                if (chain_of[q] != (int)c) continue;
+                for (const auto &r : recs[q]) {
+                    if (r.stage != st || r.blocks == 0) continue;
+                    Group *g = nullptr;
+                    for (auto &x : groups)
+                        if (x.chain == (int)c && x.stage == st && x.fn == r.fn && x.r.size() < PS2_PACK) g = &x;
+                    if (!g) {
+                        groups.push_back(Group{(int)c, st, r.fn, r.threads, {}});
+                        g = &groups.back();
+                    }
+                    if (g->threads != r.threads) return kNotMerged;
+                    if (!g->r.empty() && g->r[0]->args.size() != r.args.size()) return kNotMerged;
+                    g->r.push_back(&r);
+                }
+            }
+    // each group's pack: the argument structs, the ranges, the count
+    std::vector<std::vector<char>> packs(groups.size());
+    std::vector<uint32_t> blocks(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+        const LaunchRec &r0 = *groups[g].r[0];
+        const size_t asz = r0.args.size();
+        std::vector<char> &pk = packs[g];
+        pk.assign(r0.pack_size, 0);
+        uint32_t *rng = reinterpret_cast<uint32_t *>(pk.data() + r0.range_off);
+        uint64_t start = 0;
+        for (size_t e = 0; e < groups[g].r.size(); ++e) {
+            const LaunchRec &r = *groups[g].r[e];
+            std::memcpy(pk.data() + e * asz, r.args.data(), asz);
+            rng[2 * e] = (uint32_t)start;
+            rng[2 * e + 1] = r.blocks;
+            start = (start + r.blocks + PS2_NUM_XCD - 1) / PS2_NUM_XCD * PS2_NUM_XCD;
+            if (start >= ((uint64_t)1 << 31)) return kNotMerged;
+        }
+        const int32_t cnt = (int32_t)groups[g].r.size();
+        std::memcpy(pk.data() + r0.n_off, &cnt, sizeof(cnt));
+        // the grid ends with the last range (no padding after it)
+        blocks[g] = rng[2 * (cnt - 1)] + groups[g].r.back()->blocks;
+    }
+    // chain c runs on the stream of its first session (chain 0: ss[0]'s)
+    bg.chains.assign(nch, Chain{});
+    for (size_t c = 0; c < nch; ++c)
+        for (size_t q = 0; q < n; ++q)
+            if (chain_of[q] == (int)c) {
+                bg.chains[c].stream = ss[q]->stream;
+                break;
+            }
+    if (chain_of[0] != 0) {   // ss[0] leads chain 0 (the graph's launch stream)
+        const int c0 = chain_of[0];
+        std::swap(bg.chains[0].stream, bg.chains[c0].stream);
+        for (auto &g : groups) g.chain = g.chain == 0 ? c0 : g.chain == c0 ? 0 : g.chain;
+    }
+    for (size_t g = 0; g < groups.size(); ++g) {
+        MergedLaunch m;
+        m.fn = groups[g].fn;
+        m.blocks = blocks[g];
+        m.threads = groups[g].threads;
+        m.pack = std::move(packs[g]);
+        Chain &ch = bg.chains[groups[g].chain];
+        const int st = groups[g].stage;
+        (st == ST_PREPARE ? ch.prepare : st == ST_OUTPUTS ? ch.outputs : ch.iter).push_back(std::move(m));
+    }
+    bg.ss = ss;
+    bg.device = ss[0]->device;
+    bg.max_iter = 0;
+    for (auto *s : ss) bg.max_iter = std::max<int64_t>(bg.max_iter, s->ctl_params.max_iter);
+    bg.per_graph = bg.max_iter <= LOOP_GRAPH_MAX ? bg.max_iter : 1;
+    // a graph of every iteration also holds the starting state and the outputs
+    bg.outputs_in_graph = bg.per_graph == bg.max_iter;
+    bg.events.assign(nch, nullptr);
+    for (auto &e : bg.events) PS2_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // capture: fork the chains from chain 0's stream, each chain's iterations
+    // (with the starting state and the outputs when every iteration is in),
+    // join them back
+    hipStream_t st0 = bg.chains[0].stream;
+    std::unique_lock<std::shared_mutex> cap(capture_mutex());
+    PS2_HIP(hipStreamBeginCapture(st0, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipEventRecord(bg.events[0], st0);
+    for (size_t c = 1; c < nch && le == hipSuccess; ++c) le = hipStreamWaitEvent(bg.chains[c].stream, bg.events[0], 0);
+    for (size_t c = 0; c < nch && le == hipSuccess; ++c) {
+        const Chain &ch = bg.chains[c];
+        if (bg.outputs_in_graph)
+            for (const auto &m : ch.prepare)
+                if (le == hipSuccess) le = launch_merged(m, ch.stream);
+        for (int64_t it = 0; it < bg.per_graph && le == hipSuccess; ++it)
+            for (const auto &m : ch.iter)
+                if (le == hipSuccess) le = launch_merged(m, ch.stream);
+        if (bg.outputs_in_graph)
+            for (const auto &m : ch.outputs)
+                if (le == hipSuccess) le = launch_merged(m, ch.stream);
+    }
+    for (size_t c = 1; c < nch && le == hipSuccess; ++c) {
+        le = hipEventRecord(bg.events[c], bg.chains[c].stream);
+        if (le == hipSuccess) le = hipStreamWaitEvent(st0, bg.events[c], 0);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st0, &graph);
+    cap.unlock();
+    hipError_t ie = hipErrorUnknown;
+    if (le == hipSuccess && ee == hipSuccess && graph) ie = hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        // e.g. invalidated by another host thread's allocation meanwhile;
+        // nothing of it ran
+        if (graph) (void)hipGraphDestroy(graph);
+        bg.exec = nullptr;
+        (void)hipGetLastError();
+        return kNotMerged;
+    }
+    bg.graph = graph;
+    return PS2_OK;
+}
+
+// Runs the sessions' loops as merged launches (see above); the caller holds
+// their locks.  Returns kNotMerged without running anything when they cannot
+// be merged (timing or verbose sessions, several devices).
+int sessions_run_merged(const std::vector<ps2_session *> &ss) {
+    const size_t n = ss.size();
+    for (auto *s : ss)
+        if (s->timing || s->verbose || s->device != ss[0]->device) return kNotMerged;
+    PS2_HIP(hipSetDevice(ss[0]->device));
+    for (auto *s : ss) PS2_HIP(hipStreamSynchronize(s->stream));
+    std::shared_ptr<BatchGraph> bg;
+    {
+        std::lock_guard<std::mutex> lk(g_batch_mu);
+        for (size_t b = 0; b < g_batches.size(); ++b)
+            if (g_batches[b]->ss == ss) {
+                std::rotate(g_batches.begin() + b, g_batches.begin() + b + 1, g_batches.end());
+                bg = g_batches.back();
+                break;
+            }
+    }
+    if (!bg) {
+        auto nb = std::make_shared<BatchGraph>();
+        int rc = batch_build(ss, *nb);
+        if (rc) return rc;
+        std::vector<std::shared_ptr<BatchGraph>> drop;
+        std::lock_guard<std::mutex> lk(g_batch_mu);
+        g_batches.push_back(nb);
+        bg = nb;
+        while (g_batches.size() > kBatchGraphsKept) {
+            drop.push_back(std::move(g_batches.front()));
+            g_batches.erase(g_batches.begin());
+        }
+    }
+    hipStream_t st = bg->chains[0].stream;
+    PS2_HIP(hipEventRecord(ss[0]->ev[0], st));
+    if (!bg->outputs_in_graph)
+        for (const auto &ch : bg->chains)
+            for (const auto &m : ch.prepare) PS2_HIP(launch_merged(m, st));
+    for (int64_t it = 0; it < bg->max_iter; it += bg->per_graph) PS2_HIP(hipGraphLaunch(bg->exec, st));
+    if (!bg->outputs_in_graph)
+        for (const auto &ch : bg->chains)
+            for (const auto &m : ch.outputs) PS2_HIP(launch_merged(m, st));
+    for (auto *s : ss)
+        PS2_HIP(hipMemcpyAsync(s->h_ctl.p, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, st));
+    PS2_HIP(hipEventRecord(ss[0]->ev[1], st));
+    PS2_HIP(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    PS2_HIP(hipEventElapsedTime(&ms, ss[0]->ev[0], ss[0]->ev[1]));
+    for (size_t q = 0; q < n; ++q) loop_finish(ss[q], (double)ms * 1e-3);
     return PS2_OK;
 }
 
@@ -1346,6 +1729,8 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
 }
 
 }  // namespace
+
+void batch_forget(const ps2_session *s) { batch_forget_impl(s); }
 
 // ============================================================================
 // C ABI
@@ -1516,6 +1901,24 @@ int ps2_hip_sessions_run(size_t n, ps2_session *const *sessions, int64_t *i_loop
     if (n && !sessions) return arg_error("sessions is NULL");
     for (size_t q = 0; q < n; ++q)
         if (!sessions[q]) return arg_error("session " + std::to_string(q) + " is NULL");
+    std::vector<ps2_session *> ss(sessions, sessions + n);
+    std::vector<ps2_session *> sorted(ss);
+    std::sort(sorted.begin(), sorted.end());
+    if (n >= 2 && std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end()) {
+        // every loop in one merged launch sequence (locks taken in address order)
+        std::vector<std::unique_lock<std::mutex>> locks;
+        locks.reserve(n);
+        for (auto *s : sorted) locks.emplace_back(s->mu);
+        const int rc = sessions_run_merged(ss);
+        if (rc != kNotMerged) {
+            if (rc) return rc;
+            for (size_t q = 0; q < n; ++q) {
+                if (i_loops) i_loops[q] = ss[q]->i_loop;
+                if (seconds) seconds[q] = ss[q]->loop_seconds;
+            }
+            return PS2_OK;
+        }
+    }
     return run_jobs(n, [&](size_t q) {
         int64_t il = 0;
         double sec = 0.0;
@@ -1535,7 +1938,7 @@ int ps2_hip_run_batch(size_t n, const ps2_patch_in *in, ps2_patch_out *out, cons
         const size_t m = std::min(kBatchWorkers, n - w0);
         std::vector<ps2_session *> ss(m, nullptr);
         int rc = run_jobs(m, [&](size_t q) { return session_create(&in[w0 + q], opts, &ss[q]); }, w0);
-        if (!rc) rc = run_jobs(m, [&](size_t q) { return ps2_hip_session_run(ss[q], nullptr, nullptr); }, w0);
+        if (!rc) rc = ps2_hip_sessions_run(m, ss.data(), nullptr, nullptr);
         if (!rc)
             rc = run_jobs(m, [&](size_t q) { return ps2_hip_session_download(ss[q], &out[w0 + q]); }, w0);
         for (auto *x : ss) ps2_hip_session_destroy(x);

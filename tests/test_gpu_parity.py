@@ -622,6 +622,50 @@ def test_sessions_run_concurrently_matches_sequential(S, prep_small):
             s.close()
 
 
+def test_merged_sessions_mixed_shapes_forms_and_limits(S):
+    """Sessions that differ in shape, form, weighting, bperp layout, iteration
+    limit, scan variant and CLAP (one grid has no windows) run as one merged
+    launch sequence (ps2_hip_sessions_run): every patch equals its own run bit
+    for bit, on the first call (graph built), the second (graph reused) and
+    after one session is closed (a new graph for the rest)."""
+    base = small_prep(n_ps=5000, n_ifg=16, seed=0x5EED1003)
+    runs = [
+        (small_prep(n_ps=4000, n_ifg=20, seed=0x5EED1A00), "fp32"),
+        (small_prep(n_ps=7000, n_ifg=12, seed=0x5EED1A01), "fp64"),
+        (dataclasses.replace(base, bperp_mat=None, row_invariant_bperp=True,
+                             row_bperp_nm=base.bperp_nm.copy(), filter_weighting="SNR"), "fp32"),
+        (dataclasses.replace(small_prep(n_ps=3000, n_ifg=20, seed=0x5EED1A02), gamma_max_iterations=1),
+         "fp32"),
+        (small_prep(n_ps=800, n_ifg=10, extent=600.0, seed=0x5EED1A03), "fp32"),
+        (small_prep(n_ps=70_000, n_ifg=20, extent=12_000.0, seed=0x5EED1A04), "fp32"),
+    ]
+    assert runs[4][0].n_i < 19 and runs[5][0].n_ps >= 65536
+    single = [S.run_stage2_hip(p, form=f) for p, f in runs]
+    assert single[3]["i_loop"] == 1 and max(x["i_loop"] for x in single) > 1
+    keys = ("K_ps", "C_ps", "coh_ps", "N_opt", "ph_grid", "ph_weight", "ph_res", "ph_patch", "trial_ix")
+    ss = [S.Stage2Session(p, form=f) for p, f in runs]
+
+    def check(idx, tag):
+        loops, secs = S.run_sessions([ss[k] for k in idx])
+        for q, k in enumerate(idx):
+            o = ss[k].download()
+            assert loops[q] == single[k]["i_loop"], (tag, k)
+            assert o["gamma_change_save"] == single[k]["gamma_change_save"], (tag, k)
+            for key in keys:
+                assert np.array_equal(o[key], single[k][key], equal_nan=True), (tag, k, key)
+        assert all(x > 0 for x in secs)
+
+    try:
+        check(list(range(6)), "first")
+        check(list(range(6)), "reused")
+        ss[1].close()
+        check([0, 2, 3, 4, 5], "after close")
+        check([5, 0], "two, reordered")
+    finally:
+        for s in ss:
+            s.close()
+
+
 # --- small-baseline branch ---------------------------------------------------------
 
 def test_gpu_nr_small_baseline_matches_oracle(ora):
