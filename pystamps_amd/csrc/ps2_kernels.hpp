@@ -1412,15 +1412,19 @@ __device__ __forceinline__ void rescore_body(
     cplx<RF> *__restrict__ offset,
     FitParams prm,
     const int32_t *__restrict__ gate,
+    const int64_t row_lo, const int64_t row_hi,
     const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    // lane l holds interferograms l, l + 64, ... (up to RESCORE_PER_LANE of
-    // them) for every trial; larger I re-reads the rest per trial
+    // one wavefront per queued row (wavefront bid_ of nblk_, rows in
+    // [row_lo, row_hi) only); lane l holds interferograms l, l + 64, ... (up
+    // to RESCORE_PER_LANE of them) for every trial; larger I re-reads the rest
+    // per trial
     constexpr int RESCORE_PER_LANE = 4;
     const uint32_t n = *rescore_count;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     for (uint32_t q = bid_; q < n; q += nblk_) {
         const int64_t k = rescore_list[q];
+        if (k < row_lo || k >= row_hi) continue;
         const int I = prm.I;
         const int64_t P = prm.P, C = prm.C;
         const cplx<RF> *nm = phnm + k;
@@ -1580,9 +1584,9 @@ __global__ __launch_bounds__(64) void k_rescore(KArg<M, RescoreArgs<RF>> p) {
         RescoreArgs<RF> a;
         uint32_t bid_, nblk_;
         if (!multi_locate(p, a, bid_, nblk_)) return;
-        rescore_body<RF>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.rescore_count, a.tix, a.offset, a.prm, a.gate, bid_, nblk_);
+        rescore_body<RF>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.rescore_count, a.tix, a.offset, a.prm, a.gate, 0, a.prm.P, bid_, nblk_);
     } else {
-        rescore_body<RF>(p.phnm, p.bp64, p.filt, p.cell_of, p.rescore_list, p.rescore_count, p.tix, p.offset, p.prm, p.gate, blockIdx.x, gridDim.x);
+        rescore_body<RF>(p.phnm, p.bp64, p.filt, p.cell_of, p.rescore_list, p.rescore_count, p.tix, p.offset, p.prm, p.gate, 0, p.prm.P, blockIdx.x, gridDim.x);
     }
 }
 
@@ -2168,6 +2172,19 @@ __global__ void k_zero_u32(KArg<M, ZeroArgs> p) {
     }
 }
 
+// w = (1 - prand_hi[round_half_away(1000 coh)])^2, the index clamped into
+// the table (native_stage2.rs:1674-1682)
+__device__ inline double psquare_weight(double coh, const double *__restrict__ prand_hi, int nhi) {
+    const double v = coh * 1000.0;
+    double r = v >= 0.0 ? floor(v + 0.5) : ceil(v - 0.5);
+    const double hi = (double)(nhi > 0 ? nhi - 1 : 0);
+    if (r < 0.0) r = 0.0;
+    if (r > hi) r = hi;
+    const int ix = __builtin_isnan(r) ? 0 : (int)r;
+    const double p = prand_hi[ix];
+    return (1.0 - p) * (1.0 - p);
+}
+
 // psquare weighting lookup (native_stage2.rs:1674-1682):
 // w = (1 - prand_hi[round_half_away(1000 coh)])^2
 __device__ __forceinline__ void weights_psquare_body(
@@ -2181,14 +2198,7 @@ __device__ __forceinline__ void weights_psquare_body(
     if (loop_stopped(gate)) return;
     const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (k >= P) return;
-    const double v = coh[k] * 1000.0;
-    double r = v >= 0.0 ? floor(v + 0.5) : ceil(v - 0.5);
-    const double hi = (double)(nhi > 0 ? nhi - 1 : 0);
-    if (r < 0.0) r = 0.0;
-    if (r > hi) r = hi;
-    const int ix = __builtin_isnan(r) ? 0 : (int)r;
-    const double p = prand_hi[ix];
-    W[k] = (1.0 - p) * (1.0 - p);
+    W[k] = psquare_weight(coh[k], prand_hi, nhi);
 }
 
 struct PsquareArgs {
@@ -2660,6 +2670,65 @@ __global__ __launch_bounds__(256) void k_control(KArg<M, ControlArgs> p) {
     } else {
         control_body(p.ctl, p.rms_part, p.hist, p.counters, p.prand_hi, p.cp, blockIdx.x, gridDim.x);
     }
+}
+
+
+// --- small patches: fewer launches per iteration ------------------------------
+// A patch of at most 65,536 rows has at most one 256-row group per CU, so its
+// per-row kernels leave most of the device idle and each launch costs its
+// fixed start-up.  For such patches the loop runs k_fit_tail in place of
+// k_rescore, k_refine and k_post: per 256-row group, the FP32 form's FP64
+// re-score of the group's queued rows (one wavefront per row), then the
+// refine of the group's rows, then the post step of the group (the FP64
+// re-decisions, RMS partial, coh_prev, histogram) -- the same device
+// functions and arithmetic, so the results are identical.  (The control step
+// and the P-square weights stay two launches: one workgroup looking up 20,000
+// weights took 40 us, the separate weights kernel 7.)
+template <typename R>
+struct FitTailArgs {
+    const cplx<R> *phnm;
+    const R *bp;
+    const double *bp64;
+    const cplx<R> *filt;
+    const int32_t *cell_of;
+    int32_t *tix;
+    const R *rng;
+    cplx<R> *offset;
+    double *K, *Cc, *coh, *nopt;
+    const int32_t *rescore_list;
+    uint32_t *counters;   // [0] re-score queue length, [1] rows refined again
+    FitParams prm;
+    const double *mids;
+    HistParams hp;
+    RecheckIn rc;
+    uint8_t *flags;
+    RmsHistOut ro;
+    int rescore;          // 1: FP32 form (re-score the queued rows)
+    const int32_t *gate;
+};
+
+template <bool M, typename R>
+__global__ __launch_bounds__(256) void k_fit_tail(KArg<M, FitTailArgs<R>> p) {
+    FitTailArgs<R> a;
+    uint32_t bid, nblk;
+    if constexpr (M) {
+        if (!multi_locate(p, a, bid, nblk)) return;
+    } else {
+        a = p;
+        bid = blockIdx.x;
+        nblk = gridDim.x;
+    }
+    if (a.rescore) {
+        const int64_t r0 = (int64_t)bid * 256;
+        rescore_body<R>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.counters, a.tix, a.offset, a.prm,
+                        a.gate, r0, r0 + 256, threadIdx.x >> 6, 4);
+        __syncthreads();
+    }
+    refine_body<R>(a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.K, a.Cc, a.coh, a.nopt, a.prm,
+                   a.mids, a.hp, a.rc, a.flags, a.gate, bid, nblk);
+    __syncthreads();
+    post_body<R>(a.phnm, a.filt, a.cell_of, a.tix, a.K, a.Cc, a.coh, a.prm, a.rc, a.flags, a.ro, 1, a.counters,
+                 a.gate, bid, nblk);
 }
 
 }  // namespace ps2

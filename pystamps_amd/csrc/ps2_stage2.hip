@@ -393,6 +393,10 @@ void kgo(ps2_session *s, void (*k)(KArg<M, A>), dim3 g, dim3 b, const A &a, int 
     }
 }
 
+// Patches of at most 65,536 rows (at most one 256-row group per CU) run the
+// fused k_fit_tail (re-score + refine + post step) in their loops.
+static inline bool small_patch(const ps2_session *s) { return s->P <= 65536; }
+
 // FP32 trial scan: unit steps e^{-i phi} from a per-(pixel, ifg) table, or
 // computed in the scan from the pixel's bperp.  Computing them is the better
 // choice where the scan is bound by HBM bandwidth: 8-trial chunks (fewer than
@@ -882,22 +886,6 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     }
     PS2_LAUNCHED();
     }
-    if (sizeof(R) == 4) {
-        SlotTimer tm(s, SLOT_RESCORE);
-        // one wavefront per queued row (grid-stride); the queue holds ~0.2-1% of
-        // the rows, and launching many more workgroups than that costs more
-        // than it saves (16,384 of them: 12 us at 20,000 rows for ~20 rows)
-        const RescoreArgs<R> a{s->phnm.as<cplx<R>>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
-                               s->cell_of.as<int32_t>(), s->rescore_list.as<int32_t>(),
-                               s->rescore_count.as<uint32_t>(), s->tix.as<int32_t>(), s->offset.as<cplx<R>>(),
-                               fp, s->gate};
-        kgo<M>(s, k_rescore<M, R>, dim3((unsigned)std::min<int64_t>(s->P, 4096)), dim3(64), a, ST_RESCORE);
-        PS2_LAUNCHED();
-    }
-    // the refine kernel marks the rows whose discrete choices lie within its
-    // error bound (FP32 form); k_post refines them again in FP64 and, inside
-    // the loop, computes the gamma-change RMS partials, coh_prev <- coh and
-    // the coherence histogram (k_control then takes the loop decisions)
     RmsHistOut ro{};
     ro.mids = s->mids.as<double>();
     ro.hp.P = s->P;
@@ -914,6 +902,35 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     rc.bp64 = sizeof(R) == 4 ? s->bp64.as<double>() : nullptr;
     rc.enabled = sizeof(R) == 4 ? 1 : 0;
     rc.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
+    if (in_loop && small_patch(s)) {
+        // re-score, refine and post step of each 256-row group in one workgroup
+        SlotTimer tm(s, SLOT_REFINE);
+        const FitTailArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
+                               s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
+                               s->offset.as<cplx<R>>(), s->K.as<double>(), s->Cc.as<double>(),
+                               s->coh.as<double>(), s->nopt.as<double>(), s->rescore_list.as<int32_t>(),
+                               s->rescore_count.as<uint32_t>(), fp, ro.mids, ro.hp, rc, s->flags.as<uint8_t>(),
+                               ro, sizeof(R) == 4 ? 1 : 0, s->gate};
+        kgo<M>(s, k_fit_tail<M, R>, dim3(nb), dim3(256), a, ST_REFINE);
+        PS2_LAUNCHED();
+        return PS2_OK;
+    }
+    if (sizeof(R) == 4) {
+        SlotTimer tm(s, SLOT_RESCORE);
+        // one wavefront per queued row (grid-stride); the queue holds ~0.2-1% of
+        // the rows, and launching many more workgroups than that costs more
+        // than it saves (16,384 of them: 12 us at 20,000 rows for ~20 rows)
+        const RescoreArgs<R> a{s->phnm.as<cplx<R>>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
+                               s->cell_of.as<int32_t>(), s->rescore_list.as<int32_t>(),
+                               s->rescore_count.as<uint32_t>(), s->tix.as<int32_t>(), s->offset.as<cplx<R>>(),
+                               fp, s->gate};
+        kgo<M>(s, k_rescore<M, R>, dim3((unsigned)std::min<int64_t>(s->P, 4096)), dim3(64), a, ST_RESCORE);
+        PS2_LAUNCHED();
+    }
+    // the refine kernel marks the rows whose discrete choices lie within its
+    // error bound (FP32 form); k_post refines them again in FP64 and, inside
+    // the loop, computes the gamma-change RMS partials, coh_prev <- coh and
+    // the coherence histogram (k_control then takes the loop decisions)
     {
         SlotTimer tm(s, SLOT_REFINE);
         const RefineArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
@@ -954,10 +971,9 @@ int launch_iteration(ps2_session *s) {
     if ((rc = launch_fit<R, M>(s, true))) return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[5], s->stream));
     // gamma-change RMS, stop test, Nr scaling and the P-square table
-    kgo<M>(s, k_control<M>, dim3(1), dim3(256),
-           ControlArgs{s->ctl.as<LoopCtl>(), s->rms_part.as<double>(), s->hist.as<uint32_t>(),
-                       s->rescore_count.as<uint32_t>(), s->prand_hi.as<double>(), s->ctl_params},
-           ST_CONTROL);
+    const ControlArgs ca{s->ctl.as<LoopCtl>(), s->rms_part.as<double>(), s->hist.as<uint32_t>(),
+                         s->rescore_count.as<uint32_t>(), s->prand_hi.as<double>(), s->ctl_params};
+    kgo<M>(s, k_control<M>, dim3(1), dim3(256), ca, ST_CONTROL);
     PS2_LAUNCHED();
     SlotTimer tm(s, SLOT_WEIGHTS);
     if (s->weighting == PS2_WEIGHT_PSQUARE)

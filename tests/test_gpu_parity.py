@@ -801,6 +801,27 @@ def test_edge_inputs_fp64_vs_ref64(S, ora, case):
     assert np.mean(g32["trial_ix"] == r["trial_ix"]) > 0.995
 
 
+@pytest.mark.parametrize("form", ["fp32", "fp64"])
+def test_small_patch_fused_loop_equals_separate_steps(S, prep_small, form):
+    """A patch of at most 65,536 rows runs its loop with the fused kernel
+    (re-score + refine + post step in one workgroup per 256 rows): a
+    one-iteration loop equals the same iteration done by the separate step
+    kernels (fill with K = 0 and the 1/D_A weighting, CLAP, fit) bit for
+    bit."""
+    assert prep_small.n_ps <= 65536
+    one = dataclasses.replace(prep_small, gamma_max_iterations=1)
+    with S.Stage2Session(one, form=form) as s:
+        il, _ = s.run()
+        loop = s.download()
+        assert il == 1
+        w0 = np.zeros(one.n_ps)
+        nz = one.d_a != 0
+        w0[nz] = 1.0 / one.d_a[nz]
+        fit = s.fit(s.clap(s.fill_grid(np.zeros(one.n_ps), w0)))
+    for key in ("K_ps", "C_ps", "coh_ps", "N_opt", "trial_ix", "ph_patch", "ph_res"):
+        assert np.array_equal(loop[key], fit[key], equal_nan=True), key
+
+
 def test_graph_replays_equal_stepwise_launches(S, prep_small):
     """The default loop replays one captured iteration; with timing on the same
     kernels are launched one by one with host reads between iterations.  Every
