@@ -1348,6 +1348,63 @@ __device__ __forceinline__ void scan_body(
     }
 }
 
+// --- one row per quad of lanes (small patches, FP32 form) ---------------------
+// Broadcast of lane S of each quad (4 consecutive lanes) to the quad.
+template <int S>
+__device__ __forceinline__ uint32_t quad_bcast_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, S * 0x55, 0xf, 0xf, false);
+}
+template <int S>
+__device__ __forceinline__ float quad_bcast_f32(float v) {
+    return __uint_as_float(quad_bcast_u32<S>(__float_as_uint(v)));
+}
+template <int S>
+__device__ __forceinline__ double quad_bcast_f64(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = quad_bcast_u32<S>((uint32_t)u), hi = quad_bcast_u32<S>((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float quad_max_f32(float v) {
+    v = fmaxf(v, __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0xB1, 0xf, 0xf, false)));
+    return fmaxf(v, __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0x4E, 0xf, 0xf, false)));
+}
+__device__ __forceinline__ float quad_min_f32(float v) {
+    v = fminf(v, __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0xB1, 0xf, 0xf, false)));
+    return fminf(v, __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0x4E, 0xf, 0xf, false)));
+}
+
+// Sum over the quad, the same value in all four lanes ((x0 + x1) + (x2 + x3)
+// up to the order of each commutative addition, so bit-identical).
+__device__ __forceinline__ float quad_sum_f32(float v) {
+    v += __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0xB1, 0xf, 0xf, false));
+    return v + __uint_as_float((uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(v), 0x4E, 0xf, 0xf, false));
+}
+__device__ __forceinline__ bool quad_all(bool b) {
+    int v = b ? 1 : 0;
+    v &= __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);
+    v &= __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);
+    return v != 0;
+}
+
+// Adds the terms x of the quad's lanes 0 .. n-1 (n <= 4, uniform) to acc in
+// lane order, in every lane of the quad.
+__device__ __forceinline__ void quad_add_f64(double &acc, double x, int n) {
+    const double x0 = quad_bcast_f64<0>(x), x1 = quad_bcast_f64<1>(x);
+    const double x2 = quad_bcast_f64<2>(x), x3 = quad_bcast_f64<3>(x);
+    acc += x0;
+    if (n > 1) acc += x1;
+    if (n > 2) acc += x2;
+    if (n > 3) acc += x3;
+}
+__device__ __forceinline__ void quad_add_f32_as_f64(double &acc, float x, int n) {
+    const float x0 = quad_bcast_f32<0>(x), x1 = quad_bcast_f32<1>(x);
+    const float x2 = quad_bcast_f32<2>(x), x3 = quad_bcast_f32<3>(x);
+    acc += (double)x0;
+    if (n > 1) acc += (double)x1;
+    if (n > 2) acc += (double)x2;
+    if (n > 3) acc += (double)x3;
+}
+
 template <typename R>
 struct ScanArgs {
     const cplx<R> * phnm;
@@ -1373,6 +1430,146 @@ __global__ __launch_bounds__(256) void k_scan(KArg<M, ScanArgs<R>> p) {
         scan_body<R, TC, SINGLE, NQ>(a.phnm, a.bp, a.wt, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.rescore_list, a.rescore_count, a.prm, a.gate, bid_, nblk_);
     } else {
         scan_body<R, TC, SINGLE, NQ>(p.phnm, p.bp, p.wt, p.filt, p.cell_of, p.tix, p.rng, p.offset, p.rescore_list, p.rescore_count, p.prm, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
+// The FP32 form's single-chunk scan (fewer than 8 positive trials) for small
+// patches, one row per quad of lanes: lane `sub` of the quad accumulates the
+// trial sums of interferograms sub, sub + 4, ... (the same per-element
+// arithmetic as scan_body), the quad adds its four partial sums, and the
+// selection, the offset and the re-score queueing follow as in scan_body.
+// The f32 sums run in another order than the one-thread scan, a difference
+// inside the bound the re-score margin is derived from (an I-term f32 sum in
+// any order).  64 rows per 256-thread workgroup.
+template <int NQ>
+__device__ __forceinline__ void scan_quad_body(
+    const cplx<float> *__restrict__ phnm,
+    const float *__restrict__ bp,
+    const cplx<float> *__restrict__ wt,
+    const cplx<float> *__restrict__ filt,
+    const int32_t *__restrict__ cell_of,
+    int32_t *__restrict__ tix,
+    const float *__restrict__ rng,
+    cplx<float> *__restrict__ offset,
+    int32_t *__restrict__ rescore_list,
+    uint32_t *__restrict__ rescore_count,
+    FitParams prm,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_) {
+    using R = float;
+    constexpr int TC = 8;
+    if (loop_stopped(gate)) return;
+    const int64_t k = (int64_t)bid_ * 64 + (threadIdx.x >> 2);
+    const int sub = threadIdx.x & 3;
+    if (k >= prm.P) return;   // (a quad's lanes share k)
+    const int I = prm.I;
+    const int64_t P = prm.P, C = prm.C;
+    const cplx<R> *nm = phnm + k;
+    const cplx<R> *fl = filt + cell_of[k];
+    const R *b = prm.rowinv ? bp : bp + k;
+    const int64_t bs = prm.rowinv ? 1 : P;
+    const cplx<R> *w1 = wt ? (prm.rowinv ? wt : wt + k) : nullptr;
+    const R range = rng[k];
+    const R rcp = (R)1 / range;
+    const R pi4 = RT<R>::pi4();
+    const int tn = prm.tn;
+    const int qmax = NQ > 0 ? NQ : tn;
+    bool valid = true;
+    R denom = (R)0;
+    R pre[TC], pim[TC], nre[TC], nim[TC];
+#pragma unroll
+    for (int q = 0; q < TC; ++q) pre[q] = pim[q] = nre[q] = nim[q] = (R)0;
+    const bool table = prm.step_table != 0;
+    for (int i = sub; i < I; i += 4) {
+        const cplx<R> v = psdph_value<R>(fl[(int64_t)i * C], nm[(int64_t)i * P]);
+        if (v.re == (R)0 && v.im == (R)0) valid = false;
+        denom += RT<R>::abs_(v.re, v.im);
+        pre[0] += v.re;
+        pim[0] += v.im;
+        R wr, wi;
+        if (table) {
+            const cplx<R> e = w1[(int64_t)i * bs];
+            wr = e.re;
+            wi = e.im;
+        } else {
+            R sn, cs;
+            RT<R>::sincos_(div_from_rcp(b[(int64_t)i * bs], range, rcp) * pi4, &sn, &cs);
+            wr = cs;
+            wi = -sn;
+        }
+        R zr = wr, zi = wi;
+#pragma unroll
+        for (int q = 1; q < TC; ++q) {
+            if (q <= qmax) {
+                pre[q] = fmaf(v.re, zr, pre[q]);
+                pim[q] = fmaf(v.im, zr, pim[q]);
+                nre[q] = fmaf(v.re, zi, nre[q]);
+                nim[q] = fmaf(v.im, zi, nim[q]);
+                const R nzr = fmaf(zr, wr, -zi * wi);
+                const R nzi = fmaf(zr, wi, zi * wr);
+                zr = nzr;
+                zi = nzi;
+            }
+        }
+    }
+    valid = quad_all(valid);
+    denom = quad_sum_f32(denom);
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+        if (q <= qmax) {
+            pre[q] = quad_sum_f32(pre[q]);
+            pim[q] = quad_sum_f32(pim[q]);
+            nre[q] = quad_sum_f32(nre[q]);
+            nim[q] = quad_sum_f32(nim[q]);
+        }
+    }
+    if (sub != 0) return;
+    if (!valid) {
+        tix[k] = -1;
+        return;
+    }
+    if (denom == (R)0) denom = (R)1;
+    scan_sums_from_ab<R, TC>(pre, pim, nre, nim, 1, qmax);
+    R best = -(R)INFINITY, second = -(R)INFINITY;
+    R bre = (R)0, bim = (R)0;
+    int best_ix = 0;
+    auto offer = [&](R val, int idx, R sre, R sim) {
+        if (val > best || (val == best && idx < best_ix)) {
+            if (best_ix != idx) second = best > second ? best : second;
+            best = val;
+            best_ix = idx;
+            bre = sre;
+            bim = sim;
+        } else if (val > second) {
+            second = val;
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+        if (q <= qmax) {
+            offer(RT<R>::abs_(pre[q], pim[q]) / denom, q + tn, pre[q], pim[q]);
+            if (q != 0) offer(RT<R>::abs_(nre[q], nim[q]) / denom, tn - q, nre[q], nim[q]);
+        }
+    }
+    tix[k] = best_ix;
+    offset[k] = {bre, bim};
+    if (prm.T > 1 && best - second < (R)prm.rescore_bound) {
+        const uint32_t slot = atomicAdd(rescore_count, 1u);
+        rescore_list[slot] = (int32_t)k;
+    }
+}
+
+template <bool M, int NQ>
+__global__ __launch_bounds__(256) void k_scan_quad(KArg<M, ScanArgs<float>> p) {
+    if constexpr (M) {
+        ScanArgs<float> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        scan_quad_body<NQ>(a.phnm, a.bp, a.wt, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.rescore_list,
+                           a.rescore_count, a.prm, a.gate, bid_);
+    } else {
+        scan_quad_body<NQ>(p.phnm, p.bp, p.wt, p.filt, p.cell_of, p.tix, p.rng, p.offset, p.rescore_list,
+                           p.rescore_count, p.prm, p.gate, blockIdx.x);
     }
 }
 
@@ -1882,6 +2079,109 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
     return coh;
 }
 
+// refine_row (FP32 form) of row k by the 4 lanes of a quad: lane `sub` takes
+// interferograms sub, sub + 4, ...; each lane computes its own terms (the
+// expensive part: psdph, rotation, atan2) and every lane adds the quad's
+// terms in ascending interferogram order.  The terms are the same rounded
+// f64 values and the sums run in the same order as refine_row's, so every
+// output is bit-identical to refine_row<float>; the row's work is spread over
+// 4 lanes, for patches too small to give every SIMD more than one wavefront of
+// rows.  Every lane returns the row's coh, max |k0 bp| and cut margin; lane 0
+// writes the outputs.
+template <bool ROWINV>
+__device__ inline double refine_row_quad(int64_t k, int sub, const cplx<float> *__restrict__ phnm,
+                                         const float *__restrict__ bp, const cplx<float> *__restrict__ filt,
+                                         const int32_t *__restrict__ cell_of, const int32_t *__restrict__ tix,
+                                         const float *__restrict__ rng, const cplx<float> *__restrict__ offset,
+                                         double *__restrict__ Kout, double *__restrict__ Cout,
+                                         double *__restrict__ cohout, double *__restrict__ nopt,
+                                         const FitParams &prm, float *max_angle, float *cut_margin) {
+    using R = float;
+    const int t = tix[k];
+    if (t < 0) {
+        if (sub == 0) {
+            Kout[k] = __builtin_nan("");
+            Cout[k] = 0.0;
+            cohout[k] = 0.0;
+            nopt[k] = 0.0;
+        }
+        *max_angle = 0.0f;
+        *cut_margin = 4.0f;
+        return 0.0;
+    }
+    const int I = prm.I;
+    const int64_t P = prm.P, C = prm.C;
+    const cplx<R> *nm = phnm + k;
+    const cplx<R> *fl = filt + cell_of[k];
+    auto bat = [&](int i) { return ROWINV ? bp[i] : bp[(int64_t)i * P + k]; };
+    const R range = rng[k];
+    R k0 = (R)(RT<R>::pi4() / range * (R)(t - prm.tn));
+    const cplx<R> off = offset[k];
+    const R ocr = off.re, oci = -off.im;
+    double num = 0.0, den = 0.0;
+    R amax1 = (R)0, cut = (R)RT<R>::pi4() * (R)4;
+    for (int i0 = 0; i0 < I; i0 += 4) {
+        const int i = i0 + sub;
+        double dd = 0.0, nn = 0.0;
+        if (i < I) {
+            const cplx<R> v = psdph_value<R>(fl[(int64_t)i * C], nm[(int64_t)i * P]);
+            const R bi = bat(i);
+            const double wgt = (double)RT<R>::abs_(v.re, v.im);
+            const double wb = wgt * (double)bi;
+            dd = wb * wb;
+            const R ang = k0 * bi;
+            const cplx<R> r = rotate_neg<R>(v, ang);
+            const R pr = r.re * ocr - r.im * oci, pim = r.re * oci + r.im * ocr;
+            const R a = RT<R>::atan2_(pim, pr);
+            nn = wb * (wgt * (double)a);
+            amax1 = fmax(amax1, fabs(ang));
+            cut = fmin(cut, (R)RT<R>::pi4() * (R)4 - fabs(a));
+        }
+        const int n = I - i0 < 4 ? I - i0 : 4;
+        quad_add_f64(den, dd, n);
+        quad_add_f64(num, nn, n);
+    }
+    if (den == 0.0) den = 1.0;
+    k0 = (R)((double)k0 + num / den);
+    double mre = 0.0, mim = 0.0, dn = 0.0;
+    R amax = (R)0;
+    for (int i0 = 0; i0 < I; i0 += 4) {
+        const int i = i0 + sub;
+        R rre = (R)0, rim = (R)0, rab = (R)0;
+        if (i < I) {
+            const R ang = k0 * bat(i);
+            const cplx<R> r = rotate_neg<R>(psdph_value<R>(fl[(int64_t)i * C], nm[(int64_t)i * P]), ang);
+            amax = fmax(amax, fabs(ang));
+            rre = r.re;
+            rim = r.im;
+            rab = RT<R>::abs_(r.re, r.im);
+        }
+        const int n = I - i0 < 4 ? I - i0 : 4;
+        quad_add_f32_as_f64(mre, rre, n);
+        quad_add_f32_as_f64(mim, rim, n);
+        quad_add_f32_as_f64(dn, rab, n);
+    }
+    if (dn == 0.0) dn = 1.0;
+    const double coh = hypot(mre, mim) / dn;
+    if (sub == 0) {
+        Kout[k] = (double)k0;
+        Cout[k] = (double)RT<R>::atan2_((R)mim, (R)mre);
+        cohout[k] = coh;
+        nopt[k] = 1.0;
+    }
+    *max_angle = quad_max_f32(fmaxf(amax, amax1));
+    *cut_margin = quad_min_f32(cut);
+    return coh;
+}
+
+// The FP32 form's re-decision mark of a refined row (see refine_body).
+__device__ inline uint8_t redecide_flag(int t, double coh, float amax, float cut, const double *__restrict__ mids,
+                                        HistParams hp, const RecheckIn &rc) {
+    const double bound = 2.0 * (8.0 + (double)amax) * 5.9604644775390625e-08;
+    return t >= 0 && ((double)cut < bound + 4.0e-7 || (rc.psquare && decision_distance(coh, mids, hp) < bound)) ? 1
+                                                                                                                : 0;
+}
+
 template <typename R>
 __device__ __forceinline__ void refine_body(
     const cplx<R> *__restrict__ phnm,
@@ -1921,11 +2221,7 @@ __device__ __forceinline__ void refine_body(
         // of the +-pi cut of arg() (the same bound plus the fitted atan2's
         // 3.5e-7 rad) could take the other branch in f64.  k_post refines the
         // flagged rows again in FP64.
-        const double bound = 2.0 * (8.0 + (double)amax) * 5.9604644775390625e-08;
-        flags[k] = tix[k] >= 0 && ((double)cut < bound + 4.0e-7 ||
-                                   (rc.psquare && decision_distance(coh, mids, hp) < bound))
-                       ? 1
-                       : 0;
+        flags[k] = redecide_flag(tix[k], coh, amax, cut, mids, hp, rc);
     }
 }
 
@@ -2531,7 +2827,7 @@ __device__ inline void control_step(LoopCtl *__restrict__ ctl, const double *__r
 // at 500k x 60 instead of a second launch.)  loop = 0 (the single fit step):
 // step 1 only.  counters: [0] re-score queue, [1] rows refined again;
 // control_step clears them for the next iteration.
-template <typename R>
+template <typename R, int GR = 256>
 __device__ __forceinline__ void post_body(
     const cplx<R> *__restrict__ phnm,
     const cplx<R> *__restrict__ filt,
@@ -2549,9 +2845,11 @@ __device__ __forceinline__ void post_body(
     const int32_t *__restrict__ gate,
     const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
-    // groups of 256 rows, walked grid-stride: one RMS partial per group (the
-    // same fixed tree whatever the grid), the block's histogram counts in LDS
-    // and flushed once (one global atomic per non-empty bin and workgroup)
+    // groups of GR (256 or 64) rows, walked grid-stride: one RMS partial per
+    // group (the same fixed tree whatever the grid), the block's histogram
+    // counts in LDS and flushed once (one global atomic per non-empty bin and
+    // workgroup).  Threads 0 .. GR-1 hold the group's rows.
+    static_assert(GR == 256 || GR == 64, "post step groups of 256 or 64 rows");
     __shared__ uint32_t lh[128];
     __shared__ double red[256];
     __shared__ int n_flag;
@@ -2559,10 +2857,10 @@ __device__ __forceinline__ void post_body(
     __shared__ double new_coh[256];
     if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
     uint32_t n_rechecked = 0;
-    const int64_t n_groups = (prm.P + 255) / 256;
+    const int64_t n_groups = (prm.P + GR - 1) / GR;
     for (int64_t g = bid_; g < n_groups; g += nblk_) {
-        const int64_t k = g * 256 + threadIdx.x;
-        const bool row = k < prm.P;
+        const int64_t k = g * GR + threadIdx.x;
+        const bool row = (int)threadIdx.x < GR && k < prm.P;
         double coh = row ? cohout[k] : 0.0;
         if constexpr (sizeof(R) == 4) {
             if (rc.enabled) {
@@ -2575,7 +2873,7 @@ __device__ __forceinline__ void post_body(
                 const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
                 for (int q = wave; q < nf; q += 4) {
                     const int t = flag_tid[q];
-                    const double c = refine_row_f64_wave<4>(g * 256 + t, reinterpret_cast<const cplx<float> *>(phnm),
+                    const double c = refine_row_f64_wave<4>(g * GR + t, reinterpret_cast<const cplx<float> *>(phnm),
                                                          rc.bp64, reinterpret_cast<const cplx<float> *>(filt),
                                                          cell_of, tix, Kout, Cout, cohout, prm, lane);
                     if (lane == 0) new_coh[t] = c;
@@ -2596,7 +2894,7 @@ __device__ __forceinline__ void post_body(
             }
             red[threadIdx.x] = d2;
             __syncthreads();
-            for (int o = 128; o > 0; o >>= 1) {
+            for (int o = GR / 2; o > 0; o >>= 1) {
                 if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
                 __syncthreads();
             }
@@ -2729,6 +3027,50 @@ __global__ __launch_bounds__(256) void k_fit_tail(KArg<M, FitTailArgs<R>> p) {
     __syncthreads();
     post_body<R>(a.phnm, a.filt, a.cell_of, a.tix, a.K, a.Cc, a.coh, a.prm, a.rc, a.flags, a.ro, 1, a.counters,
                  a.gate, bid, nblk);
+}
+
+// The same step for the FP32 form's small patches in groups of 64 rows, one
+// row per quad of lanes in the refine (refine_row_quad: bit-identical values,
+// 4x the wavefronts): per group the FP64 re-score of its queued rows, the
+// refine, then the post step (re-decisions, RMS partial of the 64 rows,
+// coh_prev, histogram).  A patch of 20,000 rows gives 79 groups of 256 rows,
+// one wavefront on 316 of the 1,024 SIMDs, each thread's dependent chain of
+// sincos/atan2 evaluations the bound; in 64-row groups it spreads over every
+// CU with a quarter of the chain per lane.
+constexpr int SMALL_GROUP = 64;
+template <bool M>
+__global__ __launch_bounds__(256) void k_fit_small(KArg<M, FitTailArgs<float>> p) {
+    FitTailArgs<float> a;
+    uint32_t bid, nblk;
+    if constexpr (M) {
+        if (!multi_locate(p, a, bid, nblk)) return;
+    } else {
+        a = p;
+        bid = blockIdx.x;
+        nblk = gridDim.x;
+    }
+    if (loop_stopped(a.gate)) return;
+    const int64_t r0 = (int64_t)bid * SMALL_GROUP;
+    rescore_body<float>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.counters, a.tix, a.offset, a.prm,
+                        a.gate, r0, r0 + SMALL_GROUP, threadIdx.x >> 6, 4);
+    __syncthreads();
+    {
+        const int64_t k = r0 + (threadIdx.x >> 2);
+        const int sub = threadIdx.x & 3;
+        if (k < a.prm.P) {   // (a quad's lanes share k)
+            float amax, cut;
+            const double coh =
+                a.prm.rowinv
+                    ? refine_row_quad<true>(k, sub, a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.K,
+                                            a.Cc, a.coh, a.nopt, a.prm, &amax, &cut)
+                    : refine_row_quad<false>(k, sub, a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.K,
+                                             a.Cc, a.coh, a.nopt, a.prm, &amax, &cut);
+            if (a.rc.enabled && sub == 0) a.flags[k] = redecide_flag(a.tix[k], coh, amax, cut, a.mids, a.hp, a.rc);
+        }
+    }
+    __syncthreads();
+    post_body<float, SMALL_GROUP>(a.phnm, a.filt, a.cell_of, a.tix, a.K, a.Cc, a.coh, a.prm, a.rc, a.flags, a.ro, 1,
+                                  a.counters, a.gate, bid, nblk);
 }
 
 }  // namespace ps2

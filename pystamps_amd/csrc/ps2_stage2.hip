@@ -330,7 +330,7 @@ struct ps2_session {
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
     DBuf flags;   // FP32 form: rows k_refine marks for the FP64 re-decision
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
-    int n_rms_blocks = 0;   // 256-row groups of k_post, one RMS partial each
+    int n_rms_blocks = 0;   // row groups of the loop's post step, one RMS partial each
     int n_cu = 256;         // compute units of the device
     int clap_slots = 1;     // CLAP strip wavefronts resident on the device
     // device-resident loop: control block, gate of the loop kernels (set
@@ -634,7 +634,9 @@ int alloc_state(ps2_session *s) {
     PS2_HIP(hipMemcpyAsync(s->nr0.p, s->nr_base.data(), sizeof(double) * CTL_BINS, hipMemcpyHostToDevice,
                            s->stream));
     PS2_HIP(hipStreamSynchronize(s->stream));
-    s->n_rms_blocks = (int)blocks_for(P, 256);   // one partial per 256-row group of k_post
+    // one partial per row group of the loop's post step: 64 rows in the FP32
+    // form's small-patch fit (k_fit_small), else 256 (k_post, k_fit_tail)
+    s->n_rms_blocks = (int)blocks_for(P, s->form == PS2_FORM_FP32 && small_patch(s) ? SMALL_GROUP : 256);
     if ((rc = s->rms_part.alloc(sizeof(double) * s->n_rms_blocks))) return rc;
     {   // control-kernel constants: gausswin(7, 2.5) taps and their sum (native_stage2.rs:1685-1699)
         CtlParams &cp = s->ctl_params;
@@ -856,7 +858,22 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     if constexpr (sizeof(R) == 4) {
         // FP32: every trial fits one chunk (tn < tc) except beyond 47
         // positive trials, which take 48-trial chunks
-        if (tc == 8) {
+        if (tc == 8 && small_patch(s)) {
+            // small patches: a quad of lanes per row, 64 rows per workgroup
+            auto goq = [&](auto kern) {
+                kgo<M>(s, kern, dim3(blocks_for(s->P, 64)), dim3(256), sa, ST_SCAN);
+            };
+            switch (s->tn) {
+            case 1: goq(k_scan_quad<M, 1>); break;
+            case 2: goq(k_scan_quad<M, 2>); break;
+            case 3: goq(k_scan_quad<M, 3>); break;
+            case 4: goq(k_scan_quad<M, 4>); break;
+            case 5: goq(k_scan_quad<M, 5>); break;
+            case 6: goq(k_scan_quad<M, 6>); break;
+            case 7: goq(k_scan_quad<M, 7>); break;
+            default: goq(k_scan_quad<M, 0>); break;
+            }
+        } else if (tc == 8) {
             // the trial count as a compile-time constant (no per-trial test)
             switch (s->tn) {
             case 1: go(k_scan<M, R, 8, true, 1>); break;
@@ -903,7 +920,9 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     rc.enabled = sizeof(R) == 4 ? 1 : 0;
     rc.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
     if (in_loop && small_patch(s)) {
-        // re-score, refine and post step of each 256-row group in one workgroup
+        // re-score, refine and post step of each row group in one workgroup:
+        // 64-row groups with a quad of lanes per row in the refine (FP32
+        // form), 256-row groups with a thread per row (FP64 form)
         SlotTimer tm(s, SLOT_REFINE);
         const FitTailArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
                                s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
@@ -911,7 +930,10 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
                                s->coh.as<double>(), s->nopt.as<double>(), s->rescore_list.as<int32_t>(),
                                s->rescore_count.as<uint32_t>(), fp, ro.mids, ro.hp, rc, s->flags.as<uint8_t>(),
                                ro, sizeof(R) == 4 ? 1 : 0, s->gate};
-        kgo<M>(s, k_fit_tail<M, R>, dim3(nb), dim3(256), a, ST_REFINE);
+        if constexpr (sizeof(R) == 4)
+            kgo<M>(s, k_fit_small<M>, dim3(blocks_for(s->P, SMALL_GROUP)), dim3(256), a, ST_REFINE);
+        else
+            kgo<M>(s, k_fit_tail<M, R>, dim3(nb), dim3(256), a, ST_REFINE);
         PS2_LAUNCHED();
         return PS2_OK;
     }

@@ -258,34 +258,31 @@ def test_fit_fp32_large_patch_steps_from_bperp(S, ora):
     np.testing.assert_allclose(o["coh_ps"][agree], r64["coh_ps"][agree], rtol=0, atol=1e-5)
 
 
-def _first_rows(prep, n):
-    """The patch's first n pixels on the same grid (per-pixel arrays cut)."""
-    kw = {"n_ps": n}
-    for f in dataclasses.fields(prep):
-        v = getattr(prep, f.name)
-        if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[0] == prep.n_ps:
-            kw[f.name] = np.ascontiguousarray(v[:n])
-    return dataclasses.replace(prep, **kw)
-
-
-def test_fit_fp32_steps_from_bperp_equal_table_bits(S, ora):
-    """The unit steps the 8-trial FP32 scan computes from bperp (patches of
-    65,536 rows or more) are the step table's values bit for bit: the first
-    65,535 rows of an 80,000-row patch, fitted on their own (step table), give
-    the trial, K and coherence bits they get within the whole patch."""
-    prep = small_prep(n_ps=80_000, n_ifg=20, extent=12_000.0, seed=0x5EED1901)
-    n = 65_535
-    sub = _first_rows(prep, n)
-    assert sub.n_ps == n and sub.ph_nm.shape[0] == n and sub.grid_lin.shape[0] == n
+@pytest.mark.parametrize("n_ps", [80_000, 20_000])
+def test_fit_fp32_steps_from_bperp_equal_table_bits(S, ora, tmp_path, n_ps):
+    """The unit steps the 8-trial FP32 scan can compute from bperp are the
+    step table's values bit for bit: the same fit with the steps from the
+    table and computed (chosen through the tile cache) gives the same trial,
+    K and coherence bits, for a patch of 80,000 rows (one row per thread) and
+    one of 20,000 (a quad of lanes per row)."""
+    prep = small_prep(n_ps=n_ps, n_ifg=20, extent=12_000.0 * (n_ps / 80_000) ** 0.5, seed=0x5EED1901)
     f32 = _filt_input(ora, prep, ora.REF32)
-    with S.Stage2Session(prep, form="fp32") as s:
-        whole = s.fit(f32)
-    with S.Stage2Session(sub, form="fp32") as s:
-        part = s.fit(f32)
-    assert np.array_equal(whole["trial_ix"][:n], part["trial_ix"])
+    probe = tmp_path / "probe.txt"
+    with S.Stage2Session(prep, form="fp32", autotune_cache=str(probe)) as s:
+        t = s.tiles()
+    key = [ln for ln in probe.read_text().splitlines() if ln.strip()][0].split(" fill_cells=")[0]
+    fits = {}
+    for table in (0, 1):
+        path = tmp_path / f"t{table}.txt"
+        path.write_text(f"{key} fill_cells={t['fill_cells']} scan_step_table={table} "
+                        f"clap_segments={t['clap_segments']}\n")
+        with S.Stage2Session(prep, form="fp32", autotune_cache=str(path)) as s:
+            assert s.tiles()["scan_step_table"] == bool(table)
+            fits[table] = s.fit(f32)
+    assert np.array_equal(fits[0]["trial_ix"], fits[1]["trial_ix"])
     for key in ("K_ps", "coh_ps"):
-        a = np.ascontiguousarray(whole[key][:n], dtype=np.float64)
-        b = np.ascontiguousarray(part[key], dtype=np.float64)
+        a = np.ascontiguousarray(fits[0][key], dtype=np.float64)
+        b = np.ascontiguousarray(fits[1][key], dtype=np.float64)
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), key
 
 
@@ -803,11 +800,12 @@ def test_edge_inputs_fp64_vs_ref64(S, ora, case):
 
 @pytest.mark.parametrize("form", ["fp32", "fp64"])
 def test_small_patch_fused_loop_equals_separate_steps(S, prep_small, form):
-    """A patch of at most 65,536 rows runs its loop with the fused kernel
-    (re-score + refine + post step in one workgroup per 256 rows): a
-    one-iteration loop equals the same iteration done by the separate step
-    kernels (fill with K = 0 and the 1/D_A weighting, CLAP, fit) bit for
-    bit."""
+    """A patch of at most 65,536 rows runs its loop with a fused kernel
+    (re-score + refine + post step in one workgroup per row group: 64 rows
+    with a quad of lanes per row in the FP32 form, 256 rows with a thread per
+    row in the FP64 form): a one-iteration loop equals the same iteration done
+    by the separate step kernels (fill with K = 0 and the 1/D_A weighting,
+    CLAP, fit; a thread per row) bit for bit."""
     assert prep_small.n_ps <= 65536
     one = dataclasses.replace(prep_small, gamma_max_iterations=1)
     with S.Stage2Session(one, form=form) as s:
