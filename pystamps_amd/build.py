@@ -57,5 +57,26 @@ def build(verbose: bool = False, out: Path = LIB, defines: tuple[str, ...] = ())
     return out
 
 
+PROBE_SRC = CSRC / "ps2_probe_sync.hip"
+PROBE = PKG / "ps2_probe_sync"
+
+
+def build_probe(verbose: bool = False) -> Path:
+    """The device synchronisation probe (ps2_probe_sync.hip), a stand-alone
+    executable: the cost of a kernel boundary in a captured graph against a
+    device-wide barrier inside one persistent launch."""
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+           "-o", str(PROBE), str(PROBE_SRC)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building ps2_probe_sync")
+    return PROBE
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_probe(verbose=True))
