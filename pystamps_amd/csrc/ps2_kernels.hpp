@@ -982,18 +982,42 @@ __global__ __launch_bounds__(256) void k_planes_cplx(const float2 *__restrict__ 
     dst[(int64_t)f * P + k] = {(R)v.x, (R)v.y};
 }
 
+// Planes of the caller's f64 baselines: dst in R; with dst64 the f64 values;
+// with dst_lo (FP32 form) the int32 residuals of the exact split v = (float)v
+// + lo 2^(E - 54) (Bp64), values that do not split (denormal, infinite or NaN
+// hi, a residual off the grid) counted in *inexact.
 template <typename R>
 __global__ __launch_bounds__(256) void k_planes_real(const double *__restrict__ src,
                                                      const int64_t *__restrict__ order,
                                                      R *__restrict__ dst, double *__restrict__ dst64,
+                                                     int32_t *__restrict__ dst_lo, uint32_t *__restrict__ inexact,
                                                      int64_t P, int I) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= P * I) return;
     const int64_t k = gid / I;
     const int f = (int)(gid - k * I);
     const double v = src[order[k] * I + f];
-    dst[(int64_t)f * P + k] = (R)v;
-    if (dst64) dst64[(int64_t)f * P + k] = v;
+    const int64_t e = (int64_t)f * P + k;
+    dst[e] = (R)v;
+    if (dst64) dst64[e] = v;
+    if (dst_lo) {
+        const float h = (float)v;
+        const int be = (int)((__float_as_uint(h) >> 23) & 0xffu);
+        int32_t lo = 0;
+        bool ok;
+        if (be == 0 || be == 255) {
+            ok = v == 0.0;   // a zero (either sign) is h itself
+        } else {
+            const double r = (v - (double)h) * ldexp(1.0, 180 - be);
+            ok = fabs(r) <= 1073741824.0 && r == rint(r);
+            if (ok) {
+                lo = (int32_t)r;
+                ok = (lo == 0 ? (double)h : (double)h + ldexp((double)lo, be - 180)) == v;
+            }
+        }
+        dst_lo[e] = lo;
+        if (!ok) atomicAdd(inexact, 1u);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_planes_f32(const float *__restrict__ src,
@@ -1597,10 +1621,31 @@ __device__ inline void wave_sum4_f64(double &a, double &b, double &c, double &d)
     }
 }
 
+// The caller's f64 baselines for the FP32 form's FP64 re-score and
+// re-decision: the f64 plane itself (`full`), or, where every value of the
+// plane splits exactly, the f32 bulk plane `hi` plus an int32 residual `lo`:
+// v = hi + lo 2^(E - 54), E the frexp exponent of hi (the residual v - hi of
+// a normal hi has at most 29 significant bits on the grid 2^(E - 54), so the
+// split is exact and needs 4 bytes per element instead of 8).  k_planes_real
+// writes the split and counts the values it cannot represent.
+struct Bp64 {
+    const double *full;
+    const float *hi;
+    const int32_t *lo;
+    __device__ __forceinline__ double at(int64_t e) const {
+        if (full) return full[e];
+        const float h = hi[e];
+        const int32_t l = lo[e];
+        if (l == 0) return (double)h;   // (keeps the sign of a zero)
+        const int be = (int)((__float_as_uint(h) >> 23) & 0xffu);
+        return (double)h + ldexp((double)l, be - 180);
+    }
+};
+
 template <typename RF>
 __device__ __forceinline__ void rescore_body(
     const cplx<RF> *__restrict__ phnm,
-    const double *__restrict__ bp64,
+    const Bp64 bp64,
     const cplx<RF> *__restrict__ filt,
     const int32_t *__restrict__ cell_of,
     const int32_t *__restrict__ rescore_list,
@@ -1626,7 +1671,7 @@ __device__ __forceinline__ void rescore_body(
         const int64_t P = prm.P, C = prm.C;
         const cplx<RF> *nm = phnm + k;
         const cplx<RF> *fl = filt + cell_of[k];
-        auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
+        auto bat = [&](int i) { return bp64.at(prm.rowinv ? (int64_t)i : (int64_t)i * P + k); };
         double vr[RESCORE_PER_LANE], vi[RESCORE_PER_LANE], vb[RESCORE_PER_LANE];
         double denom = 0.0, mn = INFINITY, mx = -INFINITY;
 #pragma unroll
@@ -1764,7 +1809,7 @@ __device__ __forceinline__ void rescore_body(
 template <typename RF>
 struct RescoreArgs {
     const cplx<RF> * phnm;
-    const double * bp64;
+    Bp64 bp64;
     const cplx<RF> * filt;
     const int32_t * cell_of;
     const int32_t * rescore_list;
@@ -1947,7 +1992,7 @@ struct RmsHistOut {
 // refine_candidate (:1259-1303) does in the FP64 oracle; its K, C and coh are
 // then the FP64 values.
 struct RecheckIn {
-    const double *bp64;   // [I][P] (or [I] with invariant rows): the caller's f64 bperp
+    Bp64 bp64;            // [I][P] (or [I] with invariant rows): the caller's f64 bperp
     int enabled;          // FP32 form
     int psquare;          // P-square weighting: also rows near a lookup / bin boundary
 };
@@ -1972,7 +2017,7 @@ __device__ inline double decision_distance(double coh, const double *__restrict_
 // beyond 64 RW interferograms the rest are read again in each pass.
 template <int RW>
 __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__restrict__ phnm,
-                                             const double *__restrict__ bp64,
+                                             const Bp64 &bp64,
                                              const cplx<float> *__restrict__ filt,
                                              const int32_t *__restrict__ cell_of,
                                              const int32_t *__restrict__ tix, double *__restrict__ Kout,
@@ -1982,7 +2027,7 @@ __device__ inline double refine_row_f64_wave(int64_t k, const cplx<float> *__res
     const int64_t P = prm.P, C = prm.C;
     const cplx<float> *nm = phnm + k;
     const cplx<float> *fl = filt + cell_of[k];
-    auto bat = [&](int i) { return prm.rowinv ? bp64[i] : bp64[(int64_t)i * P + k]; };
+    auto bat = [&](int i) { return bp64.at(prm.rowinv ? (int64_t)i : (int64_t)i * P + k); };
     // psdph in f64: conj(unit filtered phase) x ph_nm
     auto load = [&](int i, double &re, double &im, double &b) {
         const cplx<float> f = fl[(int64_t)i * C], n = nm[(int64_t)i * P];
@@ -2986,7 +3031,7 @@ template <typename R>
 struct FitTailArgs {
     const cplx<R> *phnm;
     const R *bp;
-    const double *bp64;
+    Bp64 bp64;
     const cplx<R> *filt;
     const int32_t *cell_of;
     int32_t *tix;

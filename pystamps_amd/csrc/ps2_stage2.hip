@@ -324,6 +324,9 @@ struct ps2_session {
     HistRule hist_rule;
     // device state
     DBuf phnm, bp, bp64, amp, grid, filt, part, lp, rng, wt, offset;
+    // FP32 form: the f64 baselines as bp (f32) + the int32 residuals bplo
+    // (exact, Bp64), or, where some value does not split, the f64 plane bp64
+    DBuf bplo;
     // outputs in the caller's layouts (row order), written by k_final_outputs
     // at the end of every loop (and by the fit step)
     DBuf o_weight, o_patch, o_res, o_vec, o_tix, o_grid;
@@ -492,24 +495,46 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
                            s->d_order.as<int64_t>(), s->phnm.as<cplx<R>>(), P, (int)I);
         PS2_LAUNCHED();
     }
-    {   // baselines: R copy for the bulk, f64 copy for the FP64 re-score
+    {   // baselines: R copy for the bulk; the FP32 form's f64 values for the
+        // FP64 re-score as an exact split (Bp64: 4 more bytes per element) or,
+        // where some value does not split, as an f64 plane (8)
         const int64_t n = s->rowinv ? I : P * I;
         if ((rc = s->bp.alloc(sizeof(R) * n))) return rc;
-        if (sizeof(R) == 4)
-            if ((rc = s->bp64.alloc(sizeof(double) * n))) return rc;
         if (s->rowinv) {
             std::vector<R> h((size_t)n);
             for (int64_t f = 0; f < I; ++f) h[f] = (R)in->bperp_row[f];
             PS2_HIP(h2d_now(s, s->bp.p, h.data(), sizeof(R) * n));
-            if (sizeof(R) == 4)
+            if (sizeof(R) == 4) {
+                if ((rc = s->bp64.alloc(sizeof(double) * n))) return rc;
                 PS2_HIP(h2d_now(s, s->bp64.p, in->bperp_row, sizeof(double) * n));
+            }
         } else {
             PS2_HIP(hipStreamSynchronize(s->stream));  // the staging buffer is free again
             if ((rc = staging().h2d(stage.p, in->bperp_mat, sizeof(double) * P * I, s->stream))) return rc;
+            DBuf inexact;
+            if (sizeof(R) == 4) {
+                if ((rc = s->bplo.alloc(sizeof(int32_t) * n))) return rc;
+                if ((rc = inexact.alloc(sizeof(uint32_t)))) return rc;
+                PS2_HIP(hipMemsetAsync(inexact.p, 0, sizeof(uint32_t), s->stream));
+            }
             hipLaunchKernelGGL(k_planes_real<R>, dim3(nb), dim3(256), 0, s->stream, stage.as<double>(),
-                               s->d_order.as<int64_t>(), s->bp.as<R>(),
-                               sizeof(R) == 4 ? s->bp64.as<double>() : nullptr, P, (int)I);
+                               s->d_order.as<int64_t>(), s->bp.as<R>(), (double *)nullptr,
+                               sizeof(R) == 4 ? s->bplo.as<int32_t>() : nullptr,
+                               sizeof(R) == 4 ? inexact.as<uint32_t>() : nullptr, P, (int)I);
             PS2_LAUNCHED();
+            if (sizeof(R) == 4) {
+                uint32_t n_inexact = 0;
+                PS2_HIP(hipMemcpyAsync(&n_inexact, inexact.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+                PS2_HIP(hipStreamSynchronize(s->stream));
+                if (n_inexact) {   // the f64 plane instead
+                    if ((rc = s->bplo.alloc(0))) return rc;
+                    if ((rc = s->bp64.alloc(sizeof(double) * n))) return rc;
+                    hipLaunchKernelGGL(k_planes_real<R>, dim3(nb), dim3(256), 0, s->stream, stage.as<double>(),
+                                       s->d_order.as<int64_t>(), s->bp.as<R>(), s->bp64.as<double>(),
+                                       (int32_t *)nullptr, (uint32_t *)nullptr, P, (int)I);
+                    PS2_LAUNCHED();
+                }
+            }
         }
     }
     if (in->amp && s->weighting == PS2_WEIGHT_SNR) {
@@ -813,6 +838,12 @@ int launch_clap(ps2_session *s, bool unit) {
     return PS2_OK;
 }
 
+// The FP32 form's f64 baselines (Bp64); unused by the FP64 form.
+static Bp64 bp64_of(const ps2_session *s) {
+    if (s->form != PS2_FORM_FP32) return Bp64{nullptr, nullptr, nullptr};
+    return Bp64{s->bp64.as<double>(), s->bp.as<float>(), s->bplo.as<int32_t>()};
+}
+
 FitParams fit_params(ps2_session *s) {
     FitParams f;
     f.P = s->P;
@@ -916,7 +947,7 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         ro.hist = s->hist.as<uint32_t>();
     }
     RecheckIn rc{};
-    rc.bp64 = sizeof(R) == 4 ? s->bp64.as<double>() : nullptr;
+    rc.bp64 = bp64_of(s);
     rc.enabled = sizeof(R) == 4 ? 1 : 0;
     rc.psquare = s->weighting == PS2_WEIGHT_PSQUARE ? 1 : 0;
     if (in_loop && small_patch(s)) {
@@ -924,7 +955,7 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         // 64-row groups with a quad of lanes per row in the refine (FP32
         // form), 256-row groups with a thread per row (FP64 form)
         SlotTimer tm(s, SLOT_REFINE);
-        const FitTailArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
+        const FitTailArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), rc.bp64, s->filt.as<cplx<R>>(),
                                s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
                                s->offset.as<cplx<R>>(), s->K.as<double>(), s->Cc.as<double>(),
                                s->coh.as<double>(), s->nopt.as<double>(), s->rescore_list.as<int32_t>(),
@@ -942,7 +973,7 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
         // one wavefront per queued row (grid-stride); the queue holds ~0.2-1% of
         // the rows, and launching many more workgroups than that costs more
         // than it saves (16,384 of them: 12 us at 20,000 rows for ~20 rows)
-        const RescoreArgs<R> a{s->phnm.as<cplx<R>>(), s->bp64.as<double>(), s->filt.as<cplx<R>>(),
+        const RescoreArgs<R> a{s->phnm.as<cplx<R>>(), rc.bp64, s->filt.as<cplx<R>>(),
                                s->cell_of.as<int32_t>(), s->rescore_list.as<int32_t>(),
                                s->rescore_count.as<uint32_t>(), s->tix.as<int32_t>(), s->offset.as<cplx<R>>(),
                                fp, s->gate};

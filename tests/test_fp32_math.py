@@ -165,3 +165,31 @@ def test_scan_quotient_has_ieee_bits():
         e, d = e.astype(F32), d.astype(F32)
         q = _div_from_rcp(e, d)
         np.testing.assert_array_equal(q.view(np.uint32), (e / d).view(np.uint32))
+
+
+def test_f64_baseline_split_is_exact():
+    """The FP32 form stores its f64 baselines as h = float32(v) plus an int32
+    residual lo = (v - h) 2^(180 - be) (be the biased exponent bits of h), and
+    reads them back as h + lo 2^(be - 180).  Every double whose float32 image
+    is normal splits exactly, with |lo| <= 2^29: random doubles over the f32
+    normal range, doubles next to powers of two (where h rounds up into the
+    next binade) and every representable residual sign."""
+    rng = np.random.default_rng(7)
+    v = np.concatenate([
+        rng.uniform(-600.0, 600.0, 200_000),
+        np.exp(rng.uniform(np.log(1.2e-38), np.log(3.0e38), 100_000)) * rng.choice([-1.0, 1.0], 100_000),
+        np.nextafter(2.0 ** np.arange(-120, 120, dtype=np.float64), 0.0),
+        np.nextafter(2.0 ** np.arange(-120, 120, dtype=np.float64), np.inf),
+        2.0 ** np.arange(-120, 120, dtype=np.float64) * (1.0 - 2.0 ** -25),
+    ])
+    h = v.astype(np.float32)
+    assert np.all(np.isfinite(h)) and np.all(np.abs(h) >= np.finfo(np.float32).tiny)
+    be = ((h.view(np.uint32) >> 23) & 0xFF).astype(np.int64)
+    r = (v - h.astype(np.float64)) * np.ldexp(1.0, 180 - be)
+    assert np.all(r == np.rint(r)) and np.all(np.abs(r) <= 2.0 ** 29)
+    lo = r.astype(np.int32)
+    back = h.astype(np.float64) + np.ldexp(lo.astype(np.float64), be - 180)
+    assert np.array_equal(back, v)
+    # a wrong scale (one bit short) loses residual bits somewhere
+    r_bad = (v - h.astype(np.float64)) * np.ldexp(1.0, 179 - be)
+    assert not np.all(r_bad == np.rint(r_bad))

@@ -820,6 +820,29 @@ def test_small_patch_fused_loop_equals_separate_steps(S, prep_small, form):
         assert np.array_equal(loop[key], fit[key], equal_nan=True), key
 
 
+def test_fp32_f64_baselines_split_and_plane_agree(S, prep_small):
+    """The FP32 form keeps the f64 baselines of its FP64 re-score as the f32
+    bulk value plus an int32 residual (exact for every value whose f32 image
+    is normal or zero) and falls back to an f64 plane when some value does not
+    split.  A baseline of 1e-300 (f32 image 0, not split) puts the patch on
+    the plane; the same patch with 0 there runs the split: the outputs agree
+    bit for bit (the two f64 values differ by 1e-300, below every sum's
+    rounding)."""
+    assert prep_small.bperp_mat is not None and not prep_small.row_invariant_bperp
+    outs = []
+    for v in (0.0, 1e-300):
+        bm = np.array(prep_small.bperp_mat, copy=True)
+        bm[7, 3] = v
+        p = dataclasses.replace(prep_small, bperp_mat=bm)
+        with S.Stage2Session(p, form="fp32") as s:
+            s.run()
+            outs.append(s.download())
+    a, b = outs
+    assert a["n_rescored"] > 0 and a["i_loop"] == b["i_loop"]
+    for key in ("K_ps", "C_ps", "coh_ps", "trial_ix", "ph_patch", "ph_res", "ph_weight", "ph_grid", "Nr"):
+        assert np.array_equal(a[key], b[key], equal_nan=True), key
+
+
 def test_graph_replays_equal_stepwise_launches(S, prep_small):
     """The default loop replays one captured iteration; with timing on the same
     kernels are launched one by one with host reads between iterations.  Every
