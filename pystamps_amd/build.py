@@ -57,24 +57,29 @@ def build(verbose: bool = False, out: Path = LIB, defines: tuple[str, ...] = ())
     return out
 
 
-PROBE_SRC = CSRC / "ps2_probe_sync.hip"
-PROBE = PKG / "ps2_probe_sync"
+# Stand-alone measurement programs behind two design choices:
+#   ps2_probe_sync: a kernel boundary in a captured graph against a
+#     device-wide barrier inside one persistent launch;
+#   ps2_probe_dft: the CLAP window transform as FP32 MFMA DFT-matrix products
+#     against the radix-2 FFT in registers.
+PROBES = ["ps2_probe_sync", "ps2_probe_dft"]
 
 
-def build_probe(verbose: bool = False) -> Path:
-    """The device synchronisation probe (ps2_probe_sync.hip), a stand-alone
-    executable: the cost of a kernel boundary in a captured graph against a
-    device-wide barrier inside one persistent launch."""
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
-           "-o", str(PROBE), str(PROBE_SRC)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building ps2_probe_sync")
-    return PROBE
+def build_probe(verbose: bool = False) -> list[Path]:
+    out = []
+    for name in PROBES:
+        exe = PKG / name
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+               "-I", str(CSRC), "-o", str(exe), str(CSRC / f"{name}.hip")]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed building {name}")
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":
