@@ -935,6 +935,252 @@ __global__ __launch_bounds__(256) void k_clap_combine(KArg<M, CombineArgs<R>> p)
     }
 }
 
+// --- CLAP for any window size (clap_win other than 32) ------------------------
+// The same filter (native_stage2.rs:742-1184) for an n_win x n_win window in
+// an n x n transform, n = n_win_ex = round(clap_win) up to GCLAP_MAX and
+// n_win = round(0.75 clap_win), window step n_inc = max(n_win / 4, 1):
+// one 256-thread workgroup per (window, interferogram) with the window in LDS;
+// direct n-point DFTs (twiddles e^{-2 pi i m / n} in R from f64 values), the
+// reference's fftshift by floor(n / 2) and ifftshift by ceil(n / 2), the
+// separable 7-tap smoothing with zero fill, the exact median (bitonic sort of
+// the n^2 gains, mean of the two middle values for an even count), the gain
+// and low-pass, the inverse DFT, and the window weight times 1 / n^2; the
+// weighted window goes to its own slot of `gpart`.  k_gclap_combine then adds,
+// per cell, the windows over it in window order (row of windows ascending,
+// then column), in f64 as the reference's accumulator, and rounds once to R.
+// NMAX (16, 32 or 64) sizes the LDS; kernels of the default window (24 of 32)
+// are the strip kernels above.
+constexpr int GCLAP_MAX = 64;
+
+template <typename R>
+struct GClapArgs {
+    const cplx<R> *grid;     // [I][ni][nj]
+    cplx<R> *gpart;          // [I][nwi * nwj][n_win][n_win]
+    const R *lp;             // low_pass [n][n], the reference's layout
+    const double *weight;    // [n_win][n_win] window weight / n^2
+    int ni, nj, I, nwi, nwj, n_win, n, n_inc;
+    R alpha, beta;
+    R k7[7];
+    const int32_t *gate;
+};
+
+// one pass of an n-point DFT over an n x n plane held in LDS: along rows
+// (dst[r][k] = sum_m src[r][m] w^(km)) or along columns (dst[k][c] = sum_m
+// src[m][c] w^(km)), m < mlim (the rest of src is zero), outputs with both
+// indices below olim; w = e^{-2 pi i / n}, conjugated for INV.
+template <typename R, bool ROWS, bool INV>
+__device__ inline void gclap_dft_pass(const R *__restrict__ sr, const R *__restrict__ si, R *__restrict__ dr,
+                                      R *__restrict__ di, const R *__restrict__ twr, const R *__restrict__ twi,
+                                      int n, int mlim, int olim) {
+    for (int e = threadIdx.x; e < olim * olim; e += blockDim.x) {
+        const int a = e / olim, k = e - a * olim;   // ROWS: row a, output k; else output a, column k
+        const int kk = ROWS ? k : a;
+        R accr = (R)0, acci = (R)0;
+        int t = 0;
+        for (int m = 0; m < mlim; ++m) {
+            const int at = ROWS ? a * n + m : m * n + k;
+            const R xr = sr[at], xi = si[at];
+            const R wr = twr[t], wi = INV ? -twi[t] : twi[t];
+            accr += xr * wr - xi * wi;
+            acci += xr * wi + xi * wr;
+            t += kk;
+            if (t >= n) t -= n;
+        }
+        const int o = ROWS ? a * n + k : a * n + k;
+        dr[o] = accr;
+        di[o] = acci;
+    }
+}
+
+template <typename R, int NMAX, bool ALPHA_ONE>
+__device__ __forceinline__ void gclap_window_body(const GClapArgs<R> &a, const uint32_t bid_) {
+    if (loop_stopped(a.gate)) return;
+    constexpr int NN = NMAX * NMAX;
+    __shared__ R xr[NN], xi[NN], yr[NN], yi[NN], twr[NMAX], twi[NMAX];
+    const int n = a.n, nn = n * n, nw = a.n_win, t = threadIdx.x;
+    const int nwin = a.nwi * a.nwj;
+    const int w = (int)(bid_ % (uint32_t)nwin), f = (int)(bid_ / (uint32_t)nwin);
+    const int wa = w / a.nwj, wb = w - (w / a.nwj) * a.nwj;
+    int i1 = wa * a.n_inc, rs = 0;
+    if (i1 + nw > a.ni) {
+        rs = i1 + nw - a.ni;
+        i1 = a.ni - nw;
+    }
+    int j1 = wb * a.n_inc, cs = 0;
+    if (j1 + nw > a.nj) {
+        cs = j1 + nw - a.nj;
+        j1 = a.nj - nw;
+    }
+    for (int m = t; m < n; m += blockDim.x) {
+        double sn, cn;
+        sincospi(2.0 * (double)m / (double)n, &sn, &cn);
+        twr[m] = (R)cn;
+        twi[m] = (R)-sn;
+    }
+    // the window, zero padded to n x n (NaN -> 0, native_stage2.rs:852-858)
+    const cplx<R> *g = a.grid + (int64_t)f * a.ni * a.nj;
+    for (int k = t; k < nn; k += blockDim.x) {
+        const int r = k / n, c = k - r * n;
+        R vr = (R)0, vi = (R)0;
+        if (r < nw && c < nw) {
+            const cplx<R> v = g[(int64_t)(i1 + r) * a.nj + j1 + c];
+            if (!RT<R>::isnan_(v.re) && !RT<R>::isnan_(v.im)) {
+                vr = v.re;
+                vi = v.im;
+            }
+        }
+        xr[k] = vr;
+        xi[k] = vi;
+    }
+    __syncthreads();
+    gclap_dft_pass<R, true, false>(xr, xi, yr, yi, twr, twi, n, nw, n);   // rows (columns >= nw zero)
+    __syncthreads();
+    gclap_dft_pass<R, false, false>(yr, yi, xr, xi, twr, twi, n, nw, n);  // columns (rows >= nw zero)
+    __syncthreads();
+    // |F| at the fftshifted position (shift floor(n / 2)) -> yr
+    const int fsh = n / 2, ish = (n + 1) / 2;
+    for (int k = t; k < nn; k += blockDim.x) {
+        const int r = k / n, c = k - r * n;
+        const int src = ((r + fsh) % n) * n + (c + fsh) % n;
+        yr[k] = RT<R>::abs_(xr[src], xi[src]);
+    }
+    __syncthreads();
+    // 7-tap smoothing along rows (zero fill) -> yi, along columns -> yr
+    for (int k = t; k < nn; k += blockDim.x) {
+        const int r = k / n, c = k - r * n;
+        R sm = (R)0;
+        for (int q = 0; q < 7; ++q) {
+            const int sc = c + q - 3;
+            if (sc >= 0 && sc < n) sm += yr[r * n + sc] * a.k7[q];
+        }
+        yi[k] = sm;
+    }
+    __syncthreads();
+    for (int k = t; k < nn; k += blockDim.x) {
+        const int r = k / n, c = k - r * n;
+        R sm = (R)0;
+        for (int q = 0; q < 7; ++q) {
+            const int sr2 = r + q - 3;
+            if (sr2 >= 0 && sr2 < n) sm += yi[sr2 * n + c] * a.k7[q];
+        }
+        yr[k] = sm;
+    }
+    __syncthreads();
+    // ifftshift (shift ceil(n / 2)) -> yi: the gains H in transform order
+    for (int k = t; k < nn; k += blockDim.x) {
+        const int r = k / n, c = k - r * n;
+        yi[k] = yr[((r + ish) % n) * n + (c + ish) % n];
+    }
+    __syncthreads();
+    // exact median: bitonic sort of a copy (padded with +inf to a power of two)
+    int p2 = 1;
+    while (p2 < nn) p2 <<= 1;
+    for (int k = t; k < p2; k += blockDim.x) yr[k] = k < nn ? yi[k] : (R)INFINITY;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int e = t; e < p2 / 2; e += blockDim.x) {
+                const int lo = (e / stride) * stride * 2 + (e % stride), hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const R u = yr[lo], v = yr[hi];
+                if ((u > v) == up) {
+                    yr[lo] = v;
+                    yr[hi] = u;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const R med = (nn % 2 == 0) ? (yr[nn / 2 - 1] + yr[nn / 2]) / (R)2 : yr[nn / 2];
+    // gain and low-pass (native_stage2.rs:876-895)
+    for (int k = t; k < nn; k += blockDim.x) {
+        R v = yi[k];
+        if (med != (R)0) v /= med;
+        v = ALPHA_ONE ? v - (R)1 : RT<R>::pow_(v, a.alpha) - (R)1;
+        if (v < (R)0) v = (R)0;
+        const R gsc = v * a.beta + a.lp[k];
+        xr[k] *= gsc;
+        xi[k] *= gsc;
+    }
+    __syncthreads();
+    gclap_dft_pass<R, true, true>(xr, xi, yr, yi, twr, twi, n, n, n);     // inverse rows
+    __syncthreads();
+    gclap_dft_pass<R, false, true>(yr, yi, xr, xi, twr, twi, n, n, nw);   // inverse columns, the window only
+    __syncthreads();
+    cplx<R> *o = a.gpart + ((int64_t)f * nwin + w) * nw * nw;
+    for (int k = t; k < nw * nw; k += blockDim.x) {
+        const int li = k / nw, lj = k - li * nw;
+        R wt = (R)0;
+        if (li >= rs && lj >= cs) wt = (R)a.weight[(li - rs) * nw + (lj - cs)];
+        o[k] = {xr[li * n + lj] * wt, xi[li * n + lj] * wt};
+    }
+}
+
+template <bool M, typename R, int NMAX, bool ALPHA_ONE>
+__global__ __launch_bounds__(256) void k_gclap_window(KArg<M, GClapArgs<R>> p) {
+    if constexpr (M) {
+        GClapArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        gclap_window_body<R, NMAX, ALPHA_ONE>(a, bid_);
+    } else {
+        gclap_window_body<R, NMAX, ALPHA_ONE>(p, blockIdx.x);
+    }
+}
+
+// per (interferogram, cell): the windows over the cell in window order, f64 sum
+template <typename R, bool UNIT>
+__device__ __forceinline__ void gclap_combine_body(const GClapArgs<R> &a, cplx<R> *__restrict__ filt,
+                                                   const uint32_t bid_) {
+    if (loop_stopped(a.gate)) return;
+    const int64_t C = (int64_t)a.ni * a.nj;
+    const int64_t gid = (int64_t)bid_ * blockDim.x + threadIdx.x;
+    if (gid >= C * a.I) return;
+    const int f = (int)(gid / C);
+    const int64_t cell = gid - (int64_t)f * C;
+    const int r = (int)(cell / a.nj), c = (int)(cell - (int64_t)r * a.nj);
+    const int nw = a.n_win, nwin = a.nwi * a.nwj;
+    const cplx<R> *pf = a.gpart + (int64_t)f * nwin * nw * nw;
+    double sr = 0.0, si = 0.0;
+    // windows ascending; the start row (column) of window q is
+    // min(q n_inc, ni - n_win), non-decreasing in q
+    const int wa0 = r - nw + 1 > 0 ? (r - nw + 1 + a.n_inc - 1) / a.n_inc : 0;
+    const int wb0 = c - nw + 1 > 0 ? (c - nw + 1 + a.n_inc - 1) / a.n_inc : 0;
+    for (int wa = wa0 < a.nwi ? wa0 : a.nwi; wa < a.nwi; ++wa) {
+        const int i1 = wa * a.n_inc + nw > a.ni ? a.ni - nw : wa * a.n_inc;
+        if (i1 > r) break;
+        if (r >= i1 + nw) continue;
+        for (int wb = wb0 < a.nwj ? wb0 : a.nwj; wb < a.nwj; ++wb) {
+            const int j1 = wb * a.n_inc + nw > a.nj ? a.nj - nw : wb * a.n_inc;
+            if (j1 > c) break;
+            if (c >= j1 + nw) continue;
+            const cplx<R> v = pf[((int64_t)(wa * a.nwj + wb) * nw + (r - i1)) * nw + (c - j1)];
+            sr += (double)v.re;
+            si += (double)v.im;
+        }
+    }
+    const cplx<R> sum{(R)sr, (R)si};
+    filt[gid] = UNIT ? unit_phase<R>(sum) : sum;
+}
+
+template <typename R>
+struct GCombineArgs {
+    GClapArgs<R> g;
+    cplx<R> *filt;
+};
+
+template <bool M, typename R, bool UNIT>
+__global__ __launch_bounds__(256) void k_gclap_combine(KArg<M, GCombineArgs<R>> p) {
+    if constexpr (M) {
+        GCombineArgs<R> a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        gclap_combine_body<R, UNIT>(a.g, a.filt, bid_);
+    } else {
+        gclap_combine_body<R, UNIT>(p.g, p.filt, blockIdx.x);
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_unit_phase(cplx<R> *__restrict__ v, int64_t n) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

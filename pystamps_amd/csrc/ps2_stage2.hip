@@ -308,6 +308,11 @@ struct ps2_session {
     int tn = 0, T = 0, rowinv = 0;
     int nwi = 0, nwj = 0;
     bool windows = false;
+    // CLAP window: n_win x n_win in an n_ex-point transform, step n_inc;
+    // gclap: any other window than 24 of 32 (the generic kernels)
+    int n_win = CLAP_WIN, n_ex = CLAP_FFT, n_inc = CLAP_INC;
+    bool gclap = false;
+    DBuf gpart, gweight;
     ClapSegs segs{1, 1, 4};   // set with the device state
     // tiles (model rules, or the autotuner's choice): fill workgroup cells,
     // FP32 scan unit-step source, CLAP segmentation above
@@ -435,9 +440,10 @@ int validate(const ps2_patch_in *in) {
         return arg_error("a required input array is NULL");
     if (!in->row_invariant && !in->bperp_mat)
         return arg_error("bp1.bperp_mat is required for non-invariant stage-2 baselines");
-    if (in->n_win_ex != CLAP_FFT || in->clap_window != CLAP_WIN)
-        return arg_error("the HIP backend implements clap_win = 32 (24x24 windows, 32-point FFT); got n_win_ex=" +
-                         std::to_string(in->n_win_ex) + " clap_window=" + std::to_string(in->clap_window));
+    if (in->n_win_ex < 1 || in->n_win_ex > GCLAP_MAX || in->clap_window < 1 || in->clap_window > in->n_win_ex)
+        return arg_error("CLAP window outside the HIP backend's range: n_win_ex=" + std::to_string(in->n_win_ex) +
+                         " clap_window=" + std::to_string(in->clap_window) + " (1 <= clap_window <= n_win_ex <= " +
+                         std::to_string(GCLAP_MAX) + ", i.e. clap_win up to 64)");
     if (in->n_trial < 1 || (in->n_trial % 2) != 1)
         return arg_error("trial_values must hold 2*tn+1 values");
     const int64_t tn = in->n_trial / 2;
@@ -555,7 +561,7 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
         if ((rc = s->W0.alloc(sizeof(double) * P))) return rc;
         PS2_HIP(h2d_now(s, s->W0.p, w0.data(), sizeof(double) * P));
     }
-    {   // CLAP constants: low_pass, window weight x 1/n^2
+    if (!s->gclap) {   // CLAP constants: low_pass (the window weight is computed in the kernel)
         // stored transposed: column c's 32 values are contiguous (the CLAP
         // kernel's lane c reads them with vector loads)
         std::vector<R> lp(CLAP_FFT * CLAP_FFT);
@@ -563,6 +569,31 @@ int upload_inputs(ps2_session *s, const ps2_patch_in *in) {
             for (int c = 0; c < CLAP_FFT; ++c) lp[c * CLAP_FFT + r] = (R)in->low_pass[r * CLAP_FFT + c];
         if ((rc = s->lp.alloc(sizeof(R) * lp.size()))) return rc;
         PS2_HIP(h2d_now(s, s->lp.p, lp.data(), sizeof(R) * lp.size()));
+    } else {   // any other window: low_pass in its own layout, window weight x 1/n^2
+        const int n = s->n_ex, nw = s->n_win;
+        std::vector<R> lp((size_t)n * n);
+        for (size_t k = 0; k < lp.size(); ++k) lp[k] = (R)in->low_pass[k];
+        if ((rc = s->lp.alloc(sizeof(R) * lp.size()))) return rc;
+        PS2_HIP(h2d_now(s, s->lp.p, lp.data(), sizeof(R) * lp.size()));
+        // clap_window_weight (native_stage2.rs:1014-1037): r + c + 1e-6 on a
+        // half x half quadrant, mirrored; an odd window's last row and column
+        // stay 0
+        const int half = nw / 2;
+        std::vector<double> wt((size_t)nw * nw, 0.0), top((size_t)half * nw, 0.0);
+        for (int r = 0; r < half; ++r)
+            for (int c = 0; c < half; ++c) {
+                top[(size_t)r * nw + c] = (double)r + (double)c + 1.0e-6;
+                top[(size_t)r * nw + half + c] = (double)r + (double)(half - 1 - c) + 1.0e-6;
+            }
+        for (int r = 0; r < half; ++r)
+            for (int c = 0; c < nw; ++c) {
+                wt[(size_t)r * nw + c] = top[(size_t)r * nw + c];
+                wt[(size_t)(half + r) * nw + c] = top[(size_t)(half - 1 - r) * nw + c];
+            }
+        const double inv_scale = 1.0 / (double)(n * n);
+        for (double &v : wt) v = v * inv_scale;
+        if ((rc = s->gweight.alloc(sizeof(double) * wt.size()))) return rc;
+        PS2_HIP(h2d_now(s, s->gweight.p, wt.data(), sizeof(double) * wt.size()));
     }
     return PS2_OK;
 }
@@ -617,7 +648,15 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->filt.alloc(sizeof(cplx<R>) * C * I))) return rc;
     PS2_HIP(hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
     s->n_cu = std::max(1, s->n_cu);
-    if (s->windows) {
+    if (s->windows && s->gclap) {
+        const double bytes = (double)sizeof(cplx<R>) * (double)s->nwi * (double)s->nwj * (double)s->n_win *
+                             (double)s->n_win * (double)I;
+        if (bytes > 32e9)
+            return arg_error("CLAP window of " + std::to_string(s->n_win) + " cells on a " + std::to_string(s->ni) +
+                             "x" + std::to_string(s->nj) + " grid needs " + std::to_string(bytes / 1e9) +
+                             " GB of window outputs (limit 32 GB per session)");
+        if ((rc = s->gpart.alloc((size_t)bytes))) return rc;
+    } else if (s->windows) {
         // strip slots: CUs x CLAP wavefronts resident per CU (register bound)
         const int n_cu = s->n_cu;
         int per_cu = 0;
@@ -808,6 +847,57 @@ int launch_clap(ps2_session *s, bool unit) {
         const int64_t words = (int64_t)(sizeof(cplx<R>) / 4) * n;
         kgo<M>(s, k_zero_u32<M>, dim3(blocks_for(words, 256)), dim3(256),
                ZeroArgs{s->filt.as<uint32_t>(), words, s->gate}, ST_CLAP_A);
+        PS2_LAUNCHED();
+        return PS2_OK;
+    }
+    if (s->gclap) {   // any other window than 24 of 32: the generic kernels
+        GClapArgs<R> g{};
+        g.grid = s->grid.as<cplx<R>>();
+        g.gpart = s->gpart.as<cplx<R>>();
+        g.lp = s->lp.as<R>();
+        g.weight = s->gweight.as<double>();
+        g.ni = (int)s->ni;
+        g.nj = (int)s->nj;
+        g.I = (int)s->I;
+        g.nwi = s->nwi;
+        g.nwj = s->nwj;
+        g.n_win = s->n_win;
+        g.n = s->n_ex;
+        g.n_inc = s->n_inc;
+        g.alpha = (R)s->alpha;
+        g.beta = (R)s->beta;
+        const double sd = 6.0 / 5.0;  // gausswin(7, 2.5): std (7-1)/(2*2.5)
+        for (int k = 0; k < 7; ++k) {
+            const double x = (double)k - 3.0;
+            g.k7[k] = (R)std::exp(-0.5 * (x / sd) * (x / sd));
+        }
+        g.gate = s->gate;
+        const bool one = std::fabs(s->alpha - 1.0) <= 2.220446049250313e-16;
+        const dim3 gw((unsigned)((int64_t)s->nwi * s->nwj * s->I));
+        {
+            SlotTimer tm(s, SLOT_CLAP_STRIP);
+            auto win = [&](auto nmax) {
+                constexpr int NM = decltype(nmax)::value;
+                if (one)
+                    kgo<M>(s, k_gclap_window<M, R, NM, true>, gw, dim3(256), g, ST_CLAP_A);
+                else
+                    kgo<M>(s, k_gclap_window<M, R, NM, false>, gw, dim3(256), g, ST_CLAP_A);
+            };
+            if (s->n_ex <= 16)
+                win(std::integral_constant<int, 16>{});
+            else if (s->n_ex <= 32)
+                win(std::integral_constant<int, 32>{});
+            else
+                win(std::integral_constant<int, 64>{});
+            PS2_LAUNCHED();
+        }
+        SlotTimer tm(s, SLOT_CLAP_COMBINE);
+        const GCombineArgs<R> ca{g, s->filt.as<cplx<R>>()};
+        const dim3 gc(blocks_for(n, 256));
+        if (unit)
+            kgo<M>(s, k_gclap_combine<M, R, true>, gc, dim3(256), ca, ST_CLAP_B);
+        else
+            kgo<M>(s, k_gclap_combine<M, R, false>, gc, dim3(256), ca, ST_CLAP_B);
         PS2_LAUNCHED();
         return PS2_OK;
     }
@@ -1179,7 +1269,7 @@ int autotune_tiles(ps2_session *s, const char *path) {
     if (hit.fill_cb) {
         s->fill_cb = hit.fill_cb == 64 ? 64 : 128;
         if (scan_choice) s->scan_table = hit.scan_table ? 1 : 0;
-        if (s->windows && (rc = apply_segments<R>(s, hit.nseg))) return rc;
+        if (s->windows && !s->gclap && (rc = apply_segments<R>(s, hit.nseg))) return rc;
         s->tile_source = PS2_TILES_CACHE;
     } else {
         // measured on this session's inputs from the initial state (K = 0,
@@ -1216,7 +1306,7 @@ int autotune_tiles(ps2_session *s, const char *path) {
             if (best < 0.0f || ms < best) best = ms, best_cb = cb;
         }
         s->fill_cb = best_cb;
-        if (s->windows) {
+        if (s->windows && !s->gclap) {
             const int ngroups = (int)((s->I + CLAP_TEAMS - 1) / CLAP_TEAMS);
             std::vector<ClapSegs> cand = clap_segment_candidates(s->nwi, s->nwj, ngroups, (int)s->I, s->clap_slots);
             if (cand.size() > 4) cand.resize(4);
@@ -1230,6 +1320,8 @@ int autotune_tiles(ps2_session *s, const char *path) {
             }
             s->segs = best_seg;
             if ((rc = launch_clap<R>(s, true))) return done(rc);   // the filtered grid the scan reads
+        } else if (s->windows) {
+            if ((rc = launch_clap<R>(s, true))) return done(rc);   // (any other window: no tiles)
         }
         if (scan_choice) {
             best = -1.0f;
@@ -1248,7 +1340,7 @@ int autotune_tiles(ps2_session *s, const char *path) {
             TileLine t;
             t.fill_cb = s->fill_cb;
             t.scan_table = s->scan_table;
-            t.nseg = s->windows ? s->segs.nseg : 0;
+            t.nseg = s->windows && !s->gclap ? s->segs.nseg : 0;
             if ((rc = tile_cache(path, key, nullptr, &t))) return rc;
         }
     }
@@ -1770,13 +1862,17 @@ int session_create(const ps2_patch_in *in, const ps2_run_opts *opts, ps2_session
     s->nr_base.assign(in->nr_base, in->nr_base + PS2_COH_BINS);
     s->hist_rule = make_hist_rule(in->coh_bins, PS2_COH_BINS);
     // window rows/columns (native_stage2.rs:754-759)
-    s->nwi = (int)std::ceil((double)s->ni / (double)CLAP_INC) - 3;
-    s->nwj = (int)std::ceil((double)s->nj / (double)CLAP_INC) - 3;
+    s->n_win = (int)in->clap_window;
+    s->n_ex = (int)in->n_win_ex;
+    s->n_inc = std::max(1, s->n_win / 4);
+    s->gclap = s->n_win != CLAP_WIN || s->n_ex != CLAP_FFT;
+    s->nwi = (int)std::ceil((double)s->ni / (double)s->n_inc) - 3;
+    s->nwj = (int)std::ceil((double)s->nj / (double)s->n_inc) - 3;
     s->windows = s->nwi > 0 && s->nwj > 0;
-    if (s->windows && (s->ni < CLAP_WIN || s->nj < CLAP_WIN))
+    if (s->windows && (s->ni < s->n_win || s->nj < s->n_win))
         return arg_error("grid " + std::to_string(s->ni) + "x" + std::to_string(s->nj) +
-                         " has CLAP windows but is narrower than the 24-cell window (the reference's "
-                         "window placement underflows here)");
+                         " has CLAP windows but is narrower than the " + std::to_string(s->n_win) +
+                         "-cell window (the reference's window placement underflows here)");
     // derived FP32 re-score bound: the scan's worst-case rounding error on one
     // coherence, doubled (recurrence drift over one trial chunk of 8-48 trials
     // + an I-term f32 sum + the sincos anchors), in units of 2^-24
@@ -1923,8 +2019,8 @@ int ps2_hip_session_tiles(ps2_session *s, ps2_tiles *out) {
     std::lock_guard<std::mutex> lk(s->mu);
     out->fill_cells = s->form == PS2_FORM_FP32 ? s->fill_cb : 1;
     out->scan_step_table = scan_step_table(s) ? 1 : 0;
-    out->clap_segments = s->windows ? s->segs.nseg : 0;
-    out->clap_windows_per_segment = s->windows ? s->segs.wseg : 0;
+    out->clap_segments = s->windows && !s->gclap ? s->segs.nseg : 0;
+    out->clap_windows_per_segment = s->windows && !s->gclap ? s->segs.wseg : 0;
     out->source = s->tile_source;
     return PS2_OK;
 }

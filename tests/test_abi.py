@@ -74,7 +74,10 @@ def test_header_structs_match_ctypes_layout():
     (lambda h: h.grid_lin.__setitem__(0, -1), "grid_lin[0] outside"),
     (lambda h: h.trial_values.__setitem__(1, 0.5), "consecutive integers"),
     (lambda h: setattr(h.s, "n_trial", 4), "2*tn+1"),
-    (lambda h: setattr(h.s, "clap_window", 20), "clap_win = 32"),
+    # CLAP windows: 1 <= clap_window <= n_win_ex <= 64 (any clap_win up to 64)
+    (lambda h: setattr(h.s, "clap_window", 33), "CLAP window outside"),
+    (lambda h: setattr(h.s, "clap_window", 0), "CLAP window outside"),
+    (lambda h: setattr(h.s, "n_win_ex", 65), "CLAP window outside"),
     (lambda h: (setattr(h.s, "row_invariant", 0), setattr(h.s, "bperp_mat", None)), "bperp_mat is required"),
     (lambda h: (setattr(h.s, "weighting", _native.WEIGHT_SNR), setattr(h.s, "amp", None)), "amp is required"),
     (lambda h: setattr(h.s, "d_a", None), "required input array is NULL"),

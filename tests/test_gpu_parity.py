@@ -820,6 +820,62 @@ def test_small_patch_fused_loop_equals_separate_steps(S, prep_small, form):
         assert np.array_equal(loop[key], fit[key], equal_nan=True), key
 
 
+# --- other CLAP windows (clap_win other than 32) -------------------------------
+
+def _prep_clap_win(ora, clap_win, n_ps=6000, n_ifg=16, seed=0x5EED1A40):
+    from pystamps_amd import synth
+    from pystamps_amd.prepare import prepare_stage2_inputs
+    cfg = dataclasses.replace(synth.scaled(synth.CONFIGS[1], n_ps, n_ifg), seed=seed)
+    raw = synth.generate(cfg)
+    raw.parms = dict(raw.parms, clap_win=clap_win)
+    prep = prepare_stage2_inputs(raw, nr_rows=10_000,
+                                 nr_fn=lambda bp, tv, cb, n: ora.random_coherence_histogram(bp, tv, cb, n_rows=n))
+    assert prep.low_pass.shape[0] == int(np.floor(clap_win + 0.5))
+    return prep
+
+
+@pytest.mark.parametrize("clap_win", [16.0, 22.0, 27.0, 64.0])
+def test_clap_other_windows_vs_oracle(S, ora, clap_win):
+    """clap_win 16 (12 of 16), 22 (an odd 17-cell window in 22 points), 27
+    (20 of 27, an odd transform) and 64 (48 of 64) run the generic CLAP
+    kernels: FP64 within 1e-12 of ref64's largest value, FP32 within 1e-6 of
+    ref32's."""
+    prep = _prep_clap_win(ora, clap_win)
+    g = _grid_input(prep)
+    with S.Stage2Session(prep, form="fp64") as s:
+        f64 = s.clap(g)
+    r64 = ora.clap_filter_grid_stack(prep, g, mode=ora.REF64)
+    scale = np.abs(r64).max()
+    assert np.array_equal(np.isnan(f64), np.isnan(r64))
+    np.testing.assert_allclose(f64, r64, rtol=0, atol=1e-12 * scale)
+    g32 = g.astype(np.complex64)
+    with S.Stage2Session(prep, form="fp32") as s:
+        f32 = s.clap(g32)
+    r32 = ora.clap_filter_grid_stack(prep, g32, mode=ora.REF32)
+    err = np.abs(f32 - r32).max() / np.abs(r32).max()
+    print(f"clap_win {clap_win}: fp32 max|diff| / max|ref32| = {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_loop_other_window_vs_ref64(S, ora):
+    """A whole loop with clap_win 22 (an odd 17-cell window): the FP64 form
+    equals ref64 (iterations and trial indices exact, coherence to 1e-9); the
+    FP32 form takes the same iterations and trial indices on all but
+    near-tied rows, coherence within 1e-5 where the trials agree."""
+    prep = _prep_clap_win(ora, 22.0, n_ps=8000, n_ifg=20, seed=0x5EED1A41)
+    r = ora.run(prep, mode=ora.REF64)
+    g = S.run_stage2_hip(prep, form="fp64")
+    assert g["i_loop"] == r["i_loop"]
+    assert np.array_equal(g["trial_ix"], r["trial_ix"])
+    np.testing.assert_allclose(g["coh_ps"], r["coh_ps"], rtol=0, atol=1e-9)
+    g32 = S.run_stage2_hip(prep, form="fp32")
+    assert g32["i_loop"] == r["i_loop"]
+    agree = g32["trial_ix"] == r["trial_ix"]
+    print("clap_win 22 fp32 loop: trial agreement", agree.mean(), "mismatches", int((~agree).sum()))
+    assert agree.mean() >= 0.999
+    np.testing.assert_allclose(g32["coh_ps"][agree], r["coh_ps"][agree], rtol=0, atol=1e-5)
+
+
 def test_fp32_f64_baselines_split_and_plane_agree(S, prep_small):
     """The FP32 form keeps the f64 baselines of its FP64 re-score as the f32
     bulk value plus an int32 residual (exact for every value whose f32 image

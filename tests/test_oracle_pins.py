@@ -28,56 +28,65 @@ N_WIN, N_EX, N_INC = 24, 32, 6
 
 # --- library restatement of one interferogram's CLAP -------------------------
 
-def _window_weight():
+def _window_weight(n_win=N_WIN):
     # clap_window_weight (native_stage2.rs:1014-1037): quadrant value
-    # row + col + 1e-6, mirrored into the other three quadrants
-    r = np.arange(N_WIN)
-    m = np.minimum(r, N_WIN - 1 - r).astype(np.float64)
-    return m[:, None] + m[None, :] + 1e-6
+    # row + col + 1e-6 on a (n_win // 2)-square quadrant, mirrored into the
+    # other three; an odd window's last row and column stay 0
+    h2 = 2 * (n_win // 2)
+    r = np.arange(h2)
+    m = np.minimum(r, h2 - 1 - r).astype(np.float64)
+    w = np.zeros((n_win, n_win))
+    w[:h2, :h2] = m[:, None] + m[None, :] + 1e-6
+    return w
 
 
-def _windows(n_i, n_j):
-    # clap_windows (native_stage2.rs:957-995): stride 6, a window past the
-    # grid edge is moved back inside and remembers the shift
-    nwi = math.ceil(n_i / N_INC) - 3
-    nwj = math.ceil(n_j / N_INC) - 3
+def _windows(n_i, n_j, n_win=N_WIN):
+    # clap_windows (native_stage2.rs:957-995): stride max(n_win // 4, 1), a
+    # window past the grid edge is moved back inside and remembers the shift
+    n_inc = max(n_win // 4, 1)
+    nwi = math.ceil(n_i / n_inc) - 3
+    nwj = math.ceil(n_j / n_inc) - 3
     out = []
     for a in range(max(nwi, 0)):
-        i1, rs = a * N_INC, 0
-        if i1 + N_WIN > n_i:
-            rs, i1 = i1 + N_WIN - n_i, n_i - N_WIN
+        i1, rs = a * n_inc, 0
+        if i1 + n_win > n_i:
+            rs, i1 = i1 + n_win - n_i, n_i - n_win
         for b in range(max(nwj, 0)):
-            j1, cs = b * N_INC, 0
-            if j1 + N_WIN > n_j:
-                cs, j1 = j1 + N_WIN - n_j, n_j - N_WIN
+            j1, cs = b * n_inc, 0
+            if j1 + n_win > n_j:
+                cs, j1 = j1 + n_win - n_j, n_j - n_win
             out.append((i1, j1, rs, cs))
     return out if nwi > 0 and nwj > 0 else []
 
 
-def clap_plane_by_library(plane, low_pass, alpha, beta):
+def clap_plane_by_library(plane, low_pass, alpha, beta, n_win=N_WIN):
     """One interferogram plane [n_i, n_j] complex128 through CLAP, built from
-    numpy / scipy routines."""
+    numpy / scipy routines; window n_win, transform size = low_pass's."""
     n_i, n_j = plane.shape
+    n_ex = low_pass.shape[0]
     g7 = scipy.signal.windows.gaussian(7, std=(7 - 1) / (2 * 2.5))   # gausswin(7, 2.5)
     k2 = np.outer(g7, g7)
-    base_w = _window_weight()
+    base_w = _window_weight(n_win)
     acc = np.zeros((n_i, n_j), np.complex128)
-    for i1, j1, rs, cs in _windows(n_i, n_j):
-        blk = plane[i1:i1 + N_WIN, j1:j1 + N_WIN]
+    for i1, j1, rs, cs in _windows(n_i, n_j, n_win):
+        blk = plane[i1:i1 + n_win, j1:j1 + n_win]
         blk = np.where(np.isnan(blk.real) | np.isnan(blk.imag), 0.0, blk)
-        x = np.zeros((N_EX, N_EX), np.complex128)
-        x[:N_WIN, :N_WIN] = blk
+        x = np.zeros((n_ex, n_ex), np.complex128)
+        x[:n_win, :n_win] = blk
         F = np.fft.fft2(x)
-        sm = scipy.signal.convolve2d(np.fft.fftshift(np.abs(F)), k2, mode="same",
+        # the reference's shifts (shift_real_into, native_stage2.rs:1097-1120)
+        # read at offset floor(n / 2) (fftshift) and ceil(n / 2) (ifftshift):
+        # numpy's fftshift / ifftshift for an even n, swapped for an odd one
+        sm = scipy.signal.convolve2d(np.roll(np.abs(F), -(n_ex // 2), axis=(0, 1)), k2, mode="same",
                                      boundary="fill", fillvalue=0.0)
-        H = np.fft.ifftshift(sm)
+        H = np.roll(sm, -((n_ex + 1) // 2), axis=(0, 1))
         med = np.median(H)
         Hn = H / med if med != 0.0 else H
         gain = np.maximum(Hn ** alpha - 1.0, 0.0) * beta + low_pass
-        y = np.fft.ifft2(F * gain)[:N_WIN, :N_WIN]      # ifft2 carries the 1/32^2
-        w = np.zeros((N_WIN, N_WIN))
-        w[rs:, cs:] = base_w[:N_WIN - rs, :N_WIN - cs]
-        acc[i1:i1 + N_WIN, j1:j1 + N_WIN] += y * w
+        y = np.fft.ifft2(F * gain)[:n_win, :n_win]      # ifft2 carries the 1/n_ex^2
+        w = np.zeros((n_win, n_win))
+        w[rs:, cs:] = base_w[:n_win - rs, :n_win - cs]
+        acc[i1:i1 + n_win, j1:j1 + n_win] += y * w
     return acc
 
 
@@ -328,3 +337,26 @@ def test_planted_mistakes_fail_their_pins(ora, tmp_path):
             escaped.append((m, pin))
     assert not escaped, f"planted mistakes not caught: {escaped}"
     PINS["clap_gain_one_window"](ora)   # the default library again
+
+
+@pytest.mark.parametrize("clap_win", [16.0, 22.0, 27.0, 64.0])
+def test_oracle_clap_other_windows(ora, clap_win):
+    """The oracle's CLAP for windows other than 24 of 32 (clap_win 16: 12 of 16;
+    22: an odd 17-cell window in a 22-point transform; 27: 20 of 27, an odd
+    transform, where the reference's shifts are numpy's swapped; 64: 48 of
+    64) against the library composition, with the StaMPS low-pass of that
+    size, clamped edge windows and NaN cells."""
+    from pystamps_amd.prepare import Stage2Options, build_low_pass
+    o = Stage2Options()
+    o.clap_win = clap_win
+    lp = build_low_pass(o)
+    n_win = int(math.floor(clap_win * 0.75 + 0.5))
+    n_i, n_j = n_win + 11, n_win + 7
+    prep = dataclasses.replace(_prep(n_i, n_j, low_pass=lp, alpha=0.8), clap_window=n_win)
+    g = _random_grid(n_i, n_j, 2, seed=31, nan_cells=6)
+    ref = np.stack([clap_plane_by_library(g[:, f].reshape(n_i, n_j), lp, 0.8, 0.3, n_win).reshape(-1)
+                    for f in range(2)], axis=1)
+    out = ora.clap_filter_grid_stack(prep, g, mode=ora.REF64)
+    scale = np.abs(ref).max()
+    assert scale > 0 and len(_windows(n_i, n_j, n_win)) >= 4
+    assert np.abs(out - ref).max() / scale < 1e-12
