@@ -68,8 +68,8 @@ typedef struct {
     const int64_t *grid_lin;  /* [n_ps] cell = (i-1)*n_j + (j-1), each < n_i*n_j */
     const double *d_a;        /* [n_ps] amplitude dispersion */
     const double *low_pass;   /* [n_win_ex][n_win_ex] */
-    int64_t n_win_ex;         /* low_pass dimension = FFT size (clap_win) */
-    int64_t clap_window;      /* round(0.75 * clap_win) */
+    int64_t n_win_ex;         /* low_pass dimension = FFT size (clap_win), at most 64 */
+    int64_t clap_window;      /* round(0.75 * clap_win), 1 .. n_win_ex */
     double clap_alpha, clap_beta;
     const double *trial_values; /* [n_trial] */
     int64_t n_trial;
