@@ -155,6 +155,11 @@ pub struct Ps2Session {
     _private: [u8; 0],
 }
 
+/// `ps2_iteration_fn`: called after every gamma iteration of
+/// `ps2_hip_session_run_iterations` with the state in the caller's buffers;
+/// a non-zero return ends the loop with that code.
+pub type Ps2IterationFn = Option<unsafe extern "C" fn(user: *mut c_void, iteration: i64, state: *mut Ps2PatchOut) -> c_int>;
+
 extern "C" {
     pub fn ps2_hip_abi_version() -> c_int;
     pub fn ps2_hip_last_error() -> *const c_char;
@@ -193,6 +198,8 @@ extern "C" {
                                   out: *mut *mut Ps2Session) -> c_int;
     pub fn ps2_hip_session_run(s: *mut Ps2Session, out_i_loop: *mut i64, out_loop_seconds: *mut f64) -> c_int;
     pub fn ps2_hip_session_download(s: *mut Ps2Session, out: *mut Ps2PatchOut) -> c_int;
+    pub fn ps2_hip_session_run_iterations(s: *mut Ps2Session, state: *mut Ps2PatchOut, fn_: Ps2IterationFn,
+                                          user: *mut c_void, out_i_loop: *mut i64) -> c_int;
     pub fn ps2_hip_session_destroy(s: *mut Ps2Session);
     pub fn ps2_hip_sessions_run(n: usize, sessions: *const *mut Ps2Session, i_loops: *mut i64,
                                 seconds: *mut f64) -> c_int;
@@ -271,6 +278,25 @@ impl Session {
 
     pub fn download(&mut self, out: &mut Ps2PatchOut) -> Result<(), String> {
         check(unsafe { ps2_hip_session_download(self.raw, out) })
+    }
+
+    /// The loop with the state after every iteration written into `state`
+    /// and passed to `on_iteration(iteration, state)` (the
+    /// PYSTAMPS_STAGE2_NATIVE_DEBUG_PM snapshots); returns the iteration count.
+    pub fn run_iterations<F: FnMut(i64, &Ps2PatchOut)>(&mut self, state: &mut Ps2PatchOut, mut on_iteration: F)
+                                                       -> Result<i64, String> {
+        unsafe extern "C" fn tramp<F: FnMut(i64, &Ps2PatchOut)>(user: *mut c_void, it: i64,
+                                                                 st: *mut Ps2PatchOut) -> c_int {
+            let f = &mut *(user as *mut F);
+            f(it, &*st);
+            0
+        }
+        let mut i_loop = 0i64;
+        check(unsafe {
+            ps2_hip_session_run_iterations(self.raw, state, Some(tramp::<F>),
+                                           &mut on_iteration as *mut F as *mut c_void, &mut i_loop)
+        })?;
+        Ok(i_loop)
     }
 
     pub fn tiles(&mut self) -> Result<Ps2Tiles, String> {

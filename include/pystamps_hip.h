@@ -280,6 +280,19 @@ int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_se
 int ps2_hip_session_download(ps2_session *s, ps2_patch_out *out);
 void ps2_hip_session_destroy(ps2_session *s);
 
+/* The loop with the state after every gamma iteration: the iterations are
+ * launched one at a time; after each, the outputs of that state are written
+ * into `state` (the caller's buffers, as ps2_hip_session_download fills them:
+ * K_ps .. ph_weight, Nr as the iteration's control step left it,
+ * gamma_change_save, i_loop = the iteration) and fn(user, iteration, state)
+ * is called on the calling thread; a non-zero return ends the loop with that
+ * code.  The last call holds the loop's final state.  Replaces the reference's
+ * PYSTAMPS_STAGE2_NATIVE_DEBUG_PM snapshots, pm1_iter_NN.mat written after
+ * every iteration (native_stage2.rs:259, :337-356, :385-404). */
+typedef int (*ps2_iteration_fn)(void *user, int64_t iteration, ps2_patch_out *state);
+int ps2_hip_session_run_iterations(ps2_session *s, ps2_patch_out *state, ps2_iteration_fn fn, void *user,
+                                   int64_t *out_i_loop);
+
 /* Runs the loops of n sessions concurrently (each on its own stream; up to 16
  * host workers, so at most 16 loops are in flight; sessions may be on the
  * same device) and returns after every loop's stop decision.  i_loops[k] / seconds[k] receive each session's iteration

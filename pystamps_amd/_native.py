@@ -89,6 +89,10 @@ def declared_functions() -> list[str]:
     return sorted(set(re.findall(r"\b(ps2_hip_\w+)\s*\(", text)))
 
 
+# ps2_iteration_fn: int (*)(void *user, int64_t iteration, ps2_patch_out *state)
+ITERATION_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(PatchOut))
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is not None:
@@ -124,6 +128,8 @@ def lib() -> C.CDLL:
     L.ps2_hip_session_create.argtypes = [C.POINTER(PatchIn), C.POINTER(RunOpts), C.POINTER(C.c_void_p)]
     L.ps2_hip_session_run.argtypes = [C.c_void_p, _i64p, _f64p]
     L.ps2_hip_session_download.argtypes = [C.c_void_p, C.POINTER(PatchOut)]
+    L.ps2_hip_session_run_iterations.argtypes = [C.c_void_p, C.POINTER(PatchOut), ITERATION_FN, C.c_void_p,
+                                                 _i64p]
     L.ps2_hip_session_destroy.argtypes = [C.c_void_p]
     L.ps2_hip_session_destroy.restype = None
     L.ps2_hip_step_fill_grid.argtypes = [C.c_void_p, _f64p, _f64p, C.c_void_p]

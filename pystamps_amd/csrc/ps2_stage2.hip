@@ -1485,6 +1485,48 @@ int run_loop(ps2_session *s) {
     return PS2_OK;
 }
 
+// The loop launched an iteration at a time, the state after each written in
+// the caller's layouts and handed to fn (ps2_hip_session_run_iterations).
+int download_outputs(ps2_session *s, ps2_patch_out *out, bool loop_outputs);
+
+template <typename R>
+int run_loop_iterations(ps2_session *s, ps2_patch_out *state, ps2_iteration_fn fn, void *user) {
+    int rc = PS2_OK;
+    LoopCtl *hc = s->h_ctl.as<LoopCtl>();
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = loop_prepare(s, s->stream))) return rc;
+    const int32_t *gate = &s->ctl.as<LoopCtl>()->stopped;
+    for (int64_t it = 0; it < s->ctl_params.max_iter && rc == PS2_OK; ++it) {
+        s->gate = gate;
+        if ((rc = launch_iteration<R>(s))) break;
+        s->gate = nullptr;   // the state's outputs, also after the stop decision
+        if ((rc = launch_outputs<R>(s, true))) break;
+        PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
+        PS2_HIP(hipStreamSynchronize(s->stream));
+        s->i_loop = it + 1;
+        s->gamma_change_save = hc->gcs;
+        s->nr.assign(hc->nr, hc->nr + CTL_BINS);
+        s->n_rescored = (int64_t)hc->rescored;
+        s->n_rechecked = (int64_t)hc->rechecked;
+        s->ran = true;
+        s->loop_outputs = true;
+        if ((rc = download_outputs(s, state, true))) break;
+        if (fn) {
+            const int u = fn(user, it + 1, state);
+            if (u) {
+                set_error("iteration callback returned " + std::to_string(u));
+                rc = u;
+                break;
+            }
+        }
+        if (hc->stopped) break;
+    }
+    s->gate = nullptr;
+    if (rc) return rc;
+    loop_finish(s, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    return PS2_OK;
+}
+
 // download: the outputs are already in the caller's layouts on the device
 // (k_final_outputs); each goes down in one bulk copy.
 
@@ -1969,6 +2011,18 @@ int ps2_hip_session_run(ps2_session *s, int64_t *out_i_loop, double *out_loop_se
     if (rc) return rc;
     if (out_i_loop) *out_i_loop = s->i_loop;
     if (out_loop_seconds) *out_loop_seconds = s->loop_seconds;
+    return PS2_OK;
+}
+
+int ps2_hip_session_run_iterations(ps2_session *s, ps2_patch_out *state, ps2_iteration_fn fn, void *user,
+                                   int64_t *out_i_loop) {
+    if (!s || !state) return arg_error("session and state are required");
+    std::lock_guard<std::mutex> lk(s->mu);
+    PS2_HIP(hipSetDevice(s->device));
+    const int rc = s->form == PS2_FORM_FP64 ? run_loop_iterations<double>(s, state, fn, user)
+                                             : run_loop_iterations<float>(s, state, fn, user);
+    if (rc) return rc;
+    if (out_i_loop) *out_i_loop = s->i_loop;
     return PS2_OK;
 }
 

@@ -202,12 +202,20 @@ def run_stage2_patch_dir(patch_dir, form: str = "fp32", device: int = 0,
     """Drop-in for run_stage2_native(patch_dir): read the patch, prepare (the
     Nr histogram on the GPU), run the gamma loop on the GPU, write pm1.mat, and
     return the reference's summary line with the backend named."""
-    from .stage2 import run_stage2_hip
+    import os
+    from .stage2 import Stage2Session, run_stage2_hip
     patch_dir = Path(patch_dir)
     raw = read_patch(patch_dir)
     kw = {} if nr_rows is None else {"nr_rows": nr_rows}
     prep = prepare_stage2_inputs(raw, device=device, **kw)
-    pm1 = run_stage2_hip(prep, form=form, device=device)
+    if os.environ.get("PYSTAMPS_STAGE2_NATIVE_DEBUG_PM") is not None:
+        # the state after every iteration as pm1_iter_NN.mat, as the
+        # reference's loop writes them (native_stage2.rs:259, :337-404)
+        with Stage2Session(prep, form=form, device=device) as s:
+            s.run_iterations(lambda it, st: write_pm1(patch_dir, prep, st, f"pm1_iter_{it:02d}.mat"))
+            pm1 = s.download()
+    else:
+        pm1 = run_stage2_hip(prep, form=form, device=device)
     write_pm1(patch_dir, prep, pm1)
     return (f"Stage 2 computed coherence for {prep.n_ps} candidates in {pm1['i_loop']} iterations "
             f"using the hip backend ({form} form)")

@@ -232,6 +232,32 @@ class Stage2Session:
         _check(self.L.ps2_hip_session_kernel_times(self.h, _ptr(ms, _f64p), _ptr(cnt, _i64p)))
         return {name: (float(ms[k]), int(cnt[k])) for k, name in enumerate(_native.KERNEL_SLOTS)}
 
+    def run_iterations(self, on_iteration) -> int:
+        """The loop with the state after every iteration:
+        on_iteration(iteration, pm1_dict) is called after each gamma iteration
+        with that state (the pm1.mat content at that point; the arrays are
+        reused by the next call) -- the reference's
+        PYSTAMPS_STAGE2_NATIVE_DEBUG_PM snapshots.  Returns the iteration count."""
+        out = _OutHolder(self.prep)
+        err = []
+
+        def cb(user, iteration, state):
+            try:
+                out.s = state.contents
+                on_iteration(int(iteration), out.finish(self.prep))
+                return 0
+            except Exception as e:   # reported after the call returns
+                err.append(e)
+                return 1
+
+        fn = _native.ITERATION_FN(cb)
+        il = C.c_int64(0)
+        rc = self.L.ps2_hip_session_run_iterations(self.h, C.byref(out.s), fn, None, C.byref(il))
+        if err:
+            raise err[0]
+        _check(rc)
+        return int(il.value)
+
     def download(self) -> dict:
         out = _OutHolder(self.prep)
         _check(self.L.ps2_hip_session_download(self.h, C.byref(out.s)))

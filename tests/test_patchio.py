@@ -162,3 +162,36 @@ def test_run_stage2_patch_dir_matches_in_memory(tmp_path):
     assert np.array_equal(m["K_ps"][:, 0], o["K_ps"], equal_nan=True)
     assert np.array_equal(m["coh_ps"][:, 0], o["coh_ps"])
     assert np.array_equal(m["ph_grid"], o["ph_grid"].reshape(prep.n_i, prep.n_j, prep.n_ifg))
+
+
+@pytest.mark.gpu
+def test_debug_pm_iteration_snapshots(tmp_path, monkeypatch):
+    """With PYSTAMPS_STAGE2_NATIVE_DEBUG_PM set, run_stage2_patch_dir also
+    writes pm1_iter_NN.mat after every iteration (native_stage2.rs:259,
+    :337-404): one per iteration, the last equal to pm1.mat, the first equal
+    to the pm1.mat of a one-iteration loop."""
+    import dataclasses
+    from pystamps_amd.patchio import read_patch, run_stage2_patch_dir, write_patch
+    from pystamps_amd.prepare import prepare_stage2_inputs
+    from pystamps_amd.stage2 import run_stage2_hip
+    raw = _synthetic_raw(n_ps=4000, n_ifg=14, seed=0x5EED1402)
+    d = write_patch(tmp_path / "PATCH_1", raw)
+    monkeypatch.setenv("PYSTAMPS_STAGE2_NATIVE_DEBUG_PM", "1")
+    run_stage2_patch_dir(d, form="fp32", nr_rows=20_000)
+    final = sio.loadmat(d / "pm1.mat")
+    n = int(final["i_loop"][0, 0])
+    assert n >= 2
+    snaps = sorted(p.name for p in d.glob("pm1_iter_*.mat"))
+    assert snaps == [f"pm1_iter_{k:02d}.mat" for k in range(1, n + 1)]
+    last = sio.loadmat(d / f"pm1_iter_{n:02d}.mat")
+    for key in ("K_ps", "C_ps", "coh_ps", "N_opt", "coh_ps_save", "ph_res", "ph_patch", "ph_weight", "ph_grid",
+                "Nr", "gamma_change_save", "i_loop"):
+        assert np.array_equal(last[key], final[key], equal_nan=True), key
+    prep = prepare_stage2_inputs(read_patch(d), nr_rows=20_000)
+    one = run_stage2_hip(dataclasses.replace(prep, gamma_max_iterations=1), form="fp32")
+    first = sio.loadmat(d / "pm1_iter_01.mat")
+    assert float(first["i_loop"][0, 0]) == 1.0
+    for key in ("K_ps", "coh_ps", "coh_ps_save"):
+        assert np.array_equal(first[key][:, 0], one[key], equal_nan=True), key
+    assert np.array_equal(first["ph_weight"], one["ph_weight"])
+    assert float(first["gamma_change_save"][0, 0]) == one["gamma_change_save"]

@@ -22,7 +22,7 @@ STRUCTS = {"ps2_patch_in": "Ps2PatchIn", "ps2_patch_out": "Ps2PatchOut", "ps2_ru
            "ps2_tiles": "Ps2Tiles", "ps2_device_outputs": "Ps2DeviceOutputs"}
 SCALARS = {"int64_t": "i64", "int32_t": "i32", "uint32_t": "u32", "double": "f64", "float": "f32",
            "size_t": "usize", "int": "c_int", "char": "c_char", "void": "c_void",
-           "ps2_session": "Ps2Session", **STRUCTS}
+           "ps2_session": "Ps2Session", "ps2_iteration_fn": "Ps2IterationFn", **STRUCTS}
 RUST_SIZE = {"i64": 8, "i32": 4, "u32": 4, "f64": 8, "f32": 4, "usize": 8, "c_int": 4}
 
 
@@ -162,11 +162,28 @@ def test_repr_c_offsets_equal_gcc_offsetof(tmp_path):
 
 def test_extern_functions_mirror_the_header():
     h, r = header_functions(), rust_functions()
-    assert len(h) == 26
+    assert len(h) == 27
     assert set(h) == set(r), (set(h) ^ set(r))
     for name, (ret, params) in h.items():
         assert r[name][1] == params, (name, params, r[name][1])
         assert r[name][0] == ret, (name, ret, r[name][0])
+
+
+def test_callback_type_mirrors_the_header():
+    # typedef int (*ps2_iteration_fn)(void *user, int64_t iteration, ps2_patch_out *state);
+    text = _strip_c_comments(HEADER.read_text())
+    m = re.search(r"typedef\s+(\w+)\s*\(\s*\*\s*ps2_iteration_fn\s*\)\s*\(([^)]*)\)\s*;", text)
+    assert m, "ps2_iteration_fn typedef"
+    c_ret = _c_decl_to_rust(m.group(1), 0, False)
+    c_params = []
+    for a in m.group(2).split(","):
+        pm = re.match(r"\s*(const\s+)?(\w+)\s*(\**)\s*\w+\s*$", a)
+        c_params.append(_c_decl_to_rust(pm.group(2), pm.group(3).count("*"), bool(pm.group(1))))
+    rs = LIB_RS.read_text()
+    rm = re.search(r"pub type Ps2IterationFn = Option<unsafe extern \"C\" fn\((.*?)\) -> (\w+)>;", rs)
+    assert rm, "Ps2IterationFn alias"
+    r_params = [a.split(":", 1)[1].strip() for a in rm.group(1).split(",")]
+    assert (rm.group(2), r_params) == (c_ret, c_params)
 
 
 def test_constants_and_abi_version():
