@@ -878,9 +878,11 @@ __device__ __forceinline__ void clap_combine_body(
                     const int sg = s0 + t;
                     ok[q][t] = ix <= ix_hi && sg <= s1;
                     const int plane = MAXSEG == 2 ? (sg & 1) : sg % nplanes;
-                    // (an address inside the planes also where the entry is not used)
-                    const int64_t at =
-                        ok[q][t] ? ((((int64_t)ix * nplanes + plane) * (CLAP_WIN / CLAP_INC) + lr / CLAP_INC) * nj + c) *
+                    // (an address inside the planes also where the entry is not used;
+                    // one interferogram's planes hold fewer than 2^31 entries, a
+                    // host-checked size, so the offset is 32-bit arithmetic)
+                    const int at =
+                        ok[q][t] ? (((ix * nplanes + plane) * (CLAP_WIN / CLAP_INC) + lr / CLAP_INC) * nj + c) *
                                            CLAP_INC + r6
                                  : 0;
                     v[q][t] = ok[q][t] ? pf[at] : cplx<R>{(R)0, (R)0};

@@ -664,8 +664,12 @@ int alloc_state(ps2_session *s) {
             &per_cu, reinterpret_cast<const void *>(&k_clap_strip<false, R, CLAP_TEAMS, true>), 32 * CLAP_TEAMS, 0));
         s->clap_slots = std::max(1, n_cu) * std::max(1, per_cu);
         s->segs = clap_segments(s->nwi, s->nwj, (int)((I + CLAP_TEAMS - 1) / CLAP_TEAMS), (int)I, s->clap_slots);
-        // the autotuner tries segmentations with up to 4 planes
+        // the autotuner tries segmentations with up to 4 planes; one
+        // interferogram's planes stay below 2^31 entries (the combine's
+        // offsets are 32-bit)
         const int planes = s->autotune ? 4 : s->segs.nplanes;
+        if ((double)s->nwi * 4.0 * CLAP_WIN * (double)s->nj >= 2147483647.0)
+            return arg_error("grid too large for the CLAP partial planes (n_i x n_j x 16 must stay below 2^31)");
         if ((rc = s->part.alloc(sizeof(cplx<R>) * (size_t)s->nwi * planes * CLAP_WIN * s->nj * I)))
             return rc;
     }
