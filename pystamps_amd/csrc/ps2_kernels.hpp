@@ -1727,7 +1727,9 @@ __device__ __forceinline__ void scan_quad_body(
     uint32_t *__restrict__ rescore_count,
     FitParams prm,
     const int32_t *__restrict__ gate,
-    const uint32_t bid_) {
+    const uint32_t bid_,
+    int32_t *local_queue = nullptr,
+    int *local_count = nullptr) {
     using R = float;
     constexpr int TC = 8;
     if (loop_stopped(gate)) return;
@@ -1826,8 +1828,12 @@ __device__ __forceinline__ void scan_quad_body(
     tix[k] = best_ix;
     offset[k] = {bre, bim};
     if (prm.T > 1 && best - second < (R)prm.rescore_bound) {
-        const uint32_t slot = atomicAdd(rescore_count, 1u);
-        rescore_list[slot] = (int32_t)k;
+        if (local_queue) {   // the workgroup's own queue in LDS (k_fit_small<..., true>)
+            local_queue[atomicAdd(local_count, 1)] = (int32_t)k;
+        } else {
+            const uint32_t slot = atomicAdd(rescore_count, 1u);
+            rescore_list[slot] = (int32_t)k;
+        }
     }
 }
 
@@ -3149,6 +3155,7 @@ __device__ __forceinline__ void post_body(
     __shared__ int flag_tid[256];
     __shared__ double new_coh[256];
     if (threadIdx.x < 128) lh[threadIdx.x] = 0u;
+    __syncthreads();   // the bins are zero before any wavefront counts into them
     uint32_t n_rechecked = 0;
     const int64_t n_groups = (prm.P + GR - 1) / GR;
     for (int64_t g = bid_; g < n_groups; g += nblk_) {
@@ -3288,6 +3295,7 @@ struct FitTailArgs {
     double *K, *Cc, *coh, *nopt;
     const int32_t *rescore_list;
     uint32_t *counters;   // [0] re-score queue length, [1] rows refined again
+    const cplx<R> *wt;    // unit-step table of the scan (k_fit_small with the scan)
     FitParams prm;
     const double *mids;
     HistParams hp;
@@ -3330,8 +3338,13 @@ __global__ __launch_bounds__(256) void k_fit_tail(KArg<M, FitTailArgs<R>> p) {
 // one wavefront on 316 of the 1,024 SIMDs, each thread's dependent chain of
 // sincos/atan2 evaluations the bound; in 64-row groups it spreads over every
 // CU with a quarter of the chain per lane.
+// With SCAN (8-trial chunks, NQ as in k_scan_quad), the group's trial scan
+// runs first in the same workgroup (scan_quad_body, the same arithmetic as
+// k_scan_quad) and its near-tie rows are queued in LDS for the re-score: one
+// launch and no global queue per iteration; the queued-row count still goes to
+// counters[0] for the loop's statistics.
 constexpr int SMALL_GROUP = 64;
-template <bool M>
+template <bool M, bool SCAN = false, int NQ = 0>
 __global__ __launch_bounds__(256) void k_fit_small(KArg<M, FitTailArgs<float>> p) {
     FitTailArgs<float> a;
     uint32_t bid, nblk;
@@ -3344,8 +3357,25 @@ __global__ __launch_bounds__(256) void k_fit_small(KArg<M, FitTailArgs<float>> p
     }
     if (loop_stopped(a.gate)) return;
     const int64_t r0 = (int64_t)bid * SMALL_GROUP;
-    rescore_body<float>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.counters, a.tix, a.offset, a.prm,
-                        a.gate, r0, r0 + SMALL_GROUP, threadIdx.x >> 6, 4);
+    if constexpr (SCAN) {
+        __shared__ int32_t q_rows[SMALL_GROUP];
+        __shared__ int q_n;
+        if (threadIdx.x == 0) q_n = 0;
+        __syncthreads();
+        scan_quad_body<NQ>(a.phnm, a.bp, a.wt, a.filt, a.cell_of, a.tix, a.rng, a.offset, nullptr, nullptr, a.prm,
+                           a.gate, bid, q_rows, &q_n);
+        __syncthreads();
+        const int nq = q_n;
+        if (threadIdx.x == 0 && nq > 0) atomicAdd(&a.counters[0], (uint32_t)nq);
+        if (nq > 0) {
+            const uint32_t cnt = (uint32_t)nq;
+            rescore_body<float>(a.phnm, a.bp64, a.filt, a.cell_of, q_rows, &cnt, a.tix, a.offset, a.prm, a.gate,
+                                r0, r0 + SMALL_GROUP, threadIdx.x >> 6, 4);
+        }
+    } else {
+        rescore_body<float>(a.phnm, a.bp64, a.filt, a.cell_of, a.rescore_list, a.counters, a.tix, a.offset, a.prm,
+                            a.gate, r0, r0 + SMALL_GROUP, threadIdx.x >> 6, 4);
+    }
     __syncthreads();
     {
         const int64_t k = r0 + (threadIdx.x >> 2);

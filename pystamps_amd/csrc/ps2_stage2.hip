@@ -970,7 +970,9 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
                ST_SCAN);
         PS2_LAUNCHED();
     }
-    {
+    // the FP32 form's small patches in the loop scan inside k_fit_small (8-trial chunks)
+    const bool scan_in_fit = sizeof(R) == 4 && in_loop && small_patch(s) && scan_chunk(s) == 8;
+    if (!scan_in_fit) {
     SlotTimer tm(s, SLOT_SCAN);
     const ScanArgs<R> sa{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->wt.as<cplx<R>>(), s->filt.as<cplx<R>>(),
                          s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
@@ -1053,12 +1055,27 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
                                s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
                                s->offset.as<cplx<R>>(), s->K.as<double>(), s->Cc.as<double>(),
                                s->coh.as<double>(), s->nopt.as<double>(), s->rescore_list.as<int32_t>(),
-                               s->rescore_count.as<uint32_t>(), fp, ro.mids, ro.hp, rc, s->flags.as<uint8_t>(),
-                               ro, sizeof(R) == 4 ? 1 : 0, s->gate};
-        if constexpr (sizeof(R) == 4)
-            kgo<M>(s, k_fit_small<M>, dim3(blocks_for(s->P, SMALL_GROUP)), dim3(256), a, ST_REFINE);
-        else
+                               s->rescore_count.as<uint32_t>(), s->wt.as<cplx<R>>(), fp, ro.mids, ro.hp, rc,
+                               s->flags.as<uint8_t>(), ro, sizeof(R) == 4 ? 1 : 0, s->gate};
+        if constexpr (sizeof(R) == 4) {
+            const dim3 g(blocks_for(s->P, SMALL_GROUP));
+            if (scan_in_fit) {
+                switch (s->tn) {
+                case 1: kgo<M>(s, k_fit_small<M, true, 1>, g, dim3(256), a, ST_REFINE); break;
+                case 2: kgo<M>(s, k_fit_small<M, true, 2>, g, dim3(256), a, ST_REFINE); break;
+                case 3: kgo<M>(s, k_fit_small<M, true, 3>, g, dim3(256), a, ST_REFINE); break;
+                case 4: kgo<M>(s, k_fit_small<M, true, 4>, g, dim3(256), a, ST_REFINE); break;
+                case 5: kgo<M>(s, k_fit_small<M, true, 5>, g, dim3(256), a, ST_REFINE); break;
+                case 6: kgo<M>(s, k_fit_small<M, true, 6>, g, dim3(256), a, ST_REFINE); break;
+                case 7: kgo<M>(s, k_fit_small<M, true, 7>, g, dim3(256), a, ST_REFINE); break;
+                default: kgo<M>(s, k_fit_small<M, true, 0>, g, dim3(256), a, ST_REFINE); break;
+                }
+            } else {
+                kgo<M>(s, k_fit_small<M>, g, dim3(256), a, ST_REFINE);
+            }
+        } else {
             kgo<M>(s, k_fit_tail<M, R>, dim3(nb), dim3(256), a, ST_REFINE);
+        }
         PS2_LAUNCHED();
         return PS2_OK;
     }

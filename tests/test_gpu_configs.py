@@ -201,3 +201,25 @@ def test_nr_at_300000_rows(ora, config):
     assert prep.nr_base.sum() == 300_000
     assert np.array_equal(prep.nr_base, nr_o)
     assert prep.nr_max_nz_ix == mx_o
+
+
+def test_fp64_loop_repeats_bit_identical(S):
+    """The FP64 form's post step counts the coherence histogram in LDS: every
+    bin is zeroed before any wavefront counts into it, so repeated loops of a
+    patch on the k_post path (80,474 rows x 60), on their own and batched with
+    a small patch, give the same bits every time (a count lost to the zeroing
+    moved Nr and every later weight)."""
+    from pystamps_amd import synth
+    from pystamps_amd.prepare import prepare_stage2_inputs
+    cfgs = synth.c3_patches()
+    order = sorted(range(16), key=lambda k: cfgs[k].n_ps)[:2]
+    two = [prepare_stage2_inputs(synth.generate(cfgs[k]), nr_rows=300_000) for k in order]
+    assert two[1].n_ps > 65_536
+    keys = ("K_ps", "coh_ps", "trial_ix", "Nr", "ph_weight")
+    first = S.run_stage2_hip(two[1], form="fp64")
+    runs = [S.run_stage2_hip(two[1], form="fp64") for _ in range(5)]
+    runs += [S.run_stage2_hip_batch(two, form="fp64")[1] for _ in range(4)]
+    for r in runs:
+        assert r["gamma_change_save"] == first["gamma_change_save"]
+        for key in keys:
+            assert np.array_equal(r[key], first[key], equal_nan=True), key
