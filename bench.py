@@ -264,8 +264,10 @@ def main():
             iters += i_loop
             dev_s += secs
         wall = time.perf_counter() - t0
-        # per-kernel breakdown from separate loops launched step by step with
-        # HIP events (the same kernels; outside the timed region)
+        # per-kernel breakdown from separate loops whose kernels are launched
+        # one by one between HIP event pairs (the same kernels, every
+        # iteration queued before one host wait, so the device runs them back
+        # to back as in the timed loops)
         s.set_timing(True)
         for _ in range(max(1, min(steps, 3))):
             s.run()
@@ -314,7 +316,8 @@ def main():
     roof["kernel"] = dom
     roof["avg_launch_us"] = avg_s * 1e6
     roof["avg_source"] = ("HIP events on the session stream, per launch, over loops launched "
-                          "kernel by kernel after the timed loops")
+                          "kernel by kernel after the timed loops, all iterations queued before "
+                          "one host wait")
     if args.config == 2 and not args.n_ps:
         pavg = summary_avg_us(STATS_SUMMARY, dom)
         if pavg:

@@ -328,7 +328,12 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
  * one workgroup, is not timed); outputs is the once-per-loop kernel that
  * writes the write_pm1 arguments in the caller's layouts.  Timing events are recorded only when
  * ps2_hip_session_set_timing(s, 1) was called before the runs; such runs
- * launch the iterations one by one (no graph), with the same results. */
+ * launch the kernels one by one instead of as a graph, with the same results,
+ * each dispatch carrying its slot's start / stop events (no marker between
+ * kernels): every iteration is queued before the one host wait (the device
+ * runs them back to back, as in an untimed loop; at most 4096 iterations per
+ * loop), and launches of iterations after the stop decision, which return at
+ * once, are not counted. */
 #define PS2_NUM_KERNEL_SLOTS 9
 int ps2_hip_session_set_timing(ps2_session *s, int32_t enable);
 

@@ -5,6 +5,7 @@
 #include "ps2_kernels.hpp"
 #include "ps2_staging.hpp"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <fcntl.h>
@@ -356,8 +357,15 @@ struct ps2_session {
     // per-kernel timing (ps2_hip_session_set_timing)
     bool timing = false;
     std::vector<hipEvent_t> tev;
-    std::vector<std::pair<int, int>> tpending;  // (slot, index of the start event)
+    struct TimerRec {
+        int slot, start;   // kernel slot, index of the start event in tev
+        int64_t iter;      // loop iteration (1-based) the launch belongs to, 0 outside one
+        bool launched;     // a kernel of the slot was launched with the pair
+    };
+    std::vector<TimerRec> tpending;
     int tused = 0;
+    int64_t timer_iter = 0;
+    int cur_rec = -1;   // tpending entry the next launches are timed by, -1 none
     double slot_ms[PS2_NUM_KERNEL_SLOTS] = {};
     int64_t slot_launches[PS2_NUM_KERNEL_SLOTS] = {};
     // results of the last run
@@ -396,6 +404,11 @@ void kgo(ps2_session *s, void (*k)(KArg<M, A>), dim3 g, dim3 b, const A &a, int 
                     offsetof(MultiPack<A>, range), offsetof(MultiPack<A>, n)};
         r.args.assign(reinterpret_cast<const char *>(&a), reinterpret_cast<const char *>(&a) + sizeof(A));
         s->rec->push_back(std::move(r));
+    } else if (s->cur_rec >= 0) {
+        auto &tr = s->tpending[s->cur_rec];
+        hipExtLaunchKernelGGL(k, g, b, 0, s->stream, tr.launched ? nullptr : s->tev[tr.start],
+                              s->tev[tr.start + 1], 0, a);
+        tr.launched = true;
     } else {
         hipLaunchKernelGGL(k, g, b, 0, s->stream, a);
     }
@@ -742,30 +755,42 @@ int alloc_state(ps2_session *s) {
 enum Slot { SLOT_FILL = 0, SLOT_CLAP_STRIP, SLOT_CLAP_COMBINE, SLOT_SCAN, SLOT_RESCORE, SLOT_REFINE,
             SLOT_POST, SLOT_WEIGHTS, SLOT_OUTPUTS };
 
+// The kernels launched in a SlotTimer's scope carry its event pair in their
+// dispatches (kgo): the start event takes the first kernel's start time, the
+// stop event the last kernel's end time.  No marker packet goes between the
+// kernels, so they run back to back as in a graph.  A scope that launched
+// nothing leaves no record.
 struct SlotTimer {
     ps2_session *s;
     bool on;
-    SlotTimer(ps2_session *s_, int slot) : s(s_), on(s_->timing && s_->tused + 2 <= (int)s_->tev.size()) {
+    SlotTimer(ps2_session *s_, int slot)
+        : s(s_), on(s_->timing && s_->cur_rec < 0 && s_->tused + 2 <= (int)s_->tev.size()) {
         if (!on) return;
-        s->tpending.push_back({slot, s->tused});
-        (void)hipEventRecord(s->tev[s->tused++], s->stream);
+        s->tpending.push_back({slot, s->tused, s->timer_iter, false});
+        s->cur_rec = (int)s->tpending.size() - 1;
+        s->tused += 2;
     }
     ~SlotTimer() {
-        if (on) (void)hipEventRecord(s->tev[s->tused++], s->stream);
+        if (!on) return;
+        if (!s->tpending[s->cur_rec].launched) s->tpending.pop_back();
+        s->cur_rec = -1;
     }
 };
 
-// Accumulates the recorded pairs; the stream must be synchronised.
-void collect_timers(ps2_session *s) {
+// Accumulates the recorded pairs of iterations up to last_iter (launches
+// outside an iteration always); the stream must be synchronised.
+void collect_timers(ps2_session *s, int64_t last_iter = INT64_MAX) {
     for (auto &pr : s->tpending) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, s->tev[pr.second], s->tev[pr.second + 1]) == hipSuccess) {
-            s->slot_ms[pr.first] += ms;
-            s->slot_launches[pr.first] += 1;
+        if (pr.iter > last_iter || !pr.launched) continue;
+        if (hipEventElapsedTime(&ms, s->tev[pr.start], s->tev[pr.start + 1]) == hipSuccess) {
+            s->slot_ms[pr.slot] += ms;
+            s->slot_launches[pr.slot] += 1;
         }
     }
     s->tpending.clear();
     s->tused = 0;
+    s->cur_rec = -1;
 }
 
 // --- kernel launch helpers ---------------------------------------------------
@@ -1375,9 +1400,15 @@ int autotune_tiles(ps2_session *s, const char *path) {
 // HIP graph once per session (one iteration, launched that many times, when
 // there are more than LOOP_GRAPH_MAX) and launched; the control kernel's stop
 // flag turns the remaining kernels into early returns, and the host waits
-// once.  With per-kernel timing or verbose output the iterations
-// are launched one by one and the host reads the control block after each.
+// once.  With per-kernel timing the iterations are launched kernel by kernel,
+// the timing events carried in the dispatches, all of them queued before the
+// one host wait as in the graph (so the device runs them back to back, as in
+// an untimed loop), and the timers of iterations after the stop decision are
+// dropped.  With verbose output the host reads the control block after each
+// iteration.
 constexpr int64_t LOOP_GRAPH_MAX = 64;
+// timer events per queued iteration (at most 12 timed launches each)
+constexpr int TIMERS_PER_ITERATION = 24;
 
 // The loop's starting state, on stream st (not captured): K = 0,
 // coh_prev = 0, the first iteration's histogram and re-score counter (k_control
@@ -1430,6 +1461,18 @@ int run_loop(ps2_session *s) {
     s->gate = &s->ctl.as<LoopCtl>()->stopped;
     const int64_t max_iter = s->ctl_params.max_iter;
     bool stepwise = s->timing || s->verbose || s->no_graph;
+    // timing without verbose output: every iteration queued, one host wait
+    const bool queued_timing = s->timing && !s->verbose;
+    if (queued_timing) {
+        const size_t need = (size_t)TIMERS_PER_ITERATION * (size_t)max_iter + 64;
+        if (max_iter > 4096) return arg_error("per-kernel timing supports at most 4096 iterations per loop");
+        if (s->tev.size() < need) {
+            AllocGuard g(capture_mutex());
+            const size_t have = s->tev.size();
+            s->tev.resize(need, nullptr);
+            for (size_t e = have; e < need; ++e) PS2_HIP(hipEventCreate(&s->tev[e]));
+        }
+    }
     // the graph holds every iteration (up to LOOP_GRAPH_MAX of them, else one
     // iteration replayed): each graph launch costs ~30 us more than the
     // ~10 us between two of its kernels
@@ -1468,6 +1511,18 @@ int run_loop(ps2_session *s) {
             s->gate = nullptr;
             return rc;
         }
+    } else if (queued_timing) {
+        for (int64_t it = 0; it < max_iter; ++it) {
+            s->timer_iter = it + 1;
+            if ((rc = launch_iteration<R>(s))) {
+                s->timer_iter = 0;
+                s->gate = nullptr;
+                return rc;
+            }
+        }
+        s->timer_iter = 0;
+        s->gate = nullptr;
+        if ((rc = launch_outputs<R>(s, true))) return rc;
     } else {
         for (int64_t it = 0; it < max_iter; ++it) {
             const auto t_it = std::chrono::steady_clock::now();
@@ -1499,7 +1554,7 @@ int run_loop(ps2_session *s) {
     PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
     PS2_HIP(hipStreamSynchronize(s->stream));
     s->gate = nullptr;
-    collect_timers(s);
+    collect_timers(s, hc->i_loop);
     float ms = 0.0f;
     PS2_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
     loop_finish(s, (double)ms * 1e-3);

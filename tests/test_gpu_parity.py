@@ -900,18 +900,32 @@ def test_fp32_f64_baselines_split_and_plane_agree(S, prep_small):
 
 
 def test_graph_replays_equal_stepwise_launches(S, prep_small):
-    """The default loop replays one captured iteration; with timing on the same
-    kernels are launched one by one with host reads between iterations.  Every
-    replay, with eager work and host synchronisation in between, gives the
-    step-wise result bit for bit."""
+    """The default loop replays the captured iterations; with timing on the
+    same kernels are launched one by one between event pairs, every iteration
+    queued before one host wait; with verbose output the host reads the
+    control block after each iteration.  Every replay, with eager work and
+    host synchronisation in between, gives the step-wise result bit for bit,
+    and the timers count the launches of the iterations that ran (those
+    queued after the stop decision return at once and are not counted)."""
     keys = ("K_ps", "C_ps", "coh_ps", "ph_grid", "ph_weight", "ph_res", "Nr", "trial_ix")
+    with S.Stage2Session(prep_small, form="fp32", verbose=True) as s:
+        s.run()
+        host_read = s.download()
     with S.Stage2Session(prep_small, form="fp32") as s:
         s.run()
         first = s.download()
         s.set_timing(True)
         s.run()
         stepwise = s.download()
+        kt = s.kernel_times()
         s.set_timing(False)
+        n_it = stepwise["i_loop"]
+        assert n_it >= 2
+        assert kt["fill_grid"][1] == n_it and kt["clap_strip"][1] == n_it and kt["weights"][1] == n_it
+        assert kt["outputs"][1] == 1
+        assert all(ms > 0.0 for ms, n in kt.values() if n)
+        for key in keys:
+            assert np.array_equal(host_read[key], stepwise[key], equal_nan=True), key
         for _ in range(3):
             _ = s.fill_grid(np.zeros(prep_small.n_ps), np.ones(prep_small.n_ps))  # eager work between replays
             il, _ = s.run()
