@@ -2910,9 +2910,10 @@ struct LoopCtl {
     double nr[CTL_BINS];  // Nr, scaled each P-square update
 };
 
-// The loop's starting state as a kernel (the merged launches of several
-// patches start with it; a single patch's loop sets the same state with
-// memsets and copies): K = 0, coh_prev = 0, W = W0, the first iteration's
+// The loop's starting state as a kernel (the captured loop of one patch and
+// the merged launches of several start with it; a loop launched kernel by
+// kernel sets the same state with memsets and copies): K = 0, coh_prev = 0,
+// W = W0, the first iteration's
 // histogram and re-score counters zero, and the control block with i_loop 1
 // and Nr = the base histogram nr0.  Not gated.
 struct PrepareArgs {
@@ -2954,6 +2955,18 @@ __global__ __launch_bounds__(256) void k_loop_prepare(KArg<M, PrepareArgs> p) {
             a.ctl->gcs = 0.0;
         }
     }
+}
+
+// The control block copied to coherent pinned host memory, the last kernel
+// of a captured loop: after its wait on the stream the host reads the loop's
+// results there, with no copy queued behind the loop.  Not gated.
+__global__ __launch_bounds__(64) void k_publish_ctl(const LoopCtl *__restrict__ src, LoopCtl *__restrict__ dst) {
+    static_assert(sizeof(LoopCtl) % 8 == 0, "whole 8-byte words");
+    constexpr int NW = (int)(sizeof(LoopCtl) / 8);
+    const uint64_t *s = reinterpret_cast<const uint64_t *>(src);
+    uint64_t *d = reinterpret_cast<uint64_t *>(dst);
+    for (int i = threadIdx.x; i < NW; i += 64) d[i] = s[i];
+    __threadfence_system();
 }
 
 struct CtlParams {
@@ -3006,7 +3019,7 @@ __device__ inline void control_step(LoopCtl *__restrict__ ctl, const double *__r
     constexpr int CTL_THREADS = 256;   // the calling workgroup
     __shared__ double red[CTL_THREADS];
     __shared__ double na[128], nr[128], prand[128], sm[128];
-    __shared__ double up[CTL_UP];
+    __shared__ double ext[112];   // [1, smoothed prand], zeros after
     __shared__ int s_stop;
     __shared__ double s_scale;
     if (ctl->stopped) return;
@@ -3064,9 +3077,23 @@ __device__ inline void control_step(LoopCtl *__restrict__ ctl, const double *__r
     clear_counters();
     __syncthreads();
     if (t == 0) {
+        // in bin order, ten bins' loads issued at a time (b0 + j <= 99)
         double denom = 0.0, num = 0.0;
-        for (int b = 0; b < lct; ++b) denom += nr[b];
-        for (int b = 0; b < lct; ++b) num += na[b];
+        for (int b0 = 0; b0 < lct; b0 += 10) {
+            double vr[10], va[10];
+#pragma unroll
+            for (int j = 0; j < 10; ++j) {
+                vr[j] = nr[b0 + j];
+                va[j] = na[b0 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 10; ++j) {
+                if (b0 + j < lct) {
+                    denom += vr[j];
+                    num += va[j];
+                }
+            }
+        }
         s_scale = denom > 0.0 ? num / denom : 1.0;
     }
     __syncthreads();
@@ -3093,20 +3120,25 @@ __device__ inline void control_step(LoopCtl *__restrict__ ctl, const double *__r
         sm[t] = sacc / cp.gsum;
     }
     __syncthreads();
-    // interp([1, prand], 10) (:1715-1748)
-    for (int q = t; q < CTL_UP; q += CTL_THREADS) {
-        double v = 0.0;
-        if (q == 0)
-            v = 1.0;
-        else if (q % 10 == 0 && q / 10 - 1 < CTL_BINS)
-            v = sm[q / 10 - 1];
-        up[q] = v;
-    }
+    // interp([1, prand], 10) (:1715-1748): the zero-stuffed sequence up
+    // (up[10 m] = ext[m], zeros between; up[i] = 0 for i >= 1049) through
+    // the 79-tap FIR, prand_hi[q] = 10 sum_{k = 0..min(78, n)} tap[k]
+    // up[n - k], n = 39 + q.  Only the taps k = n mod 10 + 10 j meet a
+    // non-zero input; the others add a zero product, which leaves a sum
+    // that starts at +0 unchanged, so the sum of those few terms in tap
+    // order is the whole sum, bit for bit.
+    if (t < 112) ext[t] = t == 0 ? 1.0 : (t <= CTL_BINS ? sm[t - 1] : 0.0);
     __syncthreads();
     for (int q = t; q < CTL_HI; q += CTL_THREADS) {
         const int n = 39 + q;
+        const int r = n % 10;
         double sacc = 0.0;
-        for (int k = 0; k < 79 && k <= n; ++k) sacc += kCtlInterpTaps[k] * up[n - k];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = r + 10 * j;
+            // n - k = 10 m with m <= 103 (ext has 112 entries)
+            if (k <= 78 && k <= n && n - k < CTL_UP) sacc += kCtlInterpTaps[k] * ext[(n - k) / 10];
+        }
         prand_hi[q] = sacc * 10.0;
     }
     if (t == 0) ctl->i_loop = i_loop0 + 1;

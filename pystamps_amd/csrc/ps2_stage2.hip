@@ -107,9 +107,9 @@ struct HostPinned {
             (void)hipHostFree(p);
         }
     }
-    int alloc(size_t n) {
+    int alloc(size_t n, unsigned flags = hipHostMallocDefault) {
         AllocGuard g(capture_mutex());
-        hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, n, flags);
         if (e != hipSuccess) {
             p = nullptr;
             set_error(std::string("pinned allocation failed: ") + hipGetErrorString(e));
@@ -346,6 +346,10 @@ struct ps2_session {
     // while a loop is being launched), one iteration captured as a graph
     DBuf ctl, nr0;   // control block; the base Nr histogram (its starting Nr)
     HostPinned h_ctl;
+    // coherent pinned copy of the control block that a captured loop's last
+    // kernel writes (k_publish_ctl), and its device address
+    HostPinned h_ctl_pub;
+    LoopCtl *d_ctl_pub = nullptr;
     const int32_t *gate = nullptr;
     // while set, the launch functions instantiated for several patches record
     // their launches here instead of launching them
@@ -711,6 +715,8 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->prand_hi.alloc(sizeof(double) * 2048))) return rc;
     if ((rc = s->ctl.alloc(sizeof(LoopCtl)))) return rc;
     if ((rc = s->h_ctl.alloc(sizeof(LoopCtl)))) return rc;
+    if ((rc = s->h_ctl_pub.alloc(sizeof(LoopCtl), hipHostMallocCoherent))) return rc;
+    PS2_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->d_ctl_pub), s->h_ctl_pub.p, 0));
     if ((rc = s->nr0.alloc(sizeof(double) * CTL_BINS))) return rc;
     PS2_HIP(hipMemcpyAsync(s->nr0.p, s->nr_base.data(), sizeof(double) * CTL_BINS, hipMemcpyHostToDevice,
                            s->stream));
@@ -1396,9 +1402,11 @@ int autotune_tiles(ps2_session *s, const char *path) {
 }
 
 // The gamma loop from the initial state (native_stage2.rs:242-258) to the
-// stop decision, and the outputs of the last iteration.  Default: gamma_max_iterations iterations are captured as one
-// HIP graph once per session (one iteration, launched that many times, when
-// there are more than LOOP_GRAPH_MAX) and launched; the control kernel's stop
+// stop decision, and the outputs of the last iteration.  Default: the
+// starting state, gamma_max_iterations iterations and the outputs are
+// captured as one HIP graph once per session (one iteration, launched that
+// many times after the starting state is set, when there are more than
+// LOOP_GRAPH_MAX) and launched; the control kernel's stop
 // flag turns the remaining kernels into early returns, and the host waits
 // once.  With per-kernel timing the iterations are launched kernel by kernel,
 // the timing events carried in the dispatches, all of them queued before the
@@ -1456,8 +1464,6 @@ template <typename R>
 int run_loop(ps2_session *s) {
     int rc;
     LoopCtl *hc = s->h_ctl.as<LoopCtl>();
-    PS2_HIP(hipEventRecord(s->ev[0], s->stream));
-    if ((rc = loop_prepare(s, s->stream))) return rc;
     s->gate = &s->ctl.as<LoopCtl>()->stopped;
     const int64_t max_iter = s->ctl_params.max_iter;
     bool stepwise = s->timing || s->verbose || s->no_graph;
@@ -1477,15 +1483,20 @@ int run_loop(ps2_session *s) {
     // iteration replayed): each graph launch costs ~30 us more than the
     // ~10 us between two of its kernels
     const int64_t per_graph = max_iter <= LOOP_GRAPH_MAX ? max_iter : 1;
-    // a graph that holds every iteration ends with the output kernel
+    // a graph that holds every iteration starts with the loop's starting
+    // state (one kernel) and ends with the output kernel: one launch per loop
     const bool outputs_in_graph = per_graph == max_iter;
     if (!stepwise && !s->graph_exec) {
         // no allocation anywhere in the library while the capture is open
         std::unique_lock<std::shared_mutex> cap(capture_mutex());
         PS2_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        rc = 0;
+        rc = outputs_in_graph ? launch_prepare<false>(s) : 0;
         for (int64_t it = 0; it < per_graph && !rc; ++it) rc = launch_iteration<R>(s);
         if (!rc && outputs_in_graph) rc = launch_outputs<R>(s, true);
+        if (!rc && outputs_in_graph) {
+            hipLaunchKernelGGL(k_publish_ctl, dim3(1), dim3(64), 0, s->stream, s->ctl.as<LoopCtl>(), s->d_ctl_pub);
+            rc = hipGetLastError() == hipSuccess ? 0 : PS2_E_DEVICE;
+        }
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s->stream, &g);
         cap.unlock();
@@ -1504,6 +1515,11 @@ int run_loop(ps2_session *s) {
             s->no_graph = true;
             stepwise = true;
         }
+    }
+    PS2_HIP(hipEventRecord(s->ev[0], s->stream));
+    if ((stepwise || !outputs_in_graph) && (rc = loop_prepare(s, s->stream))) {
+        s->gate = nullptr;
+        return rc;
     }
     if (!stepwise) {
         for (int64_t it = 0; it < max_iter; it += per_graph) PS2_HIP(hipGraphLaunch(s->graph_exec, s->stream));
@@ -1551,8 +1567,12 @@ int run_loop(ps2_session *s) {
         if ((rc = launch_outputs<R>(s, true))) return rc;
     }
     PS2_HIP(hipEventRecord(s->ev[1], s->stream));
-    PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
+    // a graph holding the whole loop ends with the control block written to
+    // h_ctl_pub; else it is copied down
+    const bool published = !stepwise && outputs_in_graph;
+    if (!published) PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
     PS2_HIP(hipStreamSynchronize(s->stream));
+    if (published) std::memcpy(hc, s->h_ctl_pub.p, sizeof(LoopCtl));
     s->gate = nullptr;
     collect_timers(s, hc->i_loop);
     float ms = 0.0f;
