@@ -87,6 +87,34 @@ __device__ inline cplx<R> weight_value(cplx<R> s, R b, double K, double W) {
     return {re * ww, im * ww};
 }
 
+// w = (1 - prand_hi[round_half_away(1000 coh)])^2, the index clamped into
+// the table (native_stage2.rs:1674-1682)
+__device__ inline double psquare_weight(double coh, const double *__restrict__ prand_hi, int nhi) {
+    const double v = coh * 1000.0;
+    double r = v >= 0.0 ? floor(v + 0.5) : ceil(v - 0.5);
+    const double hi = (double)(nhi > 0 ? nhi - 1 : 0);
+    if (r < 0.0) r = 0.0;
+    if (r > hi) r = hi;
+    const int ix = __builtin_isnan(r) ? 0 : (int)r;
+    const double p = prand_hi[ix];
+    return (1.0 - p) * (1.0 - p);
+}
+
+// The weighting a fill applies: W[k], or, with P-square weighting in every
+// iteration after the first (the control block's iteration counter), the
+// weight of the pixel's coherence in the P-square table that the previous
+// iteration's control step built, computed here in place of a weights
+// kernel between the two iterations (psquare_weighting, native_stage2.rs:
+// 1674-1682; the first iteration's W is the initial weighting).
+struct FillWeights {
+    const double *W;
+    const double *coh, *prand_hi;   // null coh: W only
+    const int32_t *i_loop;          // the control block's iteration counter (1-based)
+    int nhi;
+    __device__ bool from_coh() const { return coh && *i_loop > 1; }
+    __device__ double at(int64_t k, bool fc) const { return fc ? psquare_weight(coh[k], prand_hi, nhi) : W[k]; }
+};
+
 // fill_phase_weight + accumulate_grid (native_stage2.rs:629-657, 722-740):
 // one thread per (ifg, cell); the cell's pixels are summed in ascending row
 // order, which is the reference's sequential order, so no atomics are needed.
@@ -98,7 +126,7 @@ __device__ __forceinline__ void fill_grid_body(
     const R *__restrict__ bp,
     int bp_rowinv,
     const double *__restrict__ K,
-    const double *__restrict__ W,
+    FillWeights fw,
     const int32_t *__restrict__ cell_start,
     cplx<R> *__restrict__ grid,
     int64_t C,
@@ -111,6 +139,7 @@ __device__ __forceinline__ void fill_grid_body(
     if (loop_stopped(gate)) return;
     const int64_t gid = (int64_t)bid_ * blockDim.x + threadIdx.x;
     if (gid >= C * I) return;
+    const bool fc = fw.from_coh();
     const int f = (int)(gid / C);
     const int64_t c = gid - (int64_t)f * C;
     const int32_t k0 = cell_start[c], k1 = cell_start[c + 1];
@@ -119,7 +148,7 @@ __device__ __forceinline__ void fill_grid_body(
     for (int32_t k = k0; k < k1; ++k) {
         const int64_t e = (int64_t)f * P + k;
         const R b = bp_rowinv ? bp[f] : bp[e];
-        const double kk = K[k], ww = W[k];
+        const double kk = K[k], ww = fw.at(k, fc);
         if (keep) {
             Kprev[k] = kk;
             Wprev[k] = ww;
@@ -137,7 +166,7 @@ struct FillArgs {
     const R * bp;
     int bp_rowinv;
     const double * K;
-    const double * W;
+    FillWeights fw;
     const int32_t * cell_start;
     cplx<R> * grid;
     int64_t C;
@@ -154,9 +183,9 @@ __global__ __launch_bounds__(256) void k_fill_grid(KArg<M, FillArgs<R>> p) {
         FillArgs<R> a;
         uint32_t bid_, nblk_;
         if (!multi_locate(p, a, bid_, nblk_)) return;
-        fill_grid_body<R>(a.phnm, a.bp, a.bp_rowinv, a.K, a.W, a.cell_start, a.grid, a.C, a.I, a.P, a.Kprev, a.Wprev, a.gate, bid_, nblk_);
+        fill_grid_body<R>(a.phnm, a.bp, a.bp_rowinv, a.K, a.fw, a.cell_start, a.grid, a.C, a.I, a.P, a.Kprev, a.Wprev, a.gate, bid_, nblk_);
     } else {
-        fill_grid_body<R>(p.phnm, p.bp, p.bp_rowinv, p.K, p.W, p.cell_start, p.grid, p.C, p.I, p.P, p.Kprev, p.Wprev, p.gate, blockIdx.x, gridDim.x);
+        fill_grid_body<R>(p.phnm, p.bp, p.bp_rowinv, p.K, p.fw, p.cell_start, p.grid, p.C, p.I, p.P, p.Kprev, p.Wprev, p.gate, blockIdx.x, gridDim.x);
     }
 }
 
@@ -174,7 +203,7 @@ __device__ __forceinline__ void fill_grid_staged_body(
     const R *__restrict__ bp,
     int bp_rowinv,
     const double *__restrict__ K,
-    const double *__restrict__ W,
+    FillWeights fw,
     const int32_t *__restrict__ cell_start,
     cplx<R> *__restrict__ grid,
     int64_t C,
@@ -206,6 +235,7 @@ __device__ __forceinline__ void fill_grid_staged_body(
     const int f = f0 + fs;
     const bool mine = c < C && f < I;
     const int32_t my0 = mine ? cell_start[c] : 0, my1 = mine ? cell_start[c + 1] : 0;
+    const bool fc = fw.from_coh();
     R sre = (R)0, sim = (R)0;
     constexpr int NJ = CHUNK / 256;   // pixels per thread and chunk
     for (int32_t base = ks; base < ke; base += CHUNK) {
@@ -221,7 +251,7 @@ __device__ __forceinline__ void fill_grid_staged_body(
             if (j < n) {
                 const int64_t k = base + j;
                 kk[u] = K[k];
-                ww[u] = W[k];
+                ww[u] = fw.at(k, fc);
                 if (Kprev && pg == 0) {   // each pixel once: its cell group's first plane group
                     Kprev[k] = kk[u];
                     Wprev[k] = ww[u];
@@ -273,7 +303,7 @@ struct FillStagedArgs {
     const R * bp;
     int bp_rowinv;
     const double * K;
-    const double * W;
+    FillWeights fw;
     const int32_t * cell_start;
     cplx<R> * grid;
     int64_t C;
@@ -291,9 +321,9 @@ __global__ __launch_bounds__(256) void k_fill_grid_staged(KArg<M, FillStagedArgs
         FillStagedArgs<R> a;
         uint32_t bid_, nblk_;
         if (!multi_locate(p, a, bid_, nblk_)) return;
-        fill_grid_staged_body<R, CB, CHUNK>(a.phnm, a.bp, a.bp_rowinv, a.K, a.W, a.cell_start, a.grid, a.C, a.I, a.P, a.npg, a.Kprev, a.Wprev, a.gate, bid_, nblk_);
+        fill_grid_staged_body<R, CB, CHUNK>(a.phnm, a.bp, a.bp_rowinv, a.K, a.fw, a.cell_start, a.grid, a.C, a.I, a.P, a.npg, a.Kprev, a.Wprev, a.gate, bid_, nblk_);
     } else {
-        fill_grid_staged_body<R, CB, CHUNK>(p.phnm, p.bp, p.bp_rowinv, p.K, p.W, p.cell_start, p.grid, p.C, p.I, p.P, p.npg, p.Kprev, p.Wprev, p.gate, blockIdx.x, gridDim.x);
+        fill_grid_staged_body<R, CB, CHUNK>(p.phnm, p.bp, p.bp_rowinv, p.K, p.fw, p.cell_start, p.grid, p.C, p.I, p.P, p.npg, p.Kprev, p.Wprev, p.gate, blockIdx.x, gridDim.x);
     }
 }
 
@@ -2767,55 +2797,6 @@ __global__ void k_zero_u32(KArg<M, ZeroArgs> p) {
     }
 }
 
-// w = (1 - prand_hi[round_half_away(1000 coh)])^2, the index clamped into
-// the table (native_stage2.rs:1674-1682)
-__device__ inline double psquare_weight(double coh, const double *__restrict__ prand_hi, int nhi) {
-    const double v = coh * 1000.0;
-    double r = v >= 0.0 ? floor(v + 0.5) : ceil(v - 0.5);
-    const double hi = (double)(nhi > 0 ? nhi - 1 : 0);
-    if (r < 0.0) r = 0.0;
-    if (r > hi) r = hi;
-    const int ix = __builtin_isnan(r) ? 0 : (int)r;
-    const double p = prand_hi[ix];
-    return (1.0 - p) * (1.0 - p);
-}
-
-// psquare weighting lookup (native_stage2.rs:1674-1682):
-// w = (1 - prand_hi[round_half_away(1000 coh)])^2
-__device__ __forceinline__ void weights_psquare_body(
-    const double *__restrict__ coh,
-    const double *__restrict__ prand_hi,
-    int nhi,
-    double *__restrict__ W,
-    int64_t P,
-    const int32_t *__restrict__ gate,
-    const uint32_t bid_, const uint32_t nblk_) {
-    if (loop_stopped(gate)) return;
-    const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
-    if (k >= P) return;
-    W[k] = psquare_weight(coh[k], prand_hi, nhi);
-}
-
-struct PsquareArgs {
-    const double * coh;
-    const double * prand_hi;
-    int nhi;
-    double * W;
-    int64_t P;
-    const int32_t * gate;
-};
-
-template <bool M>
-__global__ __launch_bounds__(256) void k_weights_psquare(KArg<M, PsquareArgs> p) {
-    if constexpr (M) {
-        PsquareArgs a;
-        uint32_t bid_, nblk_;
-        if (!multi_locate(p, a, bid_, nblk_)) return;
-        weights_psquare_body(a.coh, a.prand_hi, a.nhi, a.W, a.P, a.gate, bid_, nblk_);
-    } else {
-        weights_psquare_body(p.coh, p.prand_hi, p.nhi, p.W, p.P, p.gate, blockIdx.x, gridDim.x);
-    }
-}
 
 // snr_weighting (native_stage2.rs:1875-1897) from the current residuals.
 template <typename R>

@@ -804,9 +804,13 @@ void collect_timers(ps2_session *s, int64_t last_iter = INT64_MAX) {
 
 template <typename R, bool M = false>
 int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *Kprev = nullptr,
-                     double *Wprev = nullptr) {
+                     double *Wprev = nullptr, bool psquare_in_fill = false) {
     const int64_t n = s->C * s->I;
     SlotTimer tm(s, SLOT_FILL);
+    // inside the loop with P-square weighting, iterations after the first
+    // compute the weights from coh and the P-square table (FillWeights)
+    const FillWeights fw{W, psquare_in_fill ? s->coh.as<double>() : nullptr, s->prand_hi.as<double>(),
+                         psquare_in_fill ? &s->ctl.as<LoopCtl>()->i_loop : nullptr, CTL_HI};
     if (sizeof(R) == 4) {
         // 128 cells x 2 planes per workgroup below 12 pixels per cell, else
         // 64 cells x 4 planes (measured: 128 x 2 fastest of 32 x 8, 64 x 4,
@@ -824,7 +828,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *K
             const int64_t ncg = blocks_for(s->C, CB);
             const int npg = (int)blocks_for(s->I, 256 / CB);
             const dim3 grid((unsigned)(blocks_for(ncg, PS2_NUM_XCD) * PS2_NUM_XCD * npg));
-            const FillStagedArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
+            const FillStagedArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, fw,
                                       s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I,
                                       s->P, npg, Kprev, Wprev, s->gate};
             auto go = [&](auto kern) { kgo<M>(s, kern, grid, dim3(256), a, ST_FILL); };
@@ -840,7 +844,7 @@ int launch_fill_grid(ps2_session *s, const double *K, const double *W, double *K
         else
             launch(std::integral_constant<int, 64>{});
     } else {
-        const FillArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, W,
+        const FillArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->rowinv, K, fw,
                             s->cell_start.as<int32_t>(), s->grid.as<cplx<R>>(), s->C, (int)s->I, s->P,
                             Kprev, Wprev, s->gate};
         kgo<M>(s, k_fill_grid<M, R>, dim3(blocks_for(n, 256)), dim3(256), a, ST_FILL);
@@ -1156,9 +1160,11 @@ int launch_iteration(ps2_session *s) {
     const int64_t P = s->P;
     int rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
-    // the fill also keeps the K and weighting it used (for the ph_weight output)
+    // the fill also keeps the K and weighting it used (for the ph_weight
+    // output); P-square weights of the previous iteration are computed in it
+    const bool psquare = s->weighting == PS2_WEIGHT_PSQUARE;
     if ((rc = launch_fill_grid<R, M>(s, s->K.as<double>(), s->W.as<double>(), s->Kprev.as<double>(),
-                                     s->Wprev.as<double>())))
+                                     s->Wprev.as<double>(), psquare)))
         return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
     if ((rc = launch_clap<R, M>(s, true))) return rc;
@@ -1170,17 +1176,13 @@ int launch_iteration(ps2_session *s) {
                          s->rescore_count.as<uint32_t>(), s->prand_hi.as<double>(), s->ctl_params};
     kgo<M>(s, k_control<M>, dim3(1), dim3(256), ca, ST_CONTROL);
     PS2_LAUNCHED();
+    if (psquare) return PS2_OK;   // the next fill computes them
     SlotTimer tm(s, SLOT_WEIGHTS);
-    if (s->weighting == PS2_WEIGHT_PSQUARE)
-        kgo<M>(s, k_weights_psquare<M>, dim3(blocks_for(P, 256)), dim3(256),
-               PsquareArgs{s->coh.as<double>(), s->prand_hi.as<double>(), CTL_HI, s->W.as<double>(), P, s->gate},
-               ST_WEIGHTS);
-    else
-        kgo<M>(s, k_weights_snr<M, R>, dim3(blocks_for(P, 256)), dim3(256),
-               SnrArgs<R>{s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
-                          s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(), s->W.as<double>(),
-                          P, (int)s->I, s->rowinv, s->C, s->gate},
-               ST_WEIGHTS);
+    kgo<M>(s, k_weights_snr<M, R>, dim3(blocks_for(P, 256)), dim3(256),
+           SnrArgs<R>{s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
+                      s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(), s->W.as<double>(),
+                      P, (int)s->I, s->rowinv, s->C, s->gate},
+           ST_WEIGHTS);
     PS2_LAUNCHED();
     return PS2_OK;
 }
