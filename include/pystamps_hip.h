@@ -325,9 +325,9 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
  * time of slot k over launches[k] launches.  post is the kernel after the
  * refine: the FP64 re-decision of the rows the refine marks (FP32 form) and
  * the gamma-change RMS partials and coherence histogram (the loop control,
- * one workgroup, is not timed); weights is the SNR weighting update (P-square
- * weights are computed in the next iteration's fill, so that slot stays
- * empty with P-square weighting); outputs is the once-per-loop kernel that
+ * one workgroup, is not timed); weights is the weighting update (a patch of
+ * at most 65,536 rows computes P-square weights in the next iteration's fill,
+ * so that slot stays empty for it); outputs is the once-per-loop kernel that
  * writes the write_pm1 arguments in the caller's layouts.  Timing events are recorded only when
  * ps2_hip_session_set_timing(s, 1) was called before the runs; such runs
  * launch the kernels one by one instead of as a graph, with the same results,

@@ -105,7 +105,10 @@ __device__ inline double psquare_weight(double coh, const double *__restrict__ p
 // weight of the pixel's coherence in the P-square table that the previous
 // iteration's control step built, computed here in place of a weights
 // kernel between the two iterations (psquare_weighting, native_stage2.rs:
-// 1674-1682; the first iteration's W is the initial weighting).
+// 1674-1682; the first iteration's W is the initial weighting).  Used for
+// small patches, whose iterations are bound by launch latency: the fill's
+// plane-group workgroups each compute a pixel's weight again, which costs a
+// large patch more than the separate kernel.
 struct FillWeights {
     const double *W;
     const double *coh, *prand_hi;   // null coh: W only
@@ -2797,6 +2800,44 @@ __global__ void k_zero_u32(KArg<M, ZeroArgs> p) {
     }
 }
 
+
+// psquare weighting lookup (native_stage2.rs:1674-1682):
+// w = (1 - prand_hi[round_half_away(1000 coh)])^2, once per pixel (large
+// patches; a small patch's fill computes it, FillWeights)
+__device__ __forceinline__ void weights_psquare_body(
+    const double *__restrict__ coh,
+    const double *__restrict__ prand_hi,
+    int nhi,
+    double *__restrict__ W,
+    int64_t P,
+    const int32_t *__restrict__ gate,
+    const uint32_t bid_, const uint32_t nblk_) {
+    if (loop_stopped(gate)) return;
+    const int64_t k = (int64_t)bid_ * blockDim.x + threadIdx.x;
+    if (k >= P) return;
+    W[k] = psquare_weight(coh[k], prand_hi, nhi);
+}
+
+struct PsquareArgs {
+    const double * coh;
+    const double * prand_hi;
+    int nhi;
+    double * W;
+    int64_t P;
+    const int32_t * gate;
+};
+
+template <bool M>
+__global__ __launch_bounds__(256) void k_weights_psquare(KArg<M, PsquareArgs> p) {
+    if constexpr (M) {
+        PsquareArgs a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        weights_psquare_body(a.coh, a.prand_hi, a.nhi, a.W, a.P, a.gate, bid_, nblk_);
+    } else {
+        weights_psquare_body(p.coh, p.prand_hi, p.nhi, p.W, p.P, p.gate, blockIdx.x, gridDim.x);
+    }
+}
 
 // snr_weighting (native_stage2.rs:1875-1897) from the current residuals.
 template <typename R>

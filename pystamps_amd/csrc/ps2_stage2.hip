@@ -1161,10 +1161,12 @@ int launch_iteration(ps2_session *s) {
     int rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[2], s->stream));
     // the fill also keeps the K and weighting it used (for the ph_weight
-    // output); P-square weights of the previous iteration are computed in it
+    // output); a small patch's fill computes the P-square weights of the
+    // previous iteration
     const bool psquare = s->weighting == PS2_WEIGHT_PSQUARE;
+    const bool psquare_in_fill = psquare && small_patch(s);
     if ((rc = launch_fill_grid<R, M>(s, s->K.as<double>(), s->W.as<double>(), s->Kprev.as<double>(),
-                                     s->Wprev.as<double>(), psquare)))
+                                     s->Wprev.as<double>(), psquare_in_fill)))
         return rc;
     if (s->verbose) PS2_HIP(hipEventRecord(s->ev[3], s->stream));
     if ((rc = launch_clap<R, M>(s, true))) return rc;
@@ -1176,13 +1178,18 @@ int launch_iteration(ps2_session *s) {
                          s->rescore_count.as<uint32_t>(), s->prand_hi.as<double>(), s->ctl_params};
     kgo<M>(s, k_control<M>, dim3(1), dim3(256), ca, ST_CONTROL);
     PS2_LAUNCHED();
-    if (psquare) return PS2_OK;   // the next fill computes them
+    if (psquare_in_fill) return PS2_OK;   // the next fill computes them
     SlotTimer tm(s, SLOT_WEIGHTS);
-    kgo<M>(s, k_weights_snr<M, R>, dim3(blocks_for(P, 256)), dim3(256),
-           SnrArgs<R>{s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
-                      s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(), s->W.as<double>(),
-                      P, (int)s->I, s->rowinv, s->C, s->gate},
-           ST_WEIGHTS);
+    if (psquare)
+        kgo<M>(s, k_weights_psquare<M>, dim3(blocks_for(P, 256)), dim3(256),
+               PsquareArgs{s->coh.as<double>(), s->prand_hi.as<double>(), CTL_HI, s->W.as<double>(), P, s->gate},
+               ST_WEIGHTS);
+    else
+        kgo<M>(s, k_weights_snr<M, R>, dim3(blocks_for(P, 256)), dim3(256),
+               SnrArgs<R>{s->phnm.as<cplx<R>>(), s->amp.as<float>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
+                          s->cell_of.as<int32_t>(), s->K.as<double>(), s->nopt.as<double>(), s->W.as<double>(),
+                          P, (int)s->I, s->rowinv, s->C, s->gate},
+               ST_WEIGHTS);
     PS2_LAUNCHED();
     return PS2_OK;
 }

@@ -922,9 +922,9 @@ def test_graph_replays_equal_stepwise_launches(S, prep_small):
         n_it = stepwise["i_loop"]
         assert n_it >= 2
         assert kt["fill_grid"][1] == n_it and kt["clap_strip"][1] == n_it
-        # P-square weights are computed in the next fill: no weights launch
-        psquare = prep_small.filter_weighting.lower() == "p-square"
-        assert kt["weights"][1] == (0 if psquare else n_it)
+        # a small patch's P-square weights are computed in the next fill
+        fused = prep_small.filter_weighting.lower() == "p-square" and prep_small.n_ps <= 65536
+        assert kt["weights"][1] == (0 if fused else n_it)
         assert kt["outputs"][1] == 1
         assert all(ms > 0.0 for ms, n in kt.values() if n)
         for key in keys:
