@@ -269,7 +269,8 @@ def main():
         # iteration queued before one host wait, so the device runs them back
         # to back as in the timed loops)
         s.set_timing(True)
-        for _ in range(max(1, min(steps, 3))):
+        n_timing = max(1, min(steps, 3))
+        for _ in range(n_timing):
             s.run()
         kt = s.kernel_times()
         s.set_timing(False)
@@ -282,8 +283,14 @@ def main():
         for name, (ms, n) in kt.items():
             if n:
                 log(f"[bench]   {name:13s} {ms / n * 1e3:9.1f} us/launch x {n}")
+        # device time per loop of the timed (graph-launched) loops over the sum
+        # of the per-kernel event times per timing loop (the control step,
+        # the starting state and the control-block write are not timed per
+        # kernel; they are ~20 us of a loop)
+        ksum = sum(ms for ms, n in kt.values()) * 1e-3 / n_timing
         return {"wall": wall, "iters": iters, "dev": dev_s, "kt": kt, "out": out, "setup": setup,
-                "download": download, "tiles": tiles}
+                "download": download, "tiles": tiles,
+                "graph_over_kernel_sum": (dev_s / steps) / ksum if ksum > 0 else None}
 
     r32 = time_form("fp32", args.steps, args.warmup)
     e2e = prep_s + r32["setup"] + r32["wall"] / args.steps + r32["download"]
@@ -327,6 +334,7 @@ def main():
                                          if roof["unit"] == "TFLOP/s"
                                          else nbytes / (pavg * 1e-6) / 1e9 / PEAK_HBM_GBS)
     roof["loop_time_weighted_frac"] = aggregate_roofline(prep, kt, PEAK_FP32_TFLOPS)
+    roof["timed_loop_device_over_kernel_event_sum"] = r32["graph_over_kernel_sum"]
 
     result = {
         "metric": "stage-2 PS·ifg·iter/s, FP32+batched vs FP64 GPU at matched K/gamma error",
