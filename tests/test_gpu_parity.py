@@ -899,6 +899,36 @@ def test_fp32_f64_baselines_split_and_plane_agree(S, prep_small):
         assert np.array_equal(a[key], b[key], equal_nan=True), key
 
 
+@pytest.mark.parametrize("form", ["fp32", "fp64"])
+def test_loop_beyond_one_graph_equals_stepwise(S, form):
+    """More than 64 iterations: one captured iteration is launched once per
+    iteration after the starting state is set with copies, the outputs after
+    it, and the control block is copied down (no graph-held starting state or
+    control-block write).  With no convergence stop the loop runs all 70
+    iterations; the graph launches, their replay and the timed kernel-by-kernel
+    launches give the same outputs bit for bit."""
+    from conftest import small_prep
+    prep = dataclasses.replace(small_prep(n_ps=2500, n_ifg=12, seed=0x5EED1B70), gamma_max_iterations=70,
+                               gamma_change_convergence=0.0)
+    keys = ("K_ps", "C_ps", "coh_ps", "ph_grid", "ph_weight", "ph_res", "Nr", "trial_ix", "coh_ps_save")
+    with S.Stage2Session(prep, form=form) as s:
+        il, _ = s.run()
+        a = s.download()
+        s.set_timing(True)
+        s.run()
+        b = s.download()
+        kt = s.kernel_times()
+        s.set_timing(False)
+        s.run()
+        c = s.download()
+    assert il == 70 and a["i_loop"] == b["i_loop"] == c["i_loop"] == 70
+    assert kt["fill_grid"][1] == 70 and kt["outputs"][1] == 1
+    for key in keys:
+        assert np.array_equal(a[key], b[key], equal_nan=True), key
+        assert np.array_equal(a[key], c[key], equal_nan=True), key
+    assert a["gamma_change_save"] == b["gamma_change_save"] == c["gamma_change_save"]
+
+
 def test_graph_replays_equal_stepwise_launches(S, prep_small):
     """The default loop replays the captured iterations; with timing on the
     same kernels are launched one by one between event pairs, every iteration
