@@ -1608,6 +1608,7 @@ int run_loop_iterations(ps2_session *s, ps2_patch_out *state, ps2_iteration_fn f
         if ((rc = launch_outputs<R>(s, true))) break;
         PS2_HIP(hipMemcpyAsync(hc, s->ctl.p, sizeof(LoopCtl), hipMemcpyDeviceToHost, s->stream));
         PS2_HIP(hipStreamSynchronize(s->stream));
+        collect_timers(s);
         s->i_loop = it + 1;
         s->gamma_change_save = hc->gcs;
         s->nr.assign(hc->nr, hc->nr + CTL_BINS);

@@ -929,6 +929,25 @@ def test_loop_beyond_one_graph_equals_stepwise(S, form):
     assert a["gamma_change_save"] == b["gamma_change_save"] == c["gamma_change_save"]
 
 
+def test_iteration_snapshots_with_timing(S, prep_small):
+    """The loop with the state after every iteration (run_iterations), with
+    per-kernel timing on: one snapshot per iteration, the last equal to the
+    graph-launched loop's outputs, and the timers count every iteration's
+    launches and the outputs written after each."""
+    seen = []
+    with S.Stage2Session(prep_small, form="fp32") as s:
+        s.set_timing(True)
+        n = s.run_iterations(lambda it, st: seen.append((it, st["coh_ps"].copy(), st["K_ps"].copy())))
+        kt = s.kernel_times()
+        s.set_timing(False)
+        s.run()
+        full = s.download()
+    assert n == full["i_loop"] and [it for it, _, _ in seen] == list(range(1, n + 1))
+    assert kt["fill_grid"][1] == n and kt["clap_strip"][1] == n and kt["outputs"][1] == n
+    assert np.array_equal(seen[-1][1], full["coh_ps"])
+    assert np.array_equal(seen[-1][2], full["K_ps"], equal_nan=True)
+
+
 def test_graph_replays_equal_stepwise_launches(S, prep_small):
     """The default loop replays the captured iterations; with timing on the
     same kernels are launched one by one between event pairs, every iteration
