@@ -814,8 +814,10 @@ struct StripArgs {
     const int32_t * gate;
 };
 
+// two wavefronts per SIMD in the FP32 form (254 VGPRs); the FP64 form's
+// registers allow one
 template <bool M, typename R, int TEAMS, bool ALPHA_ONE>
-__global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clap_strip(KArg<M, StripArgs<R>> p) {
+__global__ __launch_bounds__(32 * TEAMS) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1, 8))) void k_clap_strip(KArg<M, StripArgs<R>> p) {
     if constexpr (M) {
         StripArgs<R> a;
         uint32_t bid_, nblk_;
