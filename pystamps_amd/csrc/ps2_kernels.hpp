@@ -2534,6 +2534,8 @@ __device__ __forceinline__ void refine_body(
     HistParams hp,
     RecheckIn rc,
     uint8_t *__restrict__ flags,
+    int32_t *__restrict__ recheck_list,
+    uint32_t *__restrict__ recheck_count,
     const int32_t *__restrict__ gate,
     const uint32_t bid_, const uint32_t nblk_) {
     if (loop_stopped(gate)) return;
@@ -2553,9 +2555,15 @@ __device__ __forceinline__ void refine_body(
         // rotation, angle rounding <= |k0 bp| ulp; sums in f64).  Rows whose
         // least-squares step has a residual phase within the f32 angle error
         // of the +-pi cut of arg() (the same bound plus the fitted atan2's
-        // 3.5e-7 rad) could take the other branch in f64.  k_post refines the
-        // flagged rows again in FP64.
-        flags[k] = redecide_flag(tix[k], coh, amax, cut, mids, hp, rc);
+        // 3.5e-7 rad) could take the other branch in f64.  Those rows are
+        // refined again in FP64: queued for k_redecide, or (no queue) flagged
+        // for k_post.
+        const uint8_t fl = redecide_flag(tix[k], coh, amax, cut, mids, hp, rc);
+        if (recheck_list) {
+            if (fl) recheck_list[atomicAdd(recheck_count, 1u)] = (int32_t)k;
+        } else {
+            flags[k] = fl;
+        }
     }
 }
 
@@ -2577,6 +2585,8 @@ struct RefineArgs {
     HistParams hp;
     RecheckIn rc;
     uint8_t * flags;
+    int32_t * recheck_list;     // non-null: flagged rows queued here (k_redecide)
+    uint32_t * recheck_count;
     const int32_t * gate;
 };
 
@@ -2586,9 +2596,49 @@ __global__ __launch_bounds__(256) void k_refine(KArg<M, RefineArgs<R>> p) {
         RefineArgs<R> a;
         uint32_t bid_, nblk_;
         if (!multi_locate(p, a, bid_, nblk_)) return;
-        refine_body<R>(a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.Kout, a.Cout, a.cohout, a.nopt, a.prm, a.mids, a.hp, a.rc, a.flags, a.gate, bid_, nblk_);
+        refine_body<R>(a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.Kout, a.Cout, a.cohout, a.nopt, a.prm, a.mids, a.hp, a.rc, a.flags, a.recheck_list, a.recheck_count, a.gate, bid_, nblk_);
     } else {
-        refine_body<R>(p.phnm, p.bp, p.filt, p.cell_of, p.tix, p.rng, p.offset, p.Kout, p.Cout, p.cohout, p.nopt, p.prm, p.mids, p.hp, p.rc, p.flags, p.gate, blockIdx.x, gridDim.x);
+        refine_body<R>(p.phnm, p.bp, p.filt, p.cell_of, p.tix, p.rng, p.offset, p.Kout, p.Cout, p.cohout, p.nopt, p.prm, p.mids, p.hp, p.rc, p.flags, p.recheck_list, p.recheck_count, p.gate, blockIdx.x, gridDim.x);
+    }
+}
+
+// FP64 re-decision (FP32 form) of the rows k_refine queued: one wavefront per
+// queued row, grid-stride over the queue; refine_row_f64_wave rewrites the
+// row's K, C and coh.  The rows are independent, so the queue's order (the
+// order of the refine's atomics) does not change any result; spreading them
+// over the whole GPU keeps a workgroup that owns many of them from setting
+// the step's length.
+struct RedecideArgs {
+    const cplx<float> *phnm;
+    Bp64 bp64;
+    const cplx<float> *filt;
+    const int32_t *cell_of;
+    const int32_t *tix;
+    double *Kout, *Cout, *cohout;
+    FitParams prm;
+    const int32_t *list;
+    const uint32_t *count;
+    const int32_t *gate;
+};
+
+__device__ __forceinline__ void redecide_body(const RedecideArgs &a, const uint32_t bid_, const uint32_t nblk_) {
+    if (loop_stopped(a.gate)) return;
+    const uint32_t n = *a.count;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t q = bid_; q < n; q += nblk_)
+        (void)refine_row_f64_wave<4>(a.list[q], a.phnm, a.bp64, a.filt, a.cell_of, a.tix, a.Kout, a.Cout, a.cohout,
+                                     a.prm, lane);
+}
+
+template <bool M>
+__global__ __launch_bounds__(64) void k_redecide(KArg<M, RedecideArgs> p) {
+    if constexpr (M) {
+        RedecideArgs a;
+        uint32_t bid_, nblk_;
+        if (!multi_locate(p, a, bid_, nblk_)) return;
+        redecide_body(a, bid_, nblk_);
+    } else {
+        redecide_body(p, blockIdx.x, gridDim.x);
     }
 }
 
@@ -3380,7 +3430,7 @@ __global__ __launch_bounds__(256) void k_fit_tail(KArg<M, FitTailArgs<R>> p) {
         __syncthreads();
     }
     refine_body<R>(a.phnm, a.bp, a.filt, a.cell_of, a.tix, a.rng, a.offset, a.K, a.Cc, a.coh, a.nopt, a.prm,
-                   a.mids, a.hp, a.rc, a.flags, a.gate, bid, nblk);
+                   a.mids, a.hp, a.rc, a.flags, nullptr, nullptr, a.gate, bid, nblk);
     __syncthreads();
     post_body<R>(a.phnm, a.filt, a.cell_of, a.tix, a.K, a.Cc, a.coh, a.prm, a.rc, a.flags, a.ro, 1, a.counters,
                  a.gate, bid, nblk);

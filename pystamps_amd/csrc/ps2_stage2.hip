@@ -277,7 +277,7 @@ constexpr int CLAP_TEAMS = 2;   // teams (interferograms) per CLAP wavefront
 // A recorded launch (below) carries its stage, so that the launches of
 // several patches can be merged stage by stage.
 enum Stage {
-    ST_PREPARE, ST_FILL, ST_CLAP_A, ST_CLAP_B, ST_SCAN, ST_RESCORE, ST_REFINE, ST_POST, ST_CONTROL,
+    ST_PREPARE, ST_FILL, ST_CLAP_A, ST_CLAP_B, ST_SCAN, ST_RESCORE, ST_REFINE, ST_REDECIDE, ST_POST, ST_CONTROL,
     ST_WEIGHTS, ST_OUTPUTS, ST_COUNT
 };
 
@@ -337,7 +337,8 @@ struct ps2_session {
     // at the end of every loop (and by the fit step)
     DBuf o_weight, o_patch, o_res, o_vec, o_tix, o_grid;
     DBuf d_order, cell_start, cell_of, tix, rescore_list, rescore_count, hist, mids;
-    DBuf flags;   // FP32 form: rows k_refine marks for the FP64 re-decision
+    DBuf flags;   // FP32 form: rows the fused small-patch fit marks for the FP64 re-decision
+    DBuf recheck_list;   // FP32 form: rows k_refine queues for k_redecide (count in rescore_count[1])
     DBuf K, Kprev, W, Wprev, W0, Cc, coh, cohprev, nopt, prand_hi, rms_part;
     int n_rms_blocks = 0;   // row groups of the loop's post step, one RMS partial each
     int n_cu = 256;         // compute units of the device
@@ -707,8 +708,10 @@ int alloc_state(ps2_session *s) {
     if ((rc = s->tix.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_list.alloc(sizeof(int32_t) * P))) return rc;
     if ((rc = s->rescore_count.alloc(sizeof(uint32_t) * 4))) return rc;
-    if (sizeof(R) == 4)
+    if (sizeof(R) == 4) {
         if ((rc = s->flags.alloc(std::max<int64_t>(P, 1)))) return rc;
+        if ((rc = s->recheck_list.alloc(sizeof(int32_t) * std::max<int64_t>(P, 1)))) return rc;
+    }
     if ((rc = s->hist.alloc(sizeof(uint32_t) * 128))) return rc;
     for (DBuf *b : {&s->K, &s->Kprev, &s->W, &s->Wprev, &s->Cc, &s->coh, &s->cohprev, &s->nopt})
         if ((rc = b->alloc(sizeof(double) * P))) return rc;
@@ -1132,20 +1135,36 @@ int launch_fit(ps2_session *s, bool in_loop = false) {
     // the coherence histogram (k_control then takes the loop decisions)
     {
         SlotTimer tm(s, SLOT_REFINE);
+        // the FP32 form queues its flagged rows for k_redecide (the count in
+        // rescore_count[1], the loop's re-decision statistic)
         const RefineArgs<R> a{s->phnm.as<cplx<R>>(), s->bp.as<R>(), s->filt.as<cplx<R>>(),
                               s->cell_of.as<int32_t>(), s->tix.as<int32_t>(), s->rng.as<R>(),
                               s->offset.as<cplx<R>>(), s->K.as<double>(), s->Cc.as<double>(),
                               s->coh.as<double>(), s->nopt.as<double>(), fp, ro.mids, ro.hp, rc,
-                              s->flags.as<uint8_t>(), s->gate};
+                              s->flags.as<uint8_t>(), rc.enabled ? s->recheck_list.as<int32_t>() : nullptr,
+                              s->rescore_count.as<uint32_t>() + 1, s->gate};
         kgo<M>(s, k_refine<M, R>, dim3(nb), dim3(256), a, ST_REFINE);
         PS2_LAUNCHED();
     }
     if (!in_loop && !rc.enabled) return PS2_OK;
     SlotTimer tm(s, SLOT_POST);
-    // grid-stride over the 256-row groups: 2 workgroups per CU at most
+    if (rc.enabled) {
+        // the queued rows, one wavefront each over the whole GPU
+        const RedecideArgs da{reinterpret_cast<const cplx<float> *>(s->phnm.p), rc.bp64,
+                              reinterpret_cast<const cplx<float> *>(s->filt.p), s->cell_of.as<int32_t>(),
+                              s->tix.as<int32_t>(), s->K.as<double>(), s->Cc.as<double>(), s->coh.as<double>(),
+                              fp, s->recheck_list.as<int32_t>(), s->rescore_count.as<uint32_t>() + 1, s->gate};
+        kgo<M>(s, k_redecide<M>, dim3((unsigned)std::min<int64_t>(s->P, 4096)), dim3(64), da, ST_REDECIDE);
+        PS2_LAUNCHED();
+        if (!in_loop) return PS2_OK;
+    }
+    // grid-stride over the 256-row groups: 2 workgroups per CU at most; the
+    // re-decided rows are already rewritten, so the post step only reads coh
+    RecheckIn prc = rc;
+    prc.enabled = 0;
     const PostArgs<R> pa{s->phnm.as<cplx<R>>(), s->filt.as<cplx<R>>(), s->cell_of.as<int32_t>(),
                          s->tix.as<int32_t>(), s->K.as<double>(), s->Cc.as<double>(), s->coh.as<double>(), fp,
-                         rc, s->flags.as<uint8_t>(), ro, in_loop ? 1 : 0, s->rescore_count.as<uint32_t>(),
+                         prc, s->flags.as<uint8_t>(), ro, in_loop ? 1 : 0, s->rescore_count.as<uint32_t>(),
                          s->gate};
     kgo<M>(s, k_post<M, R>, dim3(std::min<unsigned>(nb, 2 * (unsigned)s->n_cu)), dim3(256), pa, ST_POST);
     PS2_LAUNCHED();
