@@ -322,9 +322,10 @@ int ps2_hip_session_export_device(ps2_session *s, ps2_device_outputs *out);
  * the same phase breakdown per iteration, native_stage2.rs:263-305).
  * Kernel slots, in order: fill_grid, clap_strip, clap_combine, scan,
  * rescore, refine, post, weights, outputs.  ms[k] is the summed HIP-event
- * time of slot k over launches[k] launches.  post is the kernel after the
- * refine: the FP64 re-decision of the rows the refine marks (FP32 form) and
- * the gamma-change RMS partials and coherence histogram (the loop control,
+ * time of slot k over launches[k] launches.  post is the step after the
+ * refine: the FP64 re-decision of the rows the refine marks (FP32 form, its
+ * own kernel) and the gamma-change RMS partials and coherence histogram,
+ * timed together as one launch (the loop control,
  * one workgroup, is not timed); weights is the weighting update (a patch of
  * at most 65,536 rows computes P-square weights in the next iteration's fill,
  * so that slot stays empty for it); outputs is the once-per-loop kernel that
