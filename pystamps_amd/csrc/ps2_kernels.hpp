@@ -3264,10 +3264,21 @@ __device__ __forceinline__ void post_body(
     __syncthreads();   // the bins are zero before any wavefront counts into them
     uint32_t n_rechecked = 0;
     const int64_t n_groups = (prm.P + GR - 1) / GR;
+    // the next group's coh and coh_prev are loaded while this one is processed
+    auto fetch = [&](int64_t gg, double &c, double &cp) {
+        const int64_t kk = gg * GR + threadIdx.x;
+        const bool r = (int)threadIdx.x < GR && kk < prm.P;
+        c = r ? cohout[kk] : 0.0;
+        cp = r && loop ? ro.coh_prev[kk] : 0.0;
+    };
+    double coh_next = 0.0, prev_next = 0.0;
+    if ((int64_t)bid_ < n_groups) fetch(bid_, coh_next, prev_next);
     for (int64_t g = bid_; g < n_groups; g += nblk_) {
         const int64_t k = g * GR + threadIdx.x;
         const bool row = (int)threadIdx.x < GR && k < prm.P;
-        double coh = row ? cohout[k] : 0.0;
+        double coh = coh_next;
+        const double coh_prev = prev_next;
+        if (g + nblk_ < n_groups) fetch(g + nblk_, coh_next, prev_next);
         if constexpr (sizeof(R) == 4) {
             if (rc.enabled) {
                 if (threadIdx.x == 0) n_flag = 0;
@@ -3292,19 +3303,30 @@ __device__ __forceinline__ void post_body(
         if (loop) {
             double d2 = 0.0;
             if (row) {
-                const double d = coh - ro.coh_prev[k];
+                const double d = coh - coh_prev;
                 d2 = d * d;
                 ro.coh_prev[k] = coh;
                 const int bin = hist_bin(coh, ro.mids, ro.hp);
                 if (bin >= 0) atomicAdd(&lh[bin], 1u);
             }
-            red[threadIdx.x] = d2;
-            __syncthreads();
-            for (int o = GR / 2; o > 0; o >>= 1) {
-                if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            // fixed fold tree over the group's rows, s[t] += s[t + o] for
+            // o = GR/2 .. 1: the steps across wavefronts through LDS, the
+            // steps inside wavefront 0 (o <= 32) by shuffles (the same pairs
+            // and order)
+            double v = d2;
+            if constexpr (GR == 256) {
+                red[threadIdx.x] = d2;
                 __syncthreads();
+                if (threadIdx.x < 128) red[threadIdx.x] += red[threadIdx.x + 128];
+                __syncthreads();
+                if (threadIdx.x < 64) v = red[threadIdx.x] + red[threadIdx.x + 64];
+                __syncthreads();   // red is reused by the next group
             }
-            if (threadIdx.x == 0) ro.part[g] = red[0];
+            if (threadIdx.x < 64) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+                if (threadIdx.x == 0) ro.part[g] = v;
+            }
         }
     }
     if (threadIdx.x == 0 && n_rechecked > 0) atomicAdd(&counters[1], n_rechecked);
