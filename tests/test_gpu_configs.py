@@ -151,6 +151,27 @@ def test_c5_one_iteration_vs_ref32_ref64_and_full_loop(S, ora):
     _invariants("C5 fp32", g32, prep.n_ps)
 
 
+def test_large_patch_snr_loop_vs_ref64(S, ora):
+    """A patch above 65,536 rows with SNR weighting runs the large-patch
+    kernels (scan, re-score, refine with its re-decision queue, k_redecide,
+    k_post, the SNR weights): the FP64 form equals ref64 (trials exact,
+    coherence to 1e-9); the FP32 form is judged against ref64 by the FP64
+    loops on one-ulp-perturbed inputs."""
+    from conftest import small_prep
+    base = small_prep(n_ps=80_000, n_ifg=20, extent=12_000.0, seed=0x5EED1C01)
+    prep = dataclasses.replace(base, filter_weighting="SNR")
+    assert prep.n_ps > 65_536 and not prep.row_invariant_bperp
+    r64 = ora.run(prep, mode=ora.REF64)
+    g64 = S.run_stage2_hip(prep, form="fp64")
+    assert g64["i_loop"] == r64["i_loop"]
+    assert np.array_equal(g64["trial_ix"], r64["trial_ix"])
+    np.testing.assert_allclose(g64["coh_ps"], r64["coh_ps"], rtol=0, atol=1e-9)
+    g32 = S.run_stage2_hip(prep, form="fp32")
+    rps = [ora.run(perturb_one_ulp(prep, seed), mode=ora.REF64) for seed in (5, 6)]
+    floor_check("80k SNR loop fp32 vs ref64", g32, r64, rps)
+    print("re-scored", g32["n_rescored"], "re-decided", g32["n_rechecked"])
+
+
 def test_c5_scaled_full_loop_vs_ref64(S, ora):
     prep = _prep(5, n_ps=100_000)
     assert (prep.n_ifg, len(prep.trial_values), prep.n_i) == (200, 83, 316)
